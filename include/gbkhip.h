@@ -1,0 +1,190 @@
+/*
+ * gbkhip.h -- C ABI of libgbkhip, the MI355X (gfx950) driver for gbkfit's
+ * model-evaluation hot path.
+ *
+ * Every function returns GBKHIP_OK (0) or an error code; the message of the
+ * last error of the calling thread is available from gbkhip_last_error().
+ * Array arguments are DEVICE pointers unless stated otherwise; absent optional
+ * arrays are NULL (the reference passes 0, drivers/host/_backend.py:18-22).
+ * Sizes are (x, y, z) -- the reversed C-order shape, as gbkfit's native
+ * marshalling passes them (src/gbkfit/driver/drivers/_detail/native.py:70-131)
+ * -- and cubes are C-order (z, y, x) with x fastest
+ * (core/utilities/indexutils.hpp:96-101).
+ *
+ * Work is enqueued on the context's stream (gbkhip_ctx_set_stream); calls
+ * are asynchronous with respect to the host unless stated otherwise.
+ *
+ * Reference paths are relative to src/gbkfit/driver/native/ (core/ =
+ * gbkfit/include/gbkfit/, host/ = gbkfit_host/).
+ */
+#ifndef GBKHIP_H
+#define GBKHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GBKHIP_OK = 0,
+    GBKHIP_ERR_ARG = 1,          /* invalid argument (shape, null, range) */
+    GBKHIP_ERR_HIP = 2,          /* HIP runtime failure */
+    GBKHIP_ERR_ALLOC = 3,        /* allocation failure */
+    GBKHIP_ERR_UNSUPPORTED = 4,  /* valid but unsupported request */
+    GBKHIP_ERR_INTERNAL = 5
+};
+
+typedef struct gbkhip_ctx gbkhip_ctx;
+
+/* ---- library / context ------------------------------------------------- */
+const char* gbkhip_last_error(void);
+int gbkhip_version(void);
+/* One context per device; owns FFT plans, cached kernel spectra and scratch
+ * (replaces the FFT plan/data caches of host/include/gbkfit/host/fft.hpp:
+ * 20-69, which are per FFT-backend object in the reference). */
+int gbkhip_ctx_create(int device, gbkhip_ctx** out);
+int gbkhip_ctx_destroy(gbkhip_ctx* ctx);
+/* `stream` is a hipStream_t (NULL = the null stream). */
+int gbkhip_ctx_set_stream(gbkhip_ctx* ctx, void* stream);
+int gbkhip_ctx_synchronize(gbkhip_ctx* ctx);
+
+/* ---- FFT backend: FFTf32 (host/src/bindings.cpp:35-40,
+ *      host/include/gbkfit/host/fft.hpp:71-125) ---------------------------- */
+/* Unnormalised forward r2c: (nz, ny, nx) real -> (nz, ny, nx/2+1) complex
+ * (interleaved float pairs). */
+int gbkhip_fft_r2c(gbkhip_ctx* ctx, const int size[3], const float* data_r,
+                   float* data_c);
+/* Unnormalised inverse c2r; data_c is not modified. */
+int gbkhip_fft_c2r(gbkhip_ctx* ctx, const int size[3], const float* data_c,
+                   float* data_r);
+/* data1_r <- irfft(rfft(data1_r) * data2_c) / N, using data1_c as the
+ * spectrum buffer (fft.hpp:86-96,199-207). */
+int gbkhip_fft_convolve(gbkhip_ctx* ctx, const int size[3], float* data1_r,
+                        float* data1_c, const float* data2_c);
+/* As above with the spectra of data2_r and the work spectrum of data1_r
+ * cached in the context (fft.hpp:98-125).  The reference keys the cache by
+ * (size, pointer) only; here `data2_generation` is part of the key, so a
+ * caller that rewrites data2_r in place bumps it to invalidate. */
+int gbkhip_fft_convolve_cached(gbkhip_ctx* ctx, const int size[3],
+                               float* data1_r, const float* data2_r,
+                               uint64_t data2_generation);
+/* Drop every cached spectrum and plan. */
+int gbkhip_fft_clear_cache(gbkhip_ctx* ctx);
+
+/* ---- DModel backend: DModelf32 (host/src/dmodels.cpp:7-59) -------------- */
+/* Block mean over scale[0]*scale[1]*scale[2] source voxels starting at
+ * offset (core/dmodel/dmodels.hpp:7-46). */
+int gbkhip_dcube_downscale(gbkhip_ctx* ctx, const int scale[3],
+                           const int offset[3], const int src_size[3],
+                           const int dst_size[3], const float* src,
+                           float* dst);
+/* m = |d| > cutoff; d = NaN where masked if apply (dmodels.hpp:48-74).
+ * Covers the whole x*y*z cube (the CUDA launcher's x*y bug is not kept). */
+int gbkhip_dcube_mask(gbkhip_ctx* ctx, float cutoff, int apply,
+                      const int size[3], float* dcube_d, float* dcube_m,
+                      float* dcube_w);
+/* Moment maps (dmodels.hpp:76-228).  `orders` is a device array of norders
+ * sorted orders in [0, 7]; mmaps_d (norders, ny, nx), mmaps_m (ny, nx),
+ * mmaps_w (norders, ny, nx; only plane 0 written, NULL allowed when dcube_w
+ * is NULL).  Pointer order (d, m, w) is the semantic one: the reference's
+ * Python passes (d, w, m) into a (d, m, w) binding that swaps them back
+ * (_detail/native.py:122-131, host/include/gbkfit/host/kernels.hpp:124-131). */
+int gbkhip_mmaps_moments(gbkhip_ctx* ctx, const int size[3],
+                         const float step[3], const float zero[3],
+                         const float* dcube_d, const float* dcube_w,
+                         float cutoff, int norders, const int* orders,
+                         float* mmaps_d, float* mmaps_m, float* mmaps_w);
+
+/* ---- GModel backend: GModelf32 (host/src/gmodels.cpp, 83/88 positional
+ *      arguments, host/include/gbkfit/host/gmodels.hpp:25-132) ------------ */
+typedef struct {
+    int n;                   /* traits of this kind (0 when absent) */
+    const int* uids;         /* device; NULL when absent */
+    const float* cvalues;    /* device, packed constants */
+    const int* ccounts;      /* device, constants per trait */
+    const float* pvalues;    /* device, packed parameters */
+    const int* pcounts;      /* device, parameters per trait */
+} gbkhip_traits;
+
+typedef struct {
+    int loose, tilted;
+    int nrnodes;             /* number of sub-ring nodes */
+    const float* rnodes;     /* device (nrnodes) */
+    const float* vsys;       /* device, 1 or nrnodes values, may be NULL */
+    const float* xpos;
+    const float* ypos;
+    const float* posa;       /* degrees */
+    const float* incl;       /* degrees */
+    gbkhip_traits rp, rh, vp, vh, dp, dh, zp, sp, wp;
+    const float* opacity;    /* device spatial cube or NULL */
+} gbkhip_disk;
+
+typedef struct {
+    int spat_size[3];
+    float spat_step[3];
+    float spat_zero[3];
+    int spec_size;
+    float spec_step;
+    float spec_zero;
+} gbkhip_grid;
+
+typedef struct {
+    float* image;            /* (ny, nx) */
+    float* scube;            /* (spec_size, ny, nx) */
+    float* wdata;            /* spatial cubes (nz, ny, nx) */
+    float* wdata_cmp;
+    float* rdata;
+    float* rdata_cmp;
+    float* ordata;
+    float* ordata_cmp;
+    float* vdata_cmp;
+    float* ddata_cmp;
+} gbkhip_outputs;
+
+/* Smooth-disk evaluation over every spatial voxel; outputs are accumulated
+ * (+=) like the reference (core/gmodel/gmodels.hpp:645-958).  `disk`, `grid`
+ * and `out` are host structs holding device pointers. */
+int gbkhip_smdisk_evaluate(gbkhip_ctx* ctx, const gbkhip_disk* disk,
+                           const gbkhip_grid* grid, const gbkhip_outputs* out);
+/* Monte Carlo cloud disk (gmodels.hpp:325-643).  Clouds draw from a
+ * Philox4x32-10 stream keyed by (seed, cloud id), so results do not depend
+ * on launch shape; the spectral cube is accumulated in int64 fixed point and
+ * converted once, so reruns are bit-identical.  ncloudscsum and hasordint
+ * are device arrays (ncloudscsum_len entries / rp.n entries). */
+int gbkhip_mcdisk_evaluate(gbkhip_ctx* ctx, float cflux, int nclouds,
+                           const int* ncloudscsum, int ncloudscsum_len,
+                           const unsigned char* hasordint,
+                           const gbkhip_disk* disk, const gbkhip_grid* grid,
+                           const gbkhip_outputs* out, uint64_t seed);
+/* Spectral weight cube from a spatial weight cube (gmodels.hpp:52-82). */
+int gbkhip_wcube_evaluate(gbkhip_ctx* ctx, const int spat_size[3],
+                          int spec_size, const float* spat_data,
+                          float* spec_data);
+/* Direct zero-padded convolution (gmodels.hpp:9-50). */
+int gbkhip_convolve(gbkhip_ctx* ctx, const int size1[3], const int size2[3],
+                    const float* data1, const float* data2, float* result);
+
+/* ---- Objective backend: Objectivef32 (host/src/objective.cpp) ----------- */
+/* res = (mdl - obs) [/ obs_e] [* mdl_w] [* obs_m] [* mdl_m] * weight
+ * (core/objective/objective.hpp:7-33). */
+int gbkhip_residual(gbkhip_ctx* ctx, const float* obs_d, const float* obs_e,
+                    const float* obs_m, const float* mdl_d,
+                    const float* mdl_w, const float* mdl_m, long long size,
+                    float weight, float* res);
+/* sum r^2 (squared) or |r| into sum_out[0] (device float); accumulated in
+ * FP64 with a fixed reduction order, then rounded (host kernels.hpp:502-514
+ * uses an FP32 OpenMP reduction). */
+int gbkhip_residual_sum(gbkhip_ctx* ctx, const float* residual,
+                        long long size, int squared, float* sum_out);
+/* counts[0..2] += (data1 only, data2 only, both) beyond epsilon
+ * (host kernels.hpp:425-482). */
+int gbkhip_count_pixels(gbkhip_ctx* ctx, const float* data1,
+                        const float* data2, long long size, float epsilon,
+                        int* counts);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GBKHIP_H */

@@ -1,0 +1,129 @@
+/*
+ * oracle.h -- FP64 CPU restatement of gbkfit's model-evaluation hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load liboracle.so, and only as the checker.
+ * The product library (gbkfit_amd) never links or calls anything here.
+ *
+ * Layouts follow the reference: cubes are C-order (z, y, x) with x fastest,
+ * index = x + y*nx + z*nx*ny (core/utilities/indexutils.hpp:96-101).
+ *
+ * Ambiguity bound ("amb").  FP32 decisions of the reference path -- the disk
+ * edge test (core/gmodel/gmodels.hpp:197-215), the rint() channel window
+ * (gmodels.hpp:100-105) and the rint() voxel of a cloud (gmodels.hpp:499-501)
+ * -- can legitimately go either way when the FP64 value lies within a few
+ * FP32 ulps of the threshold.  When an `amb` array is supplied the oracle adds,
+ * voxel by voxel, |contribution| of every deposit whose inclusion or position
+ * hinges on such a near-tie (relative margin AMB_EPS_R on radii, AMB_EPS_CH
+ * channels / voxels on rint arguments).  A correct FP32 implementation then
+ * satisfies |out - oracle| <= tol * max|oracle| + amb element-wise.
+ */
+#ifndef GBK_ORACLE_H
+#define GBK_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_AMB_EPS_R  1e-5   /* relative margin on radius thresholds */
+#define OR_AMB_EPS_CH 2e-3   /* margin on rint() arguments (grid units) */
+
+typedef struct {
+    int n;                  /* number of traits of this kind */
+    const int* uids;        /* NULL when the kind is absent */
+    const double* cvalues;
+    const int* ccounts;
+    const double* pvalues;
+    const int* pcounts;
+} or_traits;
+
+typedef struct {
+    int loose, tilted, nrnodes;
+    const double* rnodes;   /* sub-ring nodes (nrnodes) */
+    const double* vsys;     /* may be NULL */
+    const double* xpos;
+    const double* ypos;
+    const double* posa;     /* degrees */
+    const double* incl;     /* degrees */
+    or_traits rp, rh, vp, vh, dp, dh, zp, sp, wp;
+    const double* opacity;  /* spatial cube or NULL */
+} or_disk;
+
+typedef struct {
+    int spat_size[3];
+    double spat_step[3];
+    double spat_zero[3];
+    int spec_size;
+    double spec_step;
+    double spec_zero;
+} or_grid;
+
+typedef struct {
+    double* image;      /* (ny, nx) */
+    double* scube;      /* (spec_size, ny, nx) */
+    double* wdata;      /* spatial cubes (nz, ny, nx) */
+    double* wdata_cmp;
+    double* rdata;
+    double* rdata_cmp;
+    double* ordata;
+    double* ordata_cmp;
+    double* vdata_cmp;
+    double* ddata_cmp;
+    double* amb;        /* optional ambiguity bound on scube */
+} or_out;
+
+/* Per-call work counters (algorithmic work the roofline is priced on). */
+typedef struct {
+    long long voxels_on_disk;   /* voxels / clouds that deposited a line */
+    long long channel_deposits; /* sum over them of the channel-window size */
+} or_stats;
+
+/* smdisk: every spatial voxel (core/gmodel/gmodels.hpp:645-958, host loop
+ * gbkfit_host/include/gbkfit/host/kernels.hpp:308-423). */
+void or_smdisk(const or_disk* d, const or_grid* g, const or_out* o,
+               or_stats* stats);
+
+/* mcdisk with the Philox4x32-10 cloud stream keyed by (seed, cloud id)
+ * (core/gmodel/gmodels.hpp:325-643). */
+void or_mcdisk(double cflux, int nclouds, const int* ncloudscsum,
+               int ncloudscsum_len, const unsigned char* hasordint,
+               const or_disk* d, const or_grid* g, const or_out* o,
+               unsigned long long seed, or_stats* stats);
+
+/* Philox4x32-10 block for (counter, key): Salmon et al., SC'11. */
+void or_philox4x32(const unsigned int ctr[4], const unsigned int key[2],
+                   unsigned int out[4]);
+/* i-th uniform of cloud `cid` in (0,1): ((word >> 9) + 0.5) * 2^-23. */
+double or_cloud_uniform(unsigned long long seed, unsigned int cid,
+                        unsigned int i);
+
+/* Data-model and objective stages (core/dmodel/dmodels.hpp,
+ * core/objective/objective.hpp). */
+void or_downscale(const int scale[3], const int offset[3],
+                  const int src_size[3], const int dst_size[3],
+                  const double* src, double* dst);
+void or_mask(double cutoff, int apply, const int size[3],
+             double* dcube, double* mcube, double* wcube);
+/* Pointer order (mmaps_d, mmaps_m, mmaps_w) as in the core template. */
+void or_moments(const int size[3], const double step[3], const double zero[3],
+                const double* dcube, const double* wcube, double cutoff,
+                int norders, const int* orders,
+                double* mmaps_d, double* mmaps_m, double* mmaps_w);
+void or_residual(const double* obs_d, const double* obs_e,
+                 const double* obs_m, const double* mdl_d,
+                 const double* mdl_w, const double* mdl_m,
+                 long long size, double weight, double* res);
+/* Neumaier-compensated sum of r^2 (squared) or |r|. */
+double or_residual_sum(const double* r, long long size, int squared);
+
+/* Circular 3-D convolution by brute force (tiny cubes only): the closed form
+ * the FFT convolution (gbkfit_host/include/gbkfit/host/fft.hpp:199-207)
+ * computes.  out[x] = sum_j a[j] * k[(x - j) mod n]. */
+void or_circular_convolve(const int size[3], const double* a, const double* k,
+                          double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

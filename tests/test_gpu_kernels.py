@@ -1,0 +1,197 @@
+"""GPU parity of each native entry point against the FP64 oracle, called
+through the hip driver's backends (the reference's Python signatures) and the
+C ABI underneath.  Tolerances are stated per test; the cube comparisons use
+|gpu - oracle| <= tol * max|oracle| + amb, where amb is the oracle's bound on
+contributions decided by FP32 near-ties (see oracle/oracle.h)."""
+import numpy as np
+import pytest
+
+import cases
+import hiputil as H
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+@pytest.fixture(scope='module')
+def drv():
+    return H.driver()
+
+
+@pytest.mark.parametrize('case', cases.SMDISK_CASES, ids=lambda c: c['name'])
+def test_smdisk_cases(drv, case):
+    spec = cases.build_spec(case).rounded_f32()
+    grid = oracle.grid_f32(cases.build_grid(case))
+    ref, _ = oracle.smdisk(spec, grid, want=('scube', 'amb'))
+    got = H.smdisk(drv, spec, grid)
+    nbad, err = H.within(got['scube'], ref['scube'], TOL, ref['amb'])
+    print(case['name'], 'normwise', err, 'violations', nbad)
+    assert nbad == 0
+
+
+@pytest.mark.parametrize('name', ['thin_basic', 'thick_sech2'])
+def test_smdisk_side_outputs(drv, name):
+    case = [c for c in cases.SMDISK_CASES if c['name'] == name][0]
+    spec = cases.build_spec(case).rounded_f32()
+    grid = oracle.grid_f32(cases.build_grid(case))
+    want = ('image', 'rdata', 'vdata_cmp', 'ddata_cmp')
+    ref, _ = oracle.smdisk(spec, grid, want=want)
+    got = H.smdisk(drv, spec, grid, want=want,
+                   fill=dict(vdata_cmp=np.nan, ddata_cmp=np.nan))
+    for k in ('image', 'rdata'):
+        assert H.normwise(got[k], ref[k]) < TOL, k
+    # velocity / dispersion maps: compare where the oracle wrote a value
+    for k in ('vdata_cmp', 'ddata_cmp'):
+        m = ref[k] != 0
+        assert H.normwise(got[k][m], ref[k][m]) < TOL, k
+
+
+def _c1_like(size=(32, 32, 32), thin=True, psf=1.0617, lsf=8.493):
+    geom = oracle.DCubeGeom(size, (1, 1, 10), scale=(2, 2, 1),
+                            psf_sigma=psf, lsf_sigma=lsf)
+    T = oracle.Trait
+    rn = np.arange(0, 14.0, 1.5)
+    traits = dict(rp=[T(2, params=(1.0, 4.0))],
+                  vp=[T(2, params=(3.0, 200.0))],
+                  dp=[T(1, params=(20.0,))])
+    if not thin:
+        traits['rh'] = [T(7, consts=(0, 0), params=(1.0,))]
+        traits['vh'] = [T(1, consts=(0,))]
+        traits['dh'] = [T(1, consts=(0,))]
+    spec = oracle.DiskSpec(rn, traits, posa=30, incl=45).rounded_f32()
+    return geom, spec
+
+
+def test_fft_r2c_c2r(drv):
+    be = drv.backends().fft(np.float32)
+    rng = np.random.default_rng(3)
+    for shape in [(4, 6, 8), (5, 7, 9), (1, 1, 16), (3, 1, 34), (2, 17, 64),
+                  (8, 12, 40), (1, 4, 1)]:
+        a = rng.standard_normal(shape).astype(np.float32)
+        d = H.dev(drv, a)
+        c = drv.mem_alloc_d(be.fft_complex_shape(list(shape)), np.complex64)
+        be.fft_r2c(d, c)
+        ref = np.fft.rfftn(a.astype(np.float64))
+        got = c.cpu().numpy()
+        assert H.normwise(got, ref) < 1e-5, shape
+        back = drv.mem_alloc_d(shape, np.float32)
+        be.fft_c2r(c, back)
+        n = np.prod(shape)
+        assert H.normwise(back.cpu().numpy() / n, a) < 1e-5, shape
+        # c2r leaves its input untouched
+        assert np.array_equal(c.cpu().numpy(), got)
+
+
+def test_fft_convolve_matches_closed_form(drv):
+    be = drv.backends().fft(np.float32)
+    rng = np.random.default_rng(5)
+    for shape in [(8, 8, 8), (16, 20, 30), (6, 1, 10), (1, 1, 12),
+                  (32, 32, 34)]:
+        a = rng.standard_normal(shape).astype(np.float32)
+        k = rng.standard_normal(shape).astype(np.float32)
+        da, dk = H.dev(drv, a), H.dev(drv, k)
+        be.fft_convolve_cached(da, dk)
+        ref = oracle.fft_convolve(a, k)
+        assert H.normwise(da.cpu().numpy(), ref) < 1e-5, shape
+        # the explicit-spectrum form
+        da2 = H.dev(drv, a)
+        kc = drv.mem_alloc_d(be.fft_complex_shape(list(shape)), np.complex64)
+        wc = drv.mem_alloc_d(be.fft_complex_shape(list(shape)), np.complex64)
+        be.fft_r2c(dk, kc)
+        be.fft_convolve(da2, wc, kc)
+        assert H.normwise(da2.cpu().numpy(), ref) < 1e-5, shape
+
+
+def test_fft_convolve_cached_invalidation(drv):
+    be = drv.backends().fft(np.float32)
+    rng = np.random.default_rng(9)
+    shape = (8, 10, 12)
+    a = rng.standard_normal(shape).astype(np.float32)
+    k1 = rng.standard_normal(shape).astype(np.float32)
+    k2 = rng.standard_normal(shape).astype(np.float32)
+    dk = H.dev(drv, k1)
+    da = H.dev(drv, a)
+    be.fft_convolve_cached(da, dk)
+    dk.copy_(H.dev(drv, k2))          # rewritten in place
+    da = H.dev(drv, a)
+    be.fft_convolve_cached(da, dk)
+    assert H.normwise(da.cpu().numpy(), oracle.fft_convolve(a, k2)) < 1e-5
+
+
+def test_fft_brute_force_tiny(drv):
+    be = drv.backends().fft(np.float32)
+    rng = np.random.default_rng(11)
+    shape = (3, 5, 6)
+    a = rng.standard_normal(shape).astype(np.float32)
+    k = rng.standard_normal(shape).astype(np.float32)
+    ref = oracle.circular_convolve(a, k)
+    da, dk = H.dev(drv, a), H.dev(drv, k)
+    be.fft_convolve_cached(da, dk)
+    assert H.normwise(da.cpu().numpy(), ref) < 1e-5
+
+
+def test_downscale_mask(drv):
+    be = drv.backends().dmodel(np.float32)
+    rng = np.random.default_rng(1)
+    src = rng.standard_normal((20, 30, 40)).astype(np.float32)
+    dsrc = H.dev(drv, src)
+    dst = drv.mem_alloc_d((8, 12, 17), np.float32)
+    be.dcube_downscale((2, 2, 2), (3, 2, 1), dsrc, dst)
+    ref = oracle.downscale((2, 2, 2), (3, 2, 1), src, (17, 12, 8))
+    assert H.normwise(dst.cpu().numpy(), ref) < 1e-6
+    m = drv.mem_alloc_d(src.shape, np.float32)
+    be.dcube_mask(0.3, True, m, dsrc, None)
+    rd, rm = oracle.mask(0.3, True, src)
+    got = dsrc.cpu().numpy()
+    assert np.array_equal(np.isnan(got), np.isnan(rd))
+    assert np.array_equal(got[~np.isnan(got)], src[~np.isnan(rd)])
+    assert np.array_equal(m.cpu().numpy(), rm)
+
+
+def test_moments(drv):
+    fx = np.load(cases.__file__.replace('cases.py', 'golden_ref.npz'))
+    cube, wcube = fx['moments/cube'], fx['moments/wcube']
+    be = drv.backends().dmodel(np.float32)
+    orders = np.array([0, 1, 2, 3, 4], np.int32)
+    nz, ny, nx = cube.shape
+    md = drv.mem_alloc_d((5, ny, nx), np.float32)
+    mm = drv.mem_alloc_d((ny, nx), np.float32)
+    mw = drv.mem_alloc_d((5, ny, nx), np.float32)
+    drv.mem_fill(md, np.nan)
+    drv.mem_fill(mw, 1)
+    be.mmaps_moments((nx, ny, nz), (1, 1, 10), (0, 0, -120), H.dev(drv, cube),
+                     H.dev(drv, wcube), 1e-6, H.dev(drv, orders, np.int32),
+                     md, mw, mm)
+    rd, rm, rw = oracle.moments(cube, (1, 1, 10), (0, 0, -120), orders, 1e-6,
+                                wcube)
+    got_d = md.cpu().numpy()
+    assert np.array_equal(np.isnan(got_d), np.isnan(rd))
+    for i in range(5):
+        assert H.normwise(got_d[i], rd[i]) < 1e-5, i
+    assert np.array_equal(mm.cpu().numpy(), rm)
+    assert H.normwise(mw.cpu().numpy()[0], rw[0]) < 1e-5
+
+
+def test_residual_and_sum(drv):
+    be = drv.backends().objective(np.float32)
+    rng = np.random.default_rng(2)
+    n = 100003
+    arrs = [rng.standard_normal(n).astype(np.float32) for _ in range(2)]
+    oe = rng.uniform(0.5, 2, n).astype(np.float32)
+    om = (rng.uniform(0, 1, n) > 0.2).astype(np.float32)
+    mw = rng.uniform(0, 1, n).astype(np.float32)
+    mm = (rng.uniform(0, 1, n) > 0.1).astype(np.float32)
+    res = drv.mem_alloc_d(n, np.float32)
+    be.residual(H.dev(drv, arrs[0]), H.dev(drv, oe), H.dev(drv, om),
+                H.dev(drv, arrs[1]), H.dev(drv, mw), H.dev(drv, mm), 0.7, res)
+    ref = oracle.residual(arrs[0], oe, om, arrs[1], mw, mm, 0.7)
+    assert H.normwise(res.cpu().numpy(), ref) < 1e-6
+    s = drv.mem_alloc_d(1, np.float32)
+    be.residual_sum(res, True, s)
+    rs = oracle.residual_sum(res.cpu().numpy().astype(np.float64), True)
+    assert abs(float(s.cpu()[0]) - rs) / rs < 1e-6
+    be.residual_sum(res, False, s)
+    ra = oracle.residual_sum(res.cpu().numpy().astype(np.float64), False)
+    assert abs(float(s.cpu()[0]) - ra) / ra < 1e-6

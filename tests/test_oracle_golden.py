@@ -1,0 +1,254 @@
+"""Pins the FP64 oracle (oracle/) before anything is compared with it:
+  * against golden vectors written by the reference's own core (compiled
+    from its headers into oracle/_ref by oracle/Makefile; fixtures in
+    tests/golden/golden_ref.npz, generator tests/golden/make_golden.py);
+  * against published known-answer vectors (Philox4x32-10);
+  * against closed forms and invariants (FFT convolution vs brute force,
+    flux conservation, PSF/LSF properties the reference's own tests check);
+  * and by planting mistakes in the oracle's inputs, which these checks must
+    catch."""
+import os
+
+import numpy as np
+import pytest
+
+import cases
+import oracle
+
+FX = np.load(os.path.join(os.path.dirname(cases.__file__), 'golden_ref.npz'))
+
+
+def normwise(a, ref):
+    ref = np.asarray(ref, np.float64)
+    m = np.isfinite(ref)
+    s = np.max(np.abs(ref[m]))
+    return float(np.max(np.abs(np.asarray(a, np.float64)[m] - ref[m])) / s)
+
+
+@pytest.mark.parametrize('case', cases.SMDISK_CASES, ids=lambda c: c['name'])
+def test_smdisk_vs_reference_core_f32(case):
+    """Oracle (FP64 arithmetic on FP32-rounded inputs) vs the reference core
+    instantiated with T=float: agreement to FP32 rounding, except where the
+    oracle's ambiguity bound allows a near-tie to go the other way."""
+    spec = cases.build_spec(case).rounded_f32()
+    grid = oracle.grid_f32(cases.build_grid(case))
+    ref = FX[f"{case['name']}/scube/f32"].astype(np.float64)
+    out, stats = oracle.smdisk(spec, grid, want=('scube', 'amb'))
+    scale = np.abs(ref).max()
+    assert scale > 0 and stats['voxels_on_disk'] > 0
+    bad = np.abs(out['scube'] - ref) > 2e-6 * scale + out['amb']
+    assert bad.sum() == 0, (case['name'], normwise(out['scube'], ref))
+
+
+@pytest.mark.parametrize('name', ['thin_basic', 'thick_sech2',
+                                  'thin_loose_tilted'])
+def test_smdisk_vs_reference_core_f64(name):
+    case = [c for c in cases.SMDISK_CASES if c['name'] == name][0]
+    out, _ = oracle.smdisk(cases.build_spec(case), cases.build_grid(case),
+                           want=('scube',))
+    assert normwise(out['scube'], FX[f'{name}/scube/f64']) < 1e-12
+
+
+@pytest.mark.parametrize('name', ['thin_basic', 'thick_sech2'])
+def test_smdisk_side_outputs_vs_reference(name):
+    case = [c for c in cases.SMDISK_CASES if c['name'] == name][0]
+    spec = cases.build_spec(case).rounded_f32()
+    grid = oracle.grid_f32(cases.build_grid(case))
+    want = ('image', 'rdata', 'vdata_cmp', 'ddata_cmp')
+    out, _ = oracle.smdisk(spec, grid, want=want)
+    for k in want:
+        ref = FX[f'{name}/{k}/f32'].astype(np.float64)
+        m = np.isfinite(ref)
+        got = out[k].copy()
+        if k in ('vdata_cmp', 'ddata_cmp'):
+            got[~m] = np.nan          # the harness pre-fills NaN
+            assert np.array_equal(np.isfinite(got), m), k
+        assert normwise(got, ref) < 2e-6, k
+
+
+def test_planted_rotation_sign_is_caught():
+    """A flipped position angle (a plausible sign mistake) must fail the
+    golden comparison."""
+    case = cases.SMDISK_CASES[0]
+    spec = cases.build_spec(case).rounded_f32()
+    spec.posa = -spec.posa
+    grid = oracle.grid_f32(cases.build_grid(case))
+    out, _ = oracle.smdisk(spec, grid, want=('scube',))
+    assert normwise(out['scube'], FX['thin_basic/scube/f32']) > 1e-2
+
+
+def test_planted_inclination_is_caught():
+    case = cases.SMDISK_CASES[1]
+    spec = cases.build_spec(case).rounded_f32()
+    spec.incl = spec.incl + 1.0
+    grid = oracle.grid_f32(cases.build_grid(case))
+    out, _ = oracle.smdisk(spec, grid, want=('scube',))
+    assert normwise(out['scube'], FX['thick_sech2/scube/f32']) > 1e-3
+
+
+def test_smdisk_thin_flux_closed_form():
+    """Thin exponential disk: total line flux over channels x step equals the
+    analytic surface-density integral (within sampling error)."""
+    T = oracle.Trait
+    rn = np.arange(0, 40.0, 2.0)
+    spec = oracle.DiskSpec(rn, dict(rp=[T(2, params=(1.0, 5.0))],
+                                    vp=[T(2, params=(3.0, 100.0))],
+                                    dp=[T(1, params=(30.0,))]),
+                           posa=0, incl=0)
+    g = oracle.make_grid((200, 200, 1), (0.25, 0.25, 0), (-24.875, -24.875, 0),
+                         120, 5.0, -297.5)
+    out, _ = oracle.smdisk(spec, g, want=('scube',))
+    total = out['scube'].sum() * 5.0 * 0.25 * 0.25
+    # integral of exp(-r/5) over r < 24.875 in 2-D (the grid edge)
+    r = np.linspace(0, 24.875, 200001)
+    # only the inscribed disk is fully inside the square grid; compare the
+    # same region by restricting the node range instead: r < 38 > grid, so
+    # use the numerically integrated square instead
+    xs = -24.875 + 0.25 * np.arange(200)
+    X, Y = np.meshgrid(xs, xs)
+    R = np.hypot(X, Y)
+    expect = np.sum(np.exp(-R / 5.0) * (R < rn[-1])) * 0.25 * 0.25
+    assert abs(total - expect) / expect < 2e-3
+    del r
+
+
+def test_downscale_mask_moments_residual_vs_reference():
+    src = FX['downscale/src'].astype(np.float64)
+    got = oracle.downscale((2, 3, 2), (1, 2, 1), src, (9, 5, 5))
+    assert normwise(got, FX['downscale/dst/f64']) < 1e-14
+    assert normwise(got, FX['downscale/dst/f32']) < 1e-6
+    d, m = oracle.mask(0.5, True, src)
+    assert np.array_equal(np.isnan(d), np.isnan(FX['mask/d/f64']))
+    assert np.array_equal(m, FX['mask/m/f64'])
+    cube = FX['moments/cube'].astype(np.float64)
+    wcube = FX['moments/wcube'].astype(np.float64)
+    md, mm, mw = oracle.moments(cube, (1, 1, 10), (0, 0, -120),
+                                (0, 1, 2, 3, 4), 1e-6, wcube)
+    ref_d = FX['moments/d/f64']
+    assert np.array_equal(np.isnan(md), np.isnan(ref_d))
+    for i in range(md.shape[0]):
+        assert normwise(md[i], ref_d[i]) < 1e-12
+    assert np.array_equal(mm, FX['moments/m/f64'])
+    assert normwise(mw[0], FX['moments/w/f64'][0]) < 1e-12
+    md1, mm1, _ = oracle.moments(cube, (1, 1, 10), (0, 0, -120), (1,), 1e-6)
+    assert normwise(md1, FX['moments1/d/f64']) < 1e-12
+    a = [FX[f'residual/{k}'].astype(np.float64)
+         for k in ('od', 'oe', 'om', 'md', 'mw', 'mm')]
+    full = oracle.residual(*a, 0.7)
+    assert normwise(full, FX['residual/full/f64']) < 1e-14
+    plain = oracle.residual(a[0], None, None, a[3], None, None, 1.0)
+    assert normwise(plain, FX['residual/plain/f64']) < 1e-14
+    s = oracle.residual_sum(FX['residual/full/f64'], True)
+    assert abs(s - float(FX['residual_sum/sq/f64'])) < 1e-9 * s
+    s = oracle.residual_sum(FX['residual/full/f64'], False)
+    assert abs(s - float(FX['residual_sum/abs/f64'])) < 1e-9 * s
+
+
+@pytest.mark.parametrize('case', cases.MCDISK_CASES, ids=lambda c: c['name'])
+def test_mcdisk_statistics_vs_reference_core(case):
+    """The reference's mcdisk (mt19937 stream) and the oracle's (Philox
+    stream) draw different clouds, so agreement is statistical: equal cloud
+    count and total flux, and matching smoothed profiles."""
+    spec = cases.build_spec(case)
+    grid = cases.build_grid(case)
+    cflux, csum, hoi = cases.cloud_budget(spec, case['traits'],
+                                          case['nclouds'])
+    assert np.array_equal(csum, FX[f"{case['name']}/csum"])
+    ref = FX[f"{case['name']}/scube/f64"].astype(np.float64)
+    out, stats = oracle.mcdisk(spec, grid, cflux, csum, hoi, seed=1234,
+                               want=('scube', 'rdata'))
+    # total flux: every in-cube cloud deposits ~its flux (line mostly inside
+    # the spectral range); both totals within a few Poisson sigmas
+    n = stats['voxels_on_disk']
+    t_ref, t_or = ref.sum(), out['scube'].sum()
+    assert abs(t_or - t_ref) / t_ref < 5 / np.sqrt(n)
+    # projected moment-0 maps, smoothed over 4x4 spaxel blocks
+    def blocks(m):
+        ny, nx = m.shape
+        return m[:ny // 4 * 4, :nx // 4 * 4].reshape(
+            ny // 4, 4, nx // 4, 4).sum(axis=(1, 3))
+    a = blocks(ref.sum(axis=0))
+    b = blocks(out['scube'].sum(axis=0))
+    sel = a > 0.05 * a.max()
+    rel = np.abs(a[sel] - b[sel]) / a[sel]
+    assert np.median(rel) < 0.05
+
+
+def test_mcdisk_converges_to_smdisk():
+    """E[mcdisk] = smdisk for the same thick disk (closed-form relation of the
+    two evaluators): with many clouds the block-averaged cubes agree."""
+    case = cases.MCDISK_CASES[0]
+    spec = cases.build_spec(case)
+    grid = cases.build_grid(case)
+    cflux, csum, hoi = cases.cloud_budget(spec, case['traits'], 400000)
+    mc, _ = oracle.mcdisk(spec, grid, cflux, csum, hoi, seed=7,
+                          want=('image',))
+    sm, _ = oracle.smdisk(spec, grid, want=('image',))
+    ny, nx = mc['image'].shape
+    blk = lambda m: m[:ny // 4 * 4, :nx // 4 * 4].reshape(
+        ny // 4, 4, nx // 4, 4).sum(axis=(1, 3))
+    a, b = blk(sm['image']), blk(mc['image'])
+    sel = a > 0.1 * a.max()
+    assert np.median(np.abs(a[sel] - b[sel]) / a[sel]) < 0.05
+
+
+def test_philox_known_answers():
+    """Random123 Philox4x32-10 known-answer vectors (kat_vectors)."""
+    assert oracle.philox4x32([0, 0, 0, 0], [0, 0]) == [
+        0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert oracle.philox4x32([0xffffffff] * 4, [0xffffffff] * 2) == [
+        0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert oracle.philox4x32(
+        [0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344],
+        [0xa4093822, 0x299f31d0]) == [
+        0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+    u = [oracle.cloud_uniform(5, 17, i) for i in range(8)]
+    assert all(0 < v < 1 for v in u)
+
+
+def test_fft_convolution_closed_form_vs_brute_force():
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((3, 4, 5))
+    k = rng.standard_normal((3, 4, 5))
+    assert normwise(oracle.fft_convolve(a, k),
+                    oracle.circular_convolve(a, k)) < 1e-13
+
+
+def test_psf_lsf_properties():
+    """The checks the reference's own tests make on PSF/LSF arrays
+    (tests/unit/test_psflsf.py:26-78, tests/test_psflsf.py:7-10): odd size,
+    unit sum, peak at the centre."""
+    for sigma, step in ((5.0, (0.5, 0.2)), (1.0, (1.0, 1.0))):
+        size = oracle.psf_gauss_size(sigma, step)
+        arr = oracle.psf_gauss_array(sigma, step, size)
+        assert size[0] % 2 == 1 and size[1] % 2 == 1
+        assert arr.shape[::-1] == size
+        assert abs(arr.sum() - 1) < 1e-9
+        iy, ix = np.unravel_index(np.argmax(arr), arr.shape)
+        assert (ix, iy) == (size[0] // 2, size[1] // 2)
+    size = oracle.lsf_gauss_size(5.0, 0.5)
+    arr = oracle.lsf_gauss_array(5.0, 0.5, size)
+    assert size % 2 == 1 and abs(arr.sum() - 1) < 1e-9
+    assert np.argmax(arr) == size // 2
+
+
+def test_dcube_geometry_matches_reference_expectations():
+    """DCube geometry as tests/unit/test_dmodel.py:102-170 expects it."""
+    size, step = (21, 30, 41), (0.5, 1.5, 2.0)
+    rpix, rval, scale = (5.0, 6.5, 7.5), (1.0, 1.5, 2.0), (1, 2, 3)
+    g = oracle.DCubeGeom(size, step, rpix, rval, scale)
+    zero = tuple(rval[i] - rpix[i] * step[i] for i in range(3))
+    assert g.zero_lo == zero
+    assert g.size_hi == (21, 60, 123)
+    assert g.step_hi == (0.5, 0.75, 2.0 / 3)
+    assert g.zero_hi == tuple(zero[i] - step[i] / 2 + 0.5 * g.step_hi[i]
+                              for i in range(3))
+
+
+def test_padding_rules():
+    assert oracle.fft_optimal_shape((296, 296, 110)) == (320, 320, 128)
+    assert oracle.fft_optimal_shape((7, 33, 32)) == (8, 64, 32)
+    assert oracle.smooth235(296) == 300 and oracle.smooth235(110) == 120
+    g = oracle.DCubeGeom((128, 128, 96), (1, 1, 10), scale=(2, 2, 1),
+                         psf_sigma=3 / 2.35482, lsf_sigma=20 / 2.35482)
+    assert g.size_hi == (320, 320, 128) and g.edge_hi == (20, 20, 7)
