@@ -1,0 +1,269 @@
+"""Benchmark: model-cube evaluations/s of the 128x128x96 smdisk + PSF + LSF
+workload (512 parameter sets per step), batched vs plain, with latencies.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--nsets 512]
+
+A step evaluates `nsets` parameter sets of a thin smooth disk (20 ring
+nodes at 1.5 px, exponential 1/8 px, arctan rotation curve, uniform
+dispersion; per set incl U(20,80), PA U(0,360), vmax U(100,300),
+rt U(2,20), sigma U(5,50)) on a 128x128x96 cube, 2x spatial supersampling,
+PSF FWHM 3 px, LSF FWHM 2 channels (10 km/s), reference FFT padding
+(320x320x128), each against a seeded mock cube (noise sigma 0.01) -> one
+FP64 chi^2 per set back on the host.  The timed region covers packing the
+parameter block, the upload, the batched native evaluation (one HIP graph
+replay per chunk sequence) and the download of the chi^2 vector.
+
+The plain path is the drop-in call sequence of the reference
+(Objective.log_likelihood per set: pack + upload trait arrays, smdisk,
+FFT convolution, downscale, residual, residual sum, scalar download).
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, 'tests'))
+
+METRIC = ("model-cube evals/sec (128²×96 smdisk+PSF+LSF), "
+          "batched vs plain; p99 latency")
+HBM_PEAK_GBS = 8000.0
+SIG = 1 / math.sqrt(8 * math.log(2))
+
+
+def workload(nsets, seed=1234):
+    rng = np.random.default_rng(seed)
+    p = dict(incl=rng.uniform(20, 80, nsets), posa=rng.uniform(0, 360, nsets),
+             vt=rng.uniform(100, 300, nsets), rt=rng.uniform(2, 20, nsets),
+             disp=rng.uniform(5, 50, nsets))
+    arrays = dict(xpos=np.zeros(nsets), ypos=np.zeros(nsets),
+                  vsys=np.zeros(nsets), posa=p['posa'], incl=p['incl'],
+                  bpt_a=np.ones(nsets), bpt_s=np.full(nsets, 8.0),
+                  vpt_rt=p['rt'], vpt_vt=p['vt'], dpt_a=p['disp'])
+    return arrays
+
+
+def param_dicts(arrays, n):
+    return [{k: float(v[i]) for k, v in arrays.items()} for i in range(n)]
+
+
+def build_model(driver, thick=False):
+    from gbkfit_amd import gmodels, psflsf
+    from gbkfit_amd.dmodels import DModelSCube
+    from gbkfit_amd.objective import Model
+    from gbkfit_amd.traits import make
+    rn = np.arange(0, 30, 1.5)
+    if thick:
+        cmp = gmodels.SpectralSMDisk3D(
+            False, False, make('bpt', 'exponential'),
+            make('vpt', 'tan_arctan'), make('dpt', 'uniform'),
+            make('bht', 'sech2'), rnodes=rn)
+        gm = gmodels.GModelKinematics3D([cmp], size_z=64)
+    else:
+        cmp = gmodels.SpectralSMDisk2D(
+            False, False, make('bpt', 'exponential'),
+            make('vpt', 'tan_arctan'), make('dpt', 'uniform'), rnodes=rn)
+        gm = gmodels.GModelKinematics2D([cmp])
+    dm = DModelSCube((128, 128, 96), (1, 1, 10), scale=(2, 2, 1),
+                     psf=psflsf.PSFGauss(3.0 * SIG),
+                     lsf=psflsf.LSFGauss(20.0 * SIG))
+    return Model(driver, dm, gm)
+
+
+def percentiles(xs):
+    xs = np.asarray(xs, np.float64) * 1e3
+    return dict(p50_ms=float(np.percentile(xs, 50)),
+                p99_ms=float(np.percentile(xs, 99)),
+                mean_ms=float(xs.mean()))
+
+
+def cpu_baseline(seconds, thick=False):
+    """The reference's own core (compiled by oracle/Makefile into
+    oracle/_ref, OpenMP over all host threads) for smdisk, downscale and the
+    residual / sum, with scipy.fft (workers = all threads) for the
+    convolution FFTW would do; per-set wall time on a bounded sample."""
+    import scipy.fft
+    from oracle import refcore
+    import pipeline_cases as P
+    if not refcore.available():
+        return None
+    R = refcore.ref(np.float32)
+    nthreads = refcore.num_threads()
+    arrays = workload(64, seed=99)
+    sets = param_dicts(arrays, 64)
+    base = P.c3_params(n=1)[0]
+    base['thin'] = not thick
+    geom = P.oracle_geom(base)
+    pc = geom.pcube(np.float32)
+    kspec = scipy.fft.rfftn(pc, workers=nthreads)
+    grid = oracle_grid = None
+    import oracle
+    oracle_grid = oracle.grid_f32(oracle.grid_for(geom, not thick, 64))
+    mock = np.zeros(geom.size_lo[::-1], np.float32)
+    err = np.full(mock.shape, 0.01, np.float32)
+    done, t0 = 0, time.perf_counter()
+    while done < len(sets) and time.perf_counter() - t0 < seconds:
+        p = dict(base)
+        s = sets[done]
+        p.update(incl=s['incl'], posa=s['posa'], vt=s['vpt_vt'],
+                 rt=s['vpt_rt'], disp=s['dpt_a'])
+        spec = P.oracle_spec(p)
+        cube = R.smdisk(spec, oracle_grid, ('scube',), parallel=True)['scube']
+        f = scipy.fft.rfftn(cube, workers=nthreads)
+        f *= kspec
+        conv = scipy.fft.irfftn(f, s=cube.shape, axes=(0, 1, 2),
+                                workers=nthreads).astype(np.float32)
+        lo = R.downscale(geom.scale, geom.edge_hi, conv, geom.size_lo,
+                         parallel=True)
+        res = R.residual(mock, err, None, lo, None, None, 1.0, parallel=True)
+        R.residual_sum(res, True, parallel=True)
+        done += 1
+    dt = time.perf_counter() - t0
+    del grid
+    return dict(value=done / dt, unit='evals/s', cores=nthreads,
+                kind='reference',
+                sample=f"{done} parameter sets of the same workload, "
+                       f"{dt:.1f} s; reference core (smdisk, downscale, "
+                       f"residual, residual_sum) compiled from its sources "
+                       f"with OpenMP, FFT convolution by scipy.fft (FFTW "
+                       f"is absent)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--nsets', type=int, default=512)
+    ap.add_argument('--chunk', type=int, default=64)
+    ap.add_argument('--plain-sets', type=int, default=128)
+    ap.add_argument('--cpu-seconds', type=float, default=15.0)
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-graph', action='store_true')
+    ap.add_argument('--thick', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps(dict(error="this benchmark runs on one GPU; "
+                              "multi-GPU runs are not part of this build")))
+        sys.exit(2)
+
+    import torch
+    from gbkfit_amd.batch import BatchEvaluator
+    from gbkfit_amd.driver.hip import DriverHip
+    from gbkfit_amd.objective import Data, Objective
+
+    drv = DriverHip()
+    model = build_model(drv, args.thick)
+    arrays = workload(args.nsets)
+    # mock data: the model of the first set plus seeded noise
+    truth = param_dicts(arrays, 1)[0]
+    cube0 = model.model_h(truth)['scube']['d'].astype(np.float32)
+    rng = np.random.default_rng(2024)
+    mock = (cube0 + rng.normal(0, 0.01, cube0.shape)).astype(np.float32)
+
+    ev = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock, error=0.01,
+                        max_batch=args.chunk, template_params=truth)
+    use_graph = not args.no_graph
+    ev.set_timing(True)
+
+    def step():
+        host = ev.packer.pack_arrays(arrays, args.nsets)
+        ev.upload_packed(host)
+        ev.run(args.nsets, use_graph=use_graph)
+        return ev._chi2_dev.cpu().numpy()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    lat, stage_ms = [], np.zeros(3)
+    chunks = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        chi2 = step()
+        lat.append(time.perf_counter() - ts)
+        ms, chunks = ev.stage_times()
+        stage_ms += ms
+    torch.cuda.synchronize()
+    t_batched = time.perf_counter() - t0
+    evals = args.nsets * args.steps
+    value = evals / t_batched
+
+    # plain drop-in path, one parameter set per call
+    obj = Objective([dict(scube=Data(mock, error=np.full(mock.shape, 0.01,
+                                                          np.float32)))],
+                    [model])
+    plain_sets = param_dicts(arrays, args.plain_sets)
+    for p in plain_sets[:4]:
+        obj.log_likelihood(p)
+    torch.cuda.synchronize()
+    plat = []
+    tp = time.perf_counter()
+    for p in plain_sets:
+        ts = time.perf_counter()
+        obj.log_likelihood(p)
+        plat.append(time.perf_counter() - ts)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - tp
+    plain_value = len(plain_sets) / t_plain
+
+    # roofline of the dominant stage (batched FFT convolution): algorithmic
+    # bytes per set = 12 * N_pad (read the cube, read the half spectrum,
+    # write the cube), per launch = sets per chunk
+    n_pad = int(np.prod(ev.size_hi))
+    per_chunk = args.nsets / max(chunks, 1)
+    conv_ms = stage_ms[1] / (args.steps * max(chunks, 1))
+    conv_bytes = 12.0 * n_pad * per_chunk
+    achieved = conv_bytes / (conv_ms * 1e-3) / 1e9 if conv_ms > 0 else None
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(args.cpu_seconds, args.thick)
+        except Exception as e:  # the harness may be absent
+            cpu = dict(error=str(e))
+
+    out = dict(
+        metric=METRIC, value=value, unit='evals/s', n_gpus=1,
+        steps=args.steps, warmup=args.warmup,
+        ms_per_step=1e3 * t_batched / args.steps, higher_is_better=True,
+        scaling='weak', vs_baseline=None, dtype='fp32', data='synthetic',
+        config=dict(workload=(
+            f"smdisk {'thick' if args.thick else 'thin'} 128x128x96, 2x "
+            f"spatial supersampling, PSF FWHM 3 px + LSF FWHM 2 ch FFT "
+            f"convolution (padded 320x320x128), chi^2 vs seeded mock; "
+            f"{args.nsets} parameter sets per step"),
+            nsets=args.nsets, chunk=args.chunk, graph=use_graph,
+            padding='reference'),
+        plain=dict(value=plain_value, unit='evals/s',
+                   sets=len(plain_sets), **percentiles(plat)),
+        batched_over_plain=value / plain_value,
+        batch_latency=percentiles(lat),
+        per_eval_latency_batched_ms=1e3 * t_batched / evals,
+        stages_ms_per_step=dict(smdisk=stage_ms[0] / args.steps,
+                                fft_convolution=stage_ms[1] / args.steps,
+                                downscale_chi2=stage_ms[2] / args.steps),
+        roofline=dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS,
+                      unit='GB/s',
+                      frac=(achieved / HBM_PEAK_GBS) if achieved else None,
+                      traffic=None,
+                      kernel='batched FFT convolution stage (5 kernels)'),
+        cpu_baseline=cpu,
+        host=dict(cpu=platform.processor() or platform.machine(),
+                  nproc=os.cpu_count()),
+        chi2_sample=[float(c) for c in chi2[:3]],
+    )
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,308 @@
+"""Batched and served evaluation of many parameter sets of one galaxy model.
+
+The reference evaluates one parameter set per likelihood call (its fitters
+call Objective.log_likelihood per walker, src/gbkfit/fitting/fitters/emcee/
+sampler.py:34-57).  ``BatchEvaluator`` evaluates N sets of one smdisk model
+(DModelSCube + Kinematics2D/3D with one SpectralSMDisk component) against
+one observed cube in a single native call (gbkhip_batch_evaluate): batched
+smdisk, batched FFT convolution against the shared PSF x LSF spectrum, and a
+fused downscale + residual + FP64 chi^2 pass, captured in a HIP graph.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+from .driver.hip import NativeMemoryHip, make_grid
+
+_c = ctypes
+
+
+class BTraits(_c.Structure):
+    _fields_ = [('n', _c.c_int), ('uids', _c.POINTER(_c.c_int)),
+                ('ccounts', _c.POINTER(_c.c_int)),
+                ('pcounts', _c.POINTER(_c.c_int)),
+                ('off_cvalues', _c.c_int), ('off_pvalues', _c.c_int)]
+
+
+class BDesc(_c.Structure):
+    _fields_ = [('loose', _c.c_int), ('tilted', _c.c_int),
+                ('nrnodes', _c.c_int), ('rnodes', _c.POINTER(_c.c_float)),
+                ('vsys_count', _c.c_int), ('off_vsys', _c.c_int),
+                ('off_xpos', _c.c_int), ('off_ypos', _c.c_int),
+                ('off_posa', _c.c_int), ('off_incl', _c.c_int),
+                ('traits', BTraits * 9), ('pstride', _c.c_int),
+                ('grid', N.Grid), ('size_lo', _c.c_int * 3),
+                ('size_hi', _c.c_int * 3), ('scale', _c.c_int * 3),
+                ('edge_hi', _c.c_int * 3), ('has_conv', _c.c_int),
+                ('pcube', _c.c_void_p), ('obs', _c.c_void_p),
+                ('err', _c.c_void_p), ('mask', _c.c_void_p),
+                ('err_scalar', _c.c_float), ('weight', _c.c_float),
+                ('max_batch', _c.c_int)]
+
+
+N.register_signatures({
+    'gbkhip_batch_create': ([_c.c_void_p, _c.POINTER(BDesc),
+                             _c.POINTER(_c.c_void_p)], _c.c_int),
+    'gbkhip_batch_destroy': ([_c.c_void_p], _c.c_int),
+    'gbkhip_batch_evaluate': ([_c.c_void_p, _c.c_void_p, _c.c_int,
+                               _c.c_void_p, _c.c_void_p, _c.c_int], _c.c_int),
+    'gbkhip_batch_set_timing': ([_c.c_void_p, _c.c_int], _c.c_int),
+    'gbkhip_batch_stage_times': ([_c.c_void_p, _c.POINTER(_c.c_double),
+                                  _c.POINTER(_c.c_int)], _c.c_int),
+})
+
+_SLOTS = ('rp', 'rh', 'vp', 'vh', 'dp', 'dh', 'zp', 'sp', 'wp')
+
+
+def _interp_matrix(nodes, sub):
+    """W with W @ values == np.interp(sub, nodes, values)."""
+    W = np.zeros((sub.size, nodes.size))
+    for i, x in enumerate(sub):
+        j = int(np.clip(np.searchsorted(nodes, x, side='right') - 1, 0,
+                        nodes.size - 2))
+        t = (x - nodes[j]) / (nodes[j + 1] - nodes[j])
+        W[i, j] = 1 - t
+        W[i, j + 1] = t
+    return W
+
+
+class SetPacker:
+    """Flattens parameter dicts of one Disk into fixed-layout float vectors:
+    [vsys][xpos][ypos][posa][incl] then per trait slot its constants and its
+    parameters (the arrays Disk.pack produces, concatenated)."""
+
+    def __init__(self, disk, template_params):
+        packed = disk.pack(template_params)
+        self.disk = disk
+        off = 0
+        self.offsets = {}
+        for k in ('vsys', 'xpos', 'ypos', 'posa', 'incl'):
+            a = packed[k]
+            self.offsets[k] = (off, 0 if a is None else a.size)
+            off += 0 if a is None else a.size
+        self.trait_offsets = {}
+        self.trait_ints = {}
+        for k in _SLOTS:
+            tr = packed[k]
+            if tr is None:
+                continue
+            uids, cv, cc, pv, pc = tr
+            self.trait_offsets[k] = (off, off + cv.size)
+            self.trait_ints[k] = (uids.copy(), cc.copy(), pc.copy())
+            off += cv.size + pv.size
+        self.pstride = off
+
+    def pack_arrays(self, arrays, nsets):
+        """Vectorised packing: `arrays` maps parameter name -> array of
+        shape (nsets,) (scalar parameters) or (nsets, n) (vector / node-wise
+        ones).  Node-wise values are interpolated with one matrix product."""
+        d = self.disk
+        out = np.zeros((nsets, self.pstride), np.float32)
+        W = _interp_matrix(d.rnodes, d.subrnodes)
+
+        def col(name, nw):
+            v = np.asarray(arrays[name], np.float64).reshape(nsets, -1)
+            if nw:
+                if v.shape[1] == 1:
+                    v = np.repeat(v, d.rnodes.size, axis=1)
+                return v @ W.T
+            return v
+        for k, nw in (('vsys', d.loose), ('xpos', d.loose),
+                      ('ypos', d.loose), ('posa', d.tilted),
+                      ('incl', d.tilted)):
+            o, n = self.offsets[k]
+            if n:
+                out[:, o:o + n] = col(k, nw)[:, :n]
+        for k, (oc, op) in self.trait_offsets.items():
+            pos = op
+            cpos = oc
+            for tr, names in zip(d.traits[k], d.names[k]):
+                nc = len(tr.consts())
+                out[:, cpos:cpos + nc] = np.asarray(tr.consts(), np.float32)
+                cpos += nc
+                for n, size in tr.params_sm:
+                    out[:, pos:pos + size] = col(names[n], False)[:, :size]
+                    pos += size
+                for n in tr.params_nw:
+                    ns = d.subrnodes.size
+                    out[:, pos:pos + ns] = col(names[n], True)
+                    pos += ns
+        return out
+
+    def pack(self, params_list):
+        out = np.zeros((len(params_list), self.pstride), np.float32)
+        for i, params in enumerate(params_list):
+            packed = self.disk.pack(params)
+            for k, (o, n) in self.offsets.items():
+                if n:
+                    out[i, o:o + n] = packed[k]
+            for k, (oc, op) in self.trait_offsets.items():
+                _, cv, _, pv, _ = packed[k]
+                out[i, oc:oc + cv.size] = cv
+                out[i, op:op + pv.size] = pv
+        return out
+
+
+class BatchEvaluator:
+    """chi^2 of many parameter sets of one smdisk model against one cube.
+
+    dmodel: DModelSCube (its DCube geometry, PSF and LSF are used);
+    gmodel: GModelKinematics2D/3D with exactly one SpectralSMDisk component;
+    obs:    observed low-res cube (numpy, z-y-x); error: scalar or cube.
+    """
+
+    def __init__(self, driver, dmodel, gmodel, obs, error=None, mask=None,
+                 weight=1.0, max_batch=64, template_params=None):
+        self.driver = driver
+        comps = gmodel.components
+        if len(comps) != 1 or not hasattr(comps[0].disk, 'traits') or \
+                type(comps[0].disk).__name__ != 'SMDisk':
+            raise RuntimeError("BatchEvaluator needs one smdisk component")
+        disk = comps[0].disk
+        cube = dmodel.dcube()
+        be_fft = driver.backends().fft(cube.dtype(), cube._padding)
+        size_hi, step_hi, edge_hi, zero_hi = cube.geometry(be_fft)
+        thin = getattr(comps[0], 'thin', True)
+        if thin:
+            spat = ((size_hi[0], size_hi[1], 1), (step_hi[0], step_hi[1], 0),
+                    (zero_hi[0], zero_hi[1], 0))
+        else:
+            spat = gmodel.spatial_grid(size_hi, step_hi, zero_hi)
+        self.grid = make_grid(spat[0], spat[1], spat[2], size_hi[2],
+                              step_hi[2], zero_hi[2])
+        if template_params is None:
+            template_params = {n: np.zeros(s) if s > 1 else 0.0
+                               for n, s in disk.pdescs().items()}
+        self.packer = SetPacker(disk, template_params)
+        t = driver_torch()
+        dev = driver.device
+        self._keep = []
+        d = BDesc()
+        d.loose, d.tilted = int(disk.loose), int(disk.tilted)
+        rn = np.ascontiguousarray(disk.subrnodes, np.float32)
+        self._keep.append(rn)
+        d.nrnodes = rn.size
+        d.rnodes = rn.ctypes.data_as(_c.POINTER(_c.c_float))
+        o = self.packer.offsets
+        d.vsys_count = o['vsys'][1]
+        d.off_vsys, d.off_xpos, d.off_ypos = o['vsys'][0], o['xpos'][0], \
+            o['ypos'][0]
+        d.off_posa, d.off_incl = o['posa'][0], o['incl'][0]
+        for i, k in enumerate(_SLOTS):
+            bt = BTraits()
+            if k in self.packer.trait_ints:
+                uids, cc, pc = self.packer.trait_ints[k]
+                self._keep.extend([uids, cc, pc])
+                bt.n = uids.size
+                bt.uids = uids.ctypes.data_as(_c.POINTER(_c.c_int))
+                bt.ccounts = cc.ctypes.data_as(_c.POINTER(_c.c_int))
+                bt.pcounts = pc.ctypes.data_as(_c.POINTER(_c.c_int))
+                bt.off_cvalues, bt.off_pvalues = self.packer.trait_offsets[k]
+            d.traits[i] = bt
+        d.pstride = self.packer.pstride
+        d.grid = self.grid
+        d.size_lo[:] = list(cube.size())
+        d.size_hi[:] = list(size_hi)
+        d.scale[:] = list(cube.scale())
+        d.edge_hi[:] = list(edge_hi)
+        d.has_conv = int(bool(cube.psf() or cube.lsf()))
+        if d.has_conv:
+            self._pcube = driver.mem_copy_h2d(
+                cube.make_pcube(size_hi, step_hi, be_fft))
+            d.pcube = self._pcube.data_ptr()
+        self._obs = driver.mem_copy_h2d(np.ascontiguousarray(obs, np.float32))
+        d.obs = self._obs.data_ptr()
+        if error is None or np.isscalar(error):
+            d.err = None
+            d.err_scalar = 1.0 if error is None else float(error)
+        else:
+            self._err = driver.mem_copy_h2d(np.asarray(error, np.float32))
+            d.err = self._err.data_ptr()
+            d.err_scalar = 1.0
+        if mask is not None:
+            self._mask = driver.mem_copy_h2d(np.asarray(mask, np.float32))
+            d.mask = self._mask.data_ptr()
+        d.weight = float(weight)
+        d.max_batch = int(max_batch)
+        self.desc = d
+        self.size_lo = tuple(cube.size())
+        self.size_hi = tuple(size_hi)
+        h = _c.c_void_p()
+        N.call('gbkhip_batch_create', driver.context().handle,
+               _c.byref(d), _c.byref(h))
+        self.handle = h
+        self._params_dev = None
+        self._chi2_dev = None
+        self._cubes_dev = None
+        del t, dev
+
+    def upload(self, params_list):
+        """Packs parameter dicts into the persistent device buffer."""
+        host = self.packer.pack(params_list)
+        return self.upload_packed(host)
+
+    def upload_packed(self, host):
+        t = driver_torch()
+        n = host.shape[0]
+        if self._params_dev is None or self._params_dev.shape[0] != n:
+            self._params_dev = t.empty(host.shape, dtype=t.float32,
+                                       device=self.driver.device)
+            self._chi2_dev = t.empty(n, dtype=t.float64,
+                                     device=self.driver.device)
+            self._cubes_dev = None
+        self._params_dev.copy_(t.from_numpy(host))
+        return n
+
+    def run(self, nsets, want_cubes=False, use_graph=True):
+        """Evaluates the uploaded sets (asynchronous on the driver stream)."""
+        t = driver_torch()
+        if want_cubes and (self._cubes_dev is None or
+                           self._cubes_dev.shape[0] != nsets):
+            self._cubes_dev = t.empty((nsets,) + self.size_lo[::-1],
+                                      dtype=t.float32,
+                                      device=self.driver.device)
+        N.call('gbkhip_batch_evaluate', self.handle,
+               self._params_dev.data_ptr(), int(nsets),
+               self._chi2_dev.data_ptr(),
+               self._cubes_dev.data_ptr() if want_cubes else None,
+               int(bool(use_graph)))
+
+    def evaluate(self, params_list, want_cubes=False, use_graph=True):
+        """chi^2 per set (numpy FP64), and the model cubes if asked."""
+        n = self.upload(params_list)
+        self.run(n, want_cubes, use_graph)
+        chi2 = self._chi2_dev.cpu().numpy().copy()
+        if want_cubes:
+            return chi2, self._cubes_dev.cpu().numpy().copy()
+        return chi2
+
+    def set_timing(self, enable):
+        N.call('gbkhip_batch_set_timing', self.handle, int(bool(enable)))
+
+    def stage_times(self):
+        """(smdisk, convolution, downscale+chi2) milliseconds summed over
+        the chunks of the last evaluation, and the chunk count."""
+        ms = (_c.c_double * 3)()
+        n = _c.c_int()
+        N.call('gbkhip_batch_stage_times', self.handle, ms, _c.byref(n))
+        return tuple(ms), n.value
+
+    def close(self):
+        if getattr(self, 'handle', None):
+            N.lib().gbkhip_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def driver_torch():
+    import torch
+    return torch
+
+
+__all__ = ['BatchEvaluator', 'SetPacker', 'NativeMemoryHip']

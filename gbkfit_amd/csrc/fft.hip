@@ -482,7 +482,7 @@ static void launch_lines(Context& c, const FFTPlan& p, int axis, int mode,
     GBK_HIP(hipGetLastError());
 }
 
-static void set_lds_limits()
+void fft_prepare_attributes()
 {
     static std::once_flag once;
     std::call_once(once, [] {
@@ -502,7 +502,7 @@ static void set_lds_limits()
 void fft_forward(Context& c, const FFTPlan& p, const float* in, float2* out,
                  int batch, long long br, long long bc)
 {
-    set_lds_limits();
+    fft_prepare_attributes();
     launch_rows(c, p, true, in, out, batch, br, bc);
     if (p.ny > 1) launch_lines(c, p, 1, 0, out, out, nullptr, 1, batch, bc, 0);
     if (p.nz > 1) launch_lines(c, p, 2, 0, out, out, nullptr, 1, batch, bc, 0);
@@ -512,7 +512,7 @@ void fft_forward(Context& c, const FFTPlan& p, const float* in, float2* out,
 void fft_inverse(Context& c, const FFTPlan& p, const float2* in, float2* work,
                  float* out, int batch, long long bc, long long br)
 {
-    set_lds_limits();
+    fft_prepare_attributes();
     const float2* cur = in;
     if (p.nz > 1) {
         launch_lines(c, p, 2, 1, cur, work, nullptr, 1, batch, bc, 0);
@@ -530,7 +530,7 @@ void fft_convolve_with(Context& c, const FFTPlan& p, float* data_r,
                        float2* spec, const float2* K, int batch, long long br,
                        long long bc, long long kbc)
 {
-    set_lds_limits();
+    fft_prepare_attributes();
     const float scale = 1.0f / ((float)p.nx * (float)p.ny * (float)p.nz);
     launch_rows(c, p, true, data_r, spec, batch, br, bc);
     if (p.nz > 1) {

@@ -46,6 +46,7 @@ void fft_forward(Context& c, const FFTPlan& p, const float* in, float2* out,
                  int batch, long long br, long long bc);
 void fft_inverse(Context& c, const FFTPlan& p, const float2* in, float2* work,
                  float* out, int batch, long long bc, long long br);
+void fft_prepare_attributes();
 void fft_convolve_with(Context& c, const FFTPlan& p, float* data_r,
                        float2* spec, const float2* K, int batch, long long br,
                        long long bc, long long kbc);

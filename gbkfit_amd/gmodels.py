@@ -1,0 +1,454 @@
+"""Galaxy models: disks evaluated into spectral cubes.
+
+Mirror of the reference callers of the GModel backend:
+  * Disk parameter handling and packing (src/gbkfit/model/gmodels/_disk.py:
+    16-631): sub-ring nodes, linear interpolation of node-wise parameters
+    onto them, flat trait arrays (uids / consts / params and their counts);
+  * SMDisk / MCDisk (_smdisk.py:38-96, _mcdisk.py:60-189);
+  * spectral components SpectralSMDisk2D/3D and SpectralMCDisk3D
+    (spectral_smdisk_2d.py, spectral_smdisk_3d.py, spectral_mcdisk_3d.py);
+  * GModelKinematics2D/3D (kinematics_2d.py, kinematics_3d.py).
+Parameter names follow the reference: 'xpos', 'ypos', 'posa', 'incl',
+'vsys' and '<kind>[i]_<name>' per trait (miscutils.py:70-95), with 'cmp<i>_'
+prefixed for components after the first.
+"""
+import itertools
+
+import numpy as np
+
+from . import traits as T
+
+_TRAIT_SLOTS = ('rp', 'rh', 'vp', 'vh', 'dp', 'dh', 'zp', 'sp', 'wp')
+
+
+def subrnodes(rnodes, rstep):
+    """_disk.py:151-156."""
+    s = np.arange(rnodes[0] + rstep / 2, rnodes[-1], rstep)
+    s = np.insert(s, 0, rnodes[0])
+    s = np.append(s, s[-1] + rstep / 2)
+    return s
+
+
+def parse_rnodes(rnmin=None, rnmax=None, rnsep=None, rnlen=None, rnodes=None,
+                 rstep=None):
+    """gmodels/_detail.py:13-66 (node list, arange or linspace form)."""
+    listed = rnodes is not None
+    arange = None not in (rnmin, rnmax, rnsep)
+    linsp = None not in (rnmin, rnmax, rnlen)
+    if [listed, arange, linsp].count(True) != 1:
+        raise RuntimeError("exactly one of rnodes, (rnmin, rnmax, rnsep), "
+                           "(rnmin, rnmax, rnlen) must be given")
+    if arange:
+        rnodes = np.arange(rnmin, rnmax + rnsep, rnsep)
+    elif linsp:
+        rnodes = np.linspace(rnmin, rnmax, rnlen)
+    rnodes = np.asarray(rnodes, np.float64)
+    if rnodes.size < 2:
+        raise RuntimeError("at least two rnodes must be provided")
+    d = np.diff(rnodes)
+    if np.any(d <= 0) or np.any(rnodes < 0):
+        raise RuntimeError("rnodes must be positive, unique and ascending")
+    if rstep is None:
+        rstep = min(1.0, float(d.min()) / 2)
+    if rstep <= 0 or rstep > d.min() / 2:
+        raise RuntimeError("rstep must be in (0, min(diff(rnodes)) / 2]")
+    return rnodes, float(rstep)
+
+
+def _names(traits_, prefix):
+    """Per trait: dict original name -> full parameter name."""
+    out = []
+    for i, tr in enumerate(traits_):
+        full = prefix + (str(i) if i else '') + '_'
+        names = [n for n, _ in tr.params_sm] + list(tr.params_nw)
+        out.append({n: full + n for n in names})
+    return out
+
+
+class Disk:
+    """Shared machinery of SMDisk and MCDisk (_disk.py:128-631)."""
+
+    def __init__(self, loose, tilted, rnodes, rstep, traits_):
+        self.loose = bool(loose)
+        self.tilted = bool(tilted)
+        self.rnodes = np.asarray(rnodes, np.float64)
+        self.rstep = float(rstep)
+        self.subrnodes = subrnodes(self.rnodes, self.rstep)
+        self.traits = {k: tuple(traits_.get(k, ())) for k in _TRAIT_SLOTS}
+        self.is_opacity = bool(self.traits['rp']) and \
+            self.traits['rp'][0].kind == 'opt'
+        rp, rh = ('opt', 'oht') if self.is_opacity else ('bpt', 'bht')
+        self.prefixes = dict(rp=rp, rh=rh, vp='vpt', vh='vht', dp='dpt',
+                             dh='dht', zp='zpt', sp='spt', wp='wpt')
+        self.names = {k: _names(self.traits[k], self.prefixes[k])
+                      for k in _TRAIT_SLOTS}
+        nn = self.rnodes.size
+        pd = {}
+        if self.traits['vp']:
+            pd['vsys'] = nn if loose else 1
+        pd['xpos'] = nn if loose else 1
+        pd['ypos'] = nn if loose else 1
+        pd['posa'] = nn if tilted else 1
+        pd['incl'] = nn if tilted else 1
+        for k in _TRAIT_SLOTS:
+            for tr, names in zip(self.traits[k], self.names[k]):
+                for n, size in tr.params_sm:
+                    pd[names[n]] = size
+                for n in tr.params_nw:
+                    pd[names[n]] = nn
+        self._pdescs = pd
+        self._dev = None
+        self._driver = None
+
+    def pdescs(self):
+        return dict(self._pdescs)
+
+    def _interp(self, values):
+        v = np.asarray(values, np.float64)
+        if v.size == 1:
+            v = np.full(self.rnodes.size, float(v))
+        if v.size != self.rnodes.size:
+            raise RuntimeError("node-wise parameter has the wrong length")
+        lo, hi = self.rnodes[0], self.rnodes[-1]
+        if self.subrnodes[0] < lo - 1e-9 or self.subrnodes[-1] > hi + 1e-9:
+            raise RuntimeError("sub-ring nodes outside the node range")
+        return np.interp(self.subrnodes, self.rnodes, v)
+
+    def pack(self, params):
+        """Host arrays for one parameter set: common parameters and, per
+        trait kind, (uids, cvalues, ccounts, pvalues, pcounts)
+        (_disk.py:42-125,515-553)."""
+        def common(name, nw):
+            if name not in self._pdescs:
+                return None
+            v = params[name]
+            if nw:
+                return self._interp(v).astype(np.float32)
+            return np.atleast_1d(np.asarray(v, np.float32))[:1].copy()
+        out = dict(vsys=common('vsys', self.loose),
+                   xpos=common('xpos', self.loose),
+                   ypos=common('ypos', self.loose),
+                   posa=common('posa', self.tilted),
+                   incl=common('incl', self.tilted))
+        for k in _TRAIT_SLOTS:
+            tl = self.traits[k]
+            if not tl:
+                out[k] = None
+                continue
+            uids, cv, cc, pv, pc = [], [], [], [], []
+            for tr, names in zip(tl, self.names[k]):
+                uids.append(tr.uid())
+                cv += list(tr.consts())
+                cc.append(len(tr.consts()))
+                vals = []
+                for n, size in tr.params_sm:
+                    vals += list(np.atleast_1d(np.asarray(params[names[n]],
+                                                          np.float64)))[:size]
+                for n in tr.params_nw:
+                    vals += list(self._interp(params[names[n]]))
+                pv += vals
+                pc.append(len(vals))
+            out[k] = (np.array(uids, np.int32),
+                      np.array(cv or [0.0], np.float32),
+                      np.array(cc, np.int32),
+                      np.array(pv or [0.0], np.float32),
+                      np.array(pc, np.int32))
+        return out
+
+    def _upload(self, driver, packed):
+        """Copies packed arrays into persistent device buffers."""
+        if self._driver is not driver or self._dev is None:
+            self._driver = driver
+            self._dev = {}
+            self._dev['rnodes'] = driver.mem_copy_h2d(
+                self.subrnodes.astype(np.float32))
+        dev = self._dev
+
+        def put(key, arr):
+            if arr is None:
+                dev[key] = None
+                return
+            cur = dev.get(key)
+            if cur is None or tuple(cur.shape) != arr.shape or \
+                    str(cur.dtype).split('.')[-1] != arr.dtype.name:
+                cur = driver.mem_alloc_d(arr.shape, arr.dtype)
+                dev[key] = cur
+            driver.mem_copy_h2d(arr, cur)
+
+        for k in ('vsys', 'xpos', 'ypos', 'posa', 'incl'):
+            put(k, packed[k])
+        for k in _TRAIT_SLOTS:
+            tr = packed[k]
+            for j, name in enumerate(('uids', 'cvalues', 'ccounts', 'pvalues',
+                                      'pcounts')):
+                put(f'{k}_{name}', None if tr is None else tr[j])
+        return dev
+
+    def native_args(self, driver, params):
+        """The positional disk arguments of smdisk/mcdisk_evaluate
+        (loose .. wpt_pcounts)."""
+        dev = self._upload(driver, self.pack(params))
+        args = [self.loose, self.tilted, dev['rnodes'], dev['vsys'],
+                dev['xpos'], dev['ypos'], dev['posa'], dev['incl']]
+        for k in _TRAIT_SLOTS:
+            for name in ('uids', 'cvalues', 'ccounts', 'pvalues', 'pcounts'):
+                args.append(dev[f'{k}_{name}'])
+        return args
+
+
+class SMDisk(Disk):
+    """_smdisk.py:8-96."""
+
+    def evaluate(self, driver, params, odata, image, scube, wdata, rdata,
+                 ordata, spat_size, spat_step, spat_zero, spat_rota,
+                 spec_size, spec_step, spec_zero, dtype, out_extra=None):
+        backend = driver.backends().gmodel(dtype)
+        backend.smdisk_evaluate(
+            *self.native_args(driver, params), odata,
+            spat_size, spat_step, spat_zero, spat_rota,
+            spec_size, spec_step, spec_zero,
+            image, scube, wdata, None, rdata, None, ordata, None, None, None)
+
+
+class MCDisk(Disk):
+    """_mcdisk.py:16-189: the cloud budget is the trait integral per ring
+    (or whole disk) divided by the flux of one cloud."""
+
+    def __init__(self, cflux, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+        self.cflux = float(cflux)
+        self._budget_dev = None
+
+    def cloud_budget(self, params):
+        counts = []
+        rings = self.subrnodes[1:-1]
+        for tr, names in zip(self.traits['rp'], self.names['rp']):
+            tp = {}
+            for n, _ in tr.params_sm:
+                tp[n] = params[names[n]]
+            for n in tr.params_nw:
+                tp[n] = self._interp(params[names[n]])[1:-1]
+            integral = tr.integrate(tp, rings)
+            counts.extend(np.atleast_1d(integral / self.cflux)
+                          .astype(np.int32))
+        csum = np.array(list(itertools.accumulate(counts)), np.int32)
+        hoi = np.array([tr.has_analytical_integral()
+                        for tr in self.traits['rp']], np.uint8)
+        return csum, hoi
+
+    def evaluate(self, driver, params, odata, image, scube, wdata, rdata,
+                 ordata, spat_size, spat_step, spat_zero, spat_rota,
+                 spec_size, spec_step, spec_zero, dtype, out_extra=None):
+        backend = driver.backends().gmodel(dtype)
+        csum, hoi = self.cloud_budget(params)
+        d_csum = driver.mem_copy_h2d(csum)
+        d_hoi = driver.mem_copy_h2d(hoi)
+        backend.mcdisk_evaluate(
+            self.cflux, int(csum[-1]) if csum.size else 0, d_csum, d_hoi,
+            *self.native_args(driver, params), odata,
+            spat_size, spat_step, spat_zero, spat_rota,
+            spec_size, spec_step, spec_zero,
+            image, scube, wdata, None, rdata, None, ordata, None, None, None)
+
+
+def _traits_of(kind, value):
+    if value is None:
+        return ()
+    if isinstance(value, (list, tuple)):
+        return tuple(T.load(kind, v) if isinstance(v, dict) else v
+                     for v in value)
+    return (T.load(kind, value) if isinstance(value, dict) else value,)
+
+
+class _Component:
+    def pdescs(self):
+        return self.disk.pdescs()
+
+    def has_weights(self):
+        return bool(self.disk.traits['wp'])
+
+
+class SpectralSMDisk2D(_Component):
+    """Thin smooth disk (spectral_smdisk_2d.py:60-148): evaluated on a single
+    spatial plane (size_z 1, step_z 0, zero_z 0)."""
+
+    def __init__(self, loose, tilted, bptraits, vptraits, dptraits,
+                 sptraits=None, wptraits=None, rnmin=None, rnmax=None,
+                 rnsep=None, rnlen=None, rnodes=None, rstep=None,
+                 interp='linear'):
+        if interp != 'linear':
+            raise RuntimeError("only linear interpolation is supported")
+        rn, rs = parse_rnodes(rnmin, rnmax, rnsep, rnlen, rnodes, rstep)
+        tr = dict(rp=_traits_of('bpt', bptraits), vp=_traits_of('vpt', vptraits),
+                  dp=_traits_of('dpt', dptraits), sp=_traits_of('spt', sptraits),
+                  wp=_traits_of('wpt', wptraits))
+        if not (tr['rp'] and tr['vp'] and tr['dp']):
+            raise RuntimeError("bptraits, vptraits and dptraits are required")
+        self.disk = SMDisk(loose, tilted, rn, rs, tr)
+        self.thin = True
+
+    def evaluate(self, driver, params, scube, wdata, bdata, spat_size,
+                 spat_step, spat_zero, spat_rota, spec_size, spec_step,
+                 spec_zero, dtype, out_extra=None):
+        self.disk.evaluate(driver, params, None, None, scube, wdata, bdata,
+                           None, tuple(spat_size) + (1,),
+                           tuple(spat_step) + (0,), tuple(spat_zero) + (0,),
+                           spat_rota, spec_size, spec_step, spec_zero, dtype,
+                           out_extra)
+
+
+def _thick_traits(bptraits, bhtraits, vptraits, vhtraits, dptraits, dhtraits,
+                  zptraits, sptraits, wptraits):
+    """parse_component_s3d_traits (gmodels/_detail.py:183-266): missing
+    velocity / dispersion height traits default to 'one'."""
+    tr = dict(rp=_traits_of('bpt', bptraits), rh=_traits_of('bht', bhtraits),
+              vp=_traits_of('vpt', vptraits), dp=_traits_of('dpt', dptraits),
+              zp=_traits_of('zpt', zptraits), sp=_traits_of('spt', sptraits),
+              wp=_traits_of('wpt', wptraits))
+    if not (tr['rp'] and tr['rh'] and tr['vp'] and tr['dp']):
+        raise RuntimeError("bptraits, bhtraits, vptraits and dptraits are "
+                           "required")
+    tr['vh'] = _traits_of('vht', vhtraits) or tuple(
+        T.make('vht', 'one') for _ in tr['vp'])
+    tr['dh'] = _traits_of('dht', dhtraits) or tuple(
+        T.make('dht', 'one') for _ in tr['dp'])
+    for a, b in (('rp', 'rh'), ('vp', 'vh'), ('dp', 'dh')):
+        if len(tr[a]) != len(tr[b]):
+            raise RuntimeError(f"{a} and {b} trait counts differ")
+    return tr
+
+
+class SpectralSMDisk3D(_Component):
+    """Thick smooth disk (spectral_smdisk_3d.py:62-129)."""
+
+    def __init__(self, loose, tilted, bptraits, vptraits, dptraits, bhtraits,
+                 vhtraits=None, dhtraits=None, zptraits=None, sptraits=None,
+                 wptraits=None, rnmin=None, rnmax=None, rnsep=None,
+                 rnlen=None, rnodes=None, rstep=None, interp='linear'):
+        if interp != 'linear':
+            raise RuntimeError("only linear interpolation is supported")
+        rn, rs = parse_rnodes(rnmin, rnmax, rnsep, rnlen, rnodes, rstep)
+        self.disk = SMDisk(loose, tilted, rn, rs, _thick_traits(
+            bptraits, bhtraits, vptraits, vhtraits, dptraits, dhtraits,
+            zptraits, sptraits, wptraits))
+        self.thin = False
+
+    def evaluate(self, driver, params, odata, scube, wdata, bdata, obdata,
+                 spat_size, spat_step, spat_zero, spat_rota, spec_size,
+                 spec_step, spec_zero, dtype, out_extra=None):
+        self.disk.evaluate(driver, params, odata, None, scube, wdata, bdata,
+                           obdata, spat_size, spat_step, spat_zero, spat_rota,
+                           spec_size, spec_step, spec_zero, dtype, out_extra)
+
+
+class SpectralMCDisk3D(_Component):
+    """Monte Carlo cloud disk (spectral_mcdisk_3d.py:14-133)."""
+
+    def __init__(self, cflux, loose, tilted, bptraits, vptraits, dptraits,
+                 bhtraits, vhtraits=None, dhtraits=None, zptraits=None,
+                 sptraits=None, wptraits=None, rnmin=None, rnmax=None,
+                 rnsep=None, rnlen=None, rnodes=None, rstep=None,
+                 interp='linear'):
+        if interp != 'linear':
+            raise RuntimeError("only linear interpolation is supported")
+        rn, rs = parse_rnodes(rnmin, rnmax, rnsep, rnlen, rnodes, rstep)
+        self.disk = MCDisk(cflux, loose, tilted, rn, rs, _thick_traits(
+            bptraits, bhtraits, vptraits, vhtraits, dptraits, dhtraits,
+            zptraits, sptraits, wptraits))
+        self.thin = False
+
+    def evaluate(self, driver, params, odata, scube, wdata, bdata, obdata,
+                 spat_size, spat_step, spat_zero, spat_rota, spec_size,
+                 spec_step, spec_zero, dtype, out_extra=None):
+        self.disk.evaluate(driver, params, odata, None, scube, wdata, bdata,
+                           obdata, spat_size, spat_step, spat_zero, spat_rota,
+                           spec_size, spec_step, spec_zero, dtype, out_extra)
+
+
+def _merge_params(components):
+    """Component parameters, 'cmp<i>_'-prefixed after the first
+    (gmodels/_detail.py:320-342)."""
+    merged, maps = {}, []
+    for i, c in enumerate(components):
+        m = {}
+        for n, size in c.pdescs().items():
+            full = f'cmp{i}_{n}' if i else n
+            m[n] = full
+            merged[full] = size
+        maps.append(m)
+    return merged, maps
+
+
+class GModelKinematics2D:
+    """kinematics_2d.py: thin components on the data model's spatial grid."""
+
+    def __init__(self, components):
+        self.components = tuple(components) if isinstance(
+            components, (list, tuple)) else (components,)
+        self._pdescs, self._maps = _merge_params(self.components)
+        self._wdata = None
+
+    def pdescs(self):
+        return dict(self._pdescs)
+
+    def has_weights(self):
+        return any(c.has_weights() for c in self.components)
+
+    def evaluate_scube(self, driver, params, scube_d, scube_w, size, step,
+                       zero, rota, dtype, out_extra=None):
+        spat_size, spat_step, spat_zero = size[:2], step[:2], zero[:2]
+        if scube_w is not None and self._wdata is None:
+            self._wdata = driver.mem_alloc_d(tuple(spat_size)[::-1], dtype)
+        for c, m in zip(self.components, self._maps):
+            cp = {n: params[m[n]] for n in c.pdescs()}
+            c.evaluate(driver, cp, scube_d, self._wdata, None, spat_size,
+                       spat_step, spat_zero, rota, size[2], step[2], zero[2],
+                       dtype, out_extra)
+        if scube_w is not None:
+            driver.backends().gmodel(dtype).wcube_evaluate(
+                tuple(spat_size) + (1,), size[2], self._wdata, scube_w)
+
+
+class GModelKinematics3D:
+    """kinematics_3d.py:28-187: thick components on a 3-D spatial grid whose
+    z extent defaults to the larger spatial axis (:104-109)."""
+
+    def __init__(self, components, size_z=None, step_z=None, zero_z=None):
+        self.components = tuple(components) if isinstance(
+            components, (list, tuple)) else (components,)
+        self._zspec = (size_z, step_z, zero_z)
+        self._pdescs, self._maps = _merge_params(self.components)
+        self._grid = None
+        self._wdata = None
+
+    def pdescs(self):
+        return dict(self._pdescs)
+
+    def has_weights(self):
+        return any(c.has_weights() for c in self.components)
+
+    def spatial_grid(self, size, step, zero):
+        sz, stz, zz = self._zspec
+        wide = int(size[0] < size[1])
+        if sz is None:
+            sz = size[wide]
+        if stz is None:
+            stz = step[wide]
+        if zz is None:
+            zz = -(sz / 2 - 0.5) * stz
+        return ((int(size[0]), int(size[1]), int(sz)),
+                (step[0], step[1], stz), (zero[0], zero[1], zz))
+
+    def evaluate_scube(self, driver, params, scube_d, scube_w, size, step,
+                       zero, rota, dtype, out_extra=None):
+        spat_size, spat_step, spat_zero = self.spatial_grid(size, step, zero)
+        if scube_w is not None and self._wdata is None:
+            self._wdata = driver.mem_alloc_d(spat_size[::-1], dtype)
+        for c, m in zip(self.components, self._maps):
+            cp = {n: params[m[n]] for n in c.pdescs()}
+            c.evaluate(driver, cp, None, scube_d, self._wdata, None, None,
+                       spat_size, spat_step, spat_zero, rota, size[2],
+                       step[2], zero[2], dtype, out_extra)
+        if scube_w is not None:
+            driver.backends().gmodel(dtype).wcube_evaluate(
+                spat_size, size[2], self._wdata, scube_w)

@@ -183,6 +183,65 @@ int gbkhip_count_pixels(gbkhip_ctx* ctx, const float* data1,
                         const float* data2, long long size, float epsilon,
                         int* counts);
 
+/* ---- Batched evaluation: no counterpart in the reference, which runs one
+ *      parameter set per Objective.log_likelihood call
+ *      (src/gbkfit/fitting/fitters/emcee/sampler.py:34-57).  Many sets of one
+ *      smdisk galaxy model share geometry, PSF x LSF spectrum and data; each
+ *      set is a flat float vector laid out as described below. ------------ */
+typedef struct {
+    int n;                   /* traits of this kind (0 when absent) */
+    const int* uids;         /* host */
+    const int* ccounts;      /* host */
+    const int* pcounts;      /* host */
+    int off_cvalues;         /* offset of this kind's constants in a set */
+    int off_pvalues;         /* offset of this kind's parameters in a set */
+} gbkhip_batch_traits;
+
+typedef struct {
+    int loose, tilted;
+    int nrnodes;
+    const float* rnodes;     /* host, sub-ring nodes */
+    int vsys_count;          /* 0 when the disk has no vsys */
+    int off_vsys, off_xpos, off_ypos, off_posa, off_incl;
+    gbkhip_batch_traits traits[9];   /* rp rh vp vh dp dh zp sp wp */
+    int pstride;             /* floats per parameter set */
+    gbkhip_grid grid;        /* gmodel grid; must cover the padded cube */
+    int size_lo[3];          /* low-res data cube (x, y, z) */
+    int size_hi[3];          /* padded hi-res cube */
+    int scale[3];
+    int edge_hi[3];          /* downscale offset (DCube edge) */
+    int has_conv;            /* PSF and/or LSF present */
+    const float* pcube;      /* device, padded PSF x LSF cube, centre at 0 */
+    const float* obs;        /* device, observed low-res cube */
+    const float* err;        /* device error cube or NULL */
+    const float* mask;       /* device mask cube or NULL */
+    float err_scalar;        /* error used when err is NULL (1: none) */
+    float weight;
+    int max_batch;           /* sets evaluated per chunk (buffer sizing) */
+} gbkhip_batch_desc;
+
+typedef struct gbkhip_batch gbkhip_batch;
+
+/* Builds the plan: copies the layout, computes the kernel spectrum and sizes
+ * the chunk buffers (max_batch padded cubes and spectra). */
+int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
+                        gbkhip_batch** out);
+int gbkhip_batch_destroy(gbkhip_batch* batch);
+/* chi2[s] (device, FP64) = sum of squared residuals of set s; cubes (device,
+ * nsets x low-res cube) receives the model cubes when not NULL.  With
+ * use_graph the launch sequence for these pointers is captured once in a
+ * HIP graph and replayed on later calls. */
+int gbkhip_batch_evaluate(gbkhip_batch* batch, const float* params,
+                          int nsets, double* chi2, float* cubes,
+                          int use_graph);
+/* Stage timing: with timing on, evaluate() records events around the smdisk,
+ * convolution and downscale+chi^2 stages of every chunk (inside captured
+ * graphs too); stage_times() waits for them and returns the summed stage
+ * durations (ms[3], milliseconds) of the last evaluate() and its chunk
+ * count. */
+int gbkhip_batch_set_timing(gbkhip_batch* batch, int enable);
+int gbkhip_batch_stage_times(gbkhip_batch* batch, double* ms, int* chunks);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,8 +25,8 @@
 extern "C" {
 #endif
 
-#define OR_AMB_EPS_R  1e-5   /* relative margin on radius thresholds */
-#define OR_AMB_EPS_CH 2e-3   /* margin on rint() arguments (grid units) */
+#define OR_AMB_EPS_R  2e-6   /* relative margin on radius thresholds */
+#define OR_AMB_EPS_CH 5e-4   /* margin on rint() arguments (grid units) */
 
 typedef struct {
     int n;                  /* number of traits of this kind */

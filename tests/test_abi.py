@@ -27,6 +27,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_python_binding_covers_declarations():
     from gbkfit_amd import _native
+    import gbkfit_amd.batch  # noqa: F401  (registers the batch entry points)
     _native.lib()
     bound = set(_native.exported_symbols())
     assert declared() <= bound, sorted(declared() - bound)
