@@ -173,7 +173,6 @@ def main():
     ev = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock, error=0.01,
                         max_batch=args.chunk, template_params=truth)
     use_graph = not args.no_graph
-    ev.set_timing(True)
 
     def step():
         host = ev.packer.pack_arrays(arrays, args.nsets)
@@ -184,19 +183,33 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    lat, stage_ms = [], np.zeros(3)
-    chunks = 0
+    lat = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ts = time.perf_counter()
         chi2 = step()
         lat.append(time.perf_counter() - ts)
-        ms, chunks = ev.stage_times()
-        stage_ms += ms
     torch.cuda.synchronize()
     t_batched = time.perf_counter() - t0
     evals = args.nsets * args.steps
     value = evals / t_batched
+
+    # second timed region: the same steps launched directly with HIP events
+    # around each stage (events on the kernels' own stream)
+    ev.set_timing(True)
+    step()
+    torch.cuda.synchronize()
+    stage_ms = np.zeros(3)
+    chunks = 0
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        chi2_t = step()
+        ms, chunks = ev.stage_times()
+        stage_ms += ms
+    torch.cuda.synchronize()
+    t_timed = time.perf_counter() - t1
+    ev.set_timing(False)
+    assert np.array_equal(chi2_t, chi2)
 
     # plain drop-in path, one parameter set per call
     obj = Objective([dict(scube=Data(mock, error=np.full(mock.shape, 0.01,
@@ -249,6 +262,7 @@ def main():
         batched_over_plain=value / plain_value,
         batch_latency=percentiles(lat),
         per_eval_latency_batched_ms=1e3 * t_batched / evals,
+        direct_launch_evals_per_s=evals / t_timed,
         stages_ms_per_step=dict(smdisk=stage_ms[0] / args.steps,
                                 fft_convolution=stage_ms[1] / args.steps,
                                 downscale_chi2=stage_ms[2] / args.steps),

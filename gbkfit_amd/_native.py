@@ -4,6 +4,7 @@ The library is built in-tree by ``gbkfit_amd/csrc/build.py`` (or
 ``__graft_entry__.build()``).  Loading fails loudly when it is missing: there
 is no CPU fallback behind this package.
 """
+import atexit
 import ctypes
 import os
 
@@ -95,6 +96,19 @@ _SIGS = {
 }
 
 _LIB = None
+_SHUTTING_DOWN = [False]
+
+
+@atexit.register
+def _mark_shutdown():
+    # Native handles are not released during interpreter shutdown: the
+    # order in which Python finalises objects then is arbitrary, and the
+    # process is about to release every device resource anyway.
+    _SHUTTING_DOWN[0] = True
+
+
+def shutting_down():
+    return _SHUTTING_DOWN[0]
 
 
 def lib():
@@ -166,9 +180,9 @@ class Context:
         call('gbkhip_ctx_synchronize', self.handle)
 
     def close(self):
-        if self.handle:
+        if self.handle and not shutting_down():
             lib().gbkhip_ctx_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:

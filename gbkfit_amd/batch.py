@@ -229,7 +229,9 @@ class BatchEvaluator:
         self.size_lo = tuple(cube.size())
         self.size_hi = tuple(size_hi)
         h = _c.c_void_p()
-        N.call('gbkhip_batch_create', driver.context().handle,
+        # the batch refers to the context: keep it alive as long as we are
+        self._ctx = driver.context()
+        N.call('gbkhip_batch_create', self._ctx.handle,
                _c.byref(d), _c.byref(h))
         self.handle = h
         self._params_dev = None
@@ -289,9 +291,9 @@ class BatchEvaluator:
         return tuple(ms), n.value
 
     def close(self):
-        if getattr(self, 'handle', None):
+        if getattr(self, 'handle', None) and not N.shutting_down():
             N.lib().gbkhip_batch_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:

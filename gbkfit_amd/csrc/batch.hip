@@ -193,6 +193,7 @@ __global__ void batch_chi2_final(const double* __restrict__ partials,
 
 struct Batch {
     Context* ctx;
+    int device = 0;
     gbkhip_batch_desc desc;
     gbkhip_disk tmpl;
     SetLayout lay;
@@ -390,6 +391,7 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
         auto* b = new gbkhip_batch();
         Batch& B = b->impl;
         B.ctx = c;
+        B.device = c->device;
         B.desc = D;
         try {
             build_layout(B);
@@ -428,9 +430,12 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
 
 int gbkhip_batch_destroy(gbkhip_batch* b)
 {
+    // Touches neither the context (it may already be gone) nor its stream:
+    // the graph executables and buffers are released on the batch's device,
+    // and hipFree waits for outstanding work itself.
     return guarded([&] {
         if (!b) return;
-        (void)hipStreamSynchronize(b->impl.ctx->stream);
+        (void)hipSetDevice(b->impl.device);
         delete b;
     });
 }
@@ -446,12 +451,11 @@ int gbkhip_batch_evaluate(gbkhip_batch* b, const float* params, int nsets,
         GBK_CHECK(nsets >= 0, GBKHIP_ERR_ARG, "negative set count");
         if (nsets == 0) return;
         GBK_CHECK(params && chi2, GBKHIP_ERR_ARG, "null params or chi2");
-        if (!use_graph) {
+        if (!use_graph || B.timing) {   // stage events need direct launches
             run_all(B, params, nsets, chi2, cubes);
             return;
         }
-        const auto key = std::make_tuple((uintptr_t)params,
-                                         nsets * 2 + (B.timing ? 1 : 0),
+        const auto key = std::make_tuple((uintptr_t)params, nsets,
                                          (uintptr_t)chi2, (uintptr_t)cubes);
         auto it = B.graphs.find(key);
         if (it == B.graphs.end()) {
@@ -481,8 +485,6 @@ int gbkhip_batch_evaluate(gbkhip_batch* b, const float* params, int nsets,
             it = B.graphs.emplace(key, exec).first;
         }
         GBK_HIP(hipGraphLaunch(it->second, c.stream));
-        B.timed_chunks = B.timing ?
-            (nsets + B.desc.max_batch - 1) / B.desc.max_batch : 0;
     });
 }
 

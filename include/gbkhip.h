@@ -234,9 +234,9 @@ int gbkhip_batch_destroy(gbkhip_batch* batch);
 int gbkhip_batch_evaluate(gbkhip_batch* batch, const float* params,
                           int nsets, double* chi2, float* cubes,
                           int use_graph);
-/* Stage timing: with timing on, evaluate() records events around the smdisk,
- * convolution and downscale+chi^2 stages of every chunk (inside captured
- * graphs too); stage_times() waits for them and returns the summed stage
+/* Stage timing: with timing on, evaluate() launches directly (no graph) and
+ * records events around the smdisk, convolution and downscale+chi^2 stages
+ * of every chunk; stage_times() waits for them and returns the summed stage
  * durations (ms[3], milliseconds) of the last evaluate() and its chunk
  * count. */
 int gbkhip_batch_set_timing(gbkhip_batch* batch, int enable);

@@ -35,7 +35,7 @@ def test_batch_matches_oracle(setup):
         ref, amb, _ = P.oracle_cube(p)
         assert P.excess_error(cubes[i], ref, amb) < 1e-5, i
         r = oracle.residual(mock, np.full(ref.shape, 0.01), None, ref, None,
-                            None, 1.0)
+                            None, 1.0).reshape(ref.shape)
         c_ref = oracle.residual_sum(r, True)
         # ambiguous voxels can move chi^2 by at most sum 2|r| amb / err^2
         slack = np.sum(2 * np.abs(r) * amb / 0.01 + (amb / 0.01) ** 2)

@@ -62,9 +62,16 @@ def test_smooth235_padding(drv):
 
 
 def test_c1_moment_maps(drv):
-    """Moment maps 0/1/2 compared on spaxels whose m0 is well above the
-    FP32 FFT noise floor (|m0| > 1e-3 max m0); the mask agrees wherever
-    |m0| is not within 1e-3 of the 1e-6 cutoff."""
+    """Moment maps 0/1/2 on spaxels with |m0| > 1e-3 max|m0|.
+
+    Error model: an FP32 FFT convolution leaves an absolute error of a few
+    FP32 ulps of the cube maximum in every voxel (the reference warns that
+    moments of FFT-convolved cubes pick up this noise, mmaps.py:84-93).  With
+    a per-voxel floor n = 4e-7 max|cube|, the moments move by at most
+      dm1 <= n dv sum_k |v_k - m1| / m0,
+      dm2 <= n dv sum_k (v_k - m1)^2 / (2 m0 m2),
+    which the test adds to a 1e-5 normwise tolerance.  The mask agrees
+    wherever |m0| is not within 1e-3 max|m0| of the 1e-6 cutoff."""
     p = P.c1_params()
     model, params = P.build_product(drv, p, orders=(0, 1, 2))
     out = model.model_h(params)
@@ -72,16 +79,26 @@ def test_c1_moment_maps(drv):
     geom = P.oracle_geom(p)
     md, mm, _ = oracle.moments(ref, geom.step_lo, geom.zero_lo, (0, 1, 2),
                                1e-6)
-    m0 = md[0]
+    m0, m1, m2 = md
     sel = np.abs(m0) > 1e-3 * np.nanmax(np.abs(m0))
+    dv = geom.step_lo[2]
+    v = geom.zero_lo[2] + dv * np.arange(ref.shape[0])
+    floor = 4e-7 * np.abs(ref).max()
+    dev = v[:, None, None] - m1[None]
+    bound = {
+        'mmap0': floor * dv * ref.shape[0] + 0 * m0,
+        'mmap1': floor * dv * np.abs(dev).sum(0) / np.abs(m0),
+        'mmap2': floor * dv * (dev ** 2).sum(0) / (2 * np.abs(m0) * m2),
+    }
     for i, key in enumerate(('mmap0', 'mmap1', 'mmap2')):
         g = out[key]['d'].astype(np.float64)
         r = md[i]
         assert np.all(np.isfinite(g[sel])), key
         scale = np.nanmax(np.abs(r[sel]))
-        err = np.max(np.abs(g[sel] - r[sel])) / scale
-        print(key, err)
-        assert err < 1e-4, key
+        excess = np.abs(g[sel] - r[sel]) - bound[key][sel]
+        print(key, np.max(np.abs(g[sel] - r[sel])) / scale,
+              np.max(excess) / scale)
+        assert np.max(excess) / scale < 1e-5, key
     gm = out['mmap0']['m']
     clear = np.abs(np.abs(m0) - 1e-6) > 1e-3 * np.nanmax(np.abs(m0))
     assert np.array_equal(gm[clear & sel], mm[clear & sel])
