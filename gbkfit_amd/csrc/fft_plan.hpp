@@ -13,11 +13,36 @@ struct Axis {
     int radix[kMaxStages] = {};   // radices in stage order
 };
 
+// Stage table of the register-tiled engine (lengths whose prime factors are
+// 2, 3 and 5).  ns[s] is the product of the radices before stage s; nsm[s]
+// is the magic multiplier of a division by ns[s].
+constexpr int kFMax = 16;
+
+struct FAxis {
+    int n = 1, nst = 0;
+    int R[kFMax] = {};
+    int ns[kFMax] = {};
+    int nb[kFMax] = {};           // n / R[s], butterflies per line
+    int tws[kFMax] = {};          // n / (ns[s] R[s]), twiddle index step
+    unsigned nsm[kFMax] = {};
+};
+
+// Launch shape of one axis on the register-tiled engine: lshift = log2 of
+// the lines per block, emax = elements held per thread (template bucket).
+struct FastShape {
+    bool ok = false;
+    FAxis a;
+    int lshift = 0;
+    int emax = 0;
+    size_t lds = 0;
+};
+
 struct FFTPlan {
     int nx, ny, nz, nxh;
     bool even;                    // nx even: half-length row transform
     Axis ax;                      // row transform (nx/2 or nx)
     Axis ay, az;
+    FastShape fx, fy, fz;         // register-tiled engine, when applicable
     DevBuf twx, twy, twz;         // twiddle tables of the three transforms
     DevBuf twpost;                // exp(-2 pi i k / nx), k = 0..nx/2
 };

@@ -82,8 +82,10 @@ def mcdisk(drv, spec, grid, cflux, csum, hoi, seed, want=('scube',)):
 
 def normwise(a, ref):
     """max|a - ref| / max|ref| over finite entries of ref."""
-    ref = np.asarray(ref, np.float64)
-    a = np.asarray(a, np.float64)
+    cplx = np.iscomplexobj(ref) or np.iscomplexobj(a)
+    dt = np.complex128 if cplx else np.float64
+    ref = np.asarray(ref, dt)
+    a = np.asarray(a, dt)
     m = np.isfinite(ref)
     scale = np.max(np.abs(ref[m])) if m.any() else 0.0
     if scale == 0:

@@ -84,6 +84,30 @@ def test_fft_r2c_c2r(drv):
         assert np.array_equal(c.cpu().numpy(), got)
 
 
+@pytest.mark.parametrize('shape', [(128, 48, 320), (30, 300, 50),
+                                   (9, 40, 1024), (2, 1536, 6),
+                                   (3, 2048, 4), (5, 17, 34), (7, 96, 192)])
+def test_fft_sizes_all_radices(drv, shape):
+    """Lengths that exercise every radix (16, 8, 4, 2, 3, 5), every
+    lines-per-block and register bucket of the tiled engine, and the generic
+    fallback (2048 lines, prime 17)."""
+    be = drv.backends().fft(np.float32)
+    rng = np.random.default_rng(sum(shape))
+    a = rng.standard_normal(shape).astype(np.float32)
+    k = rng.standard_normal(shape).astype(np.float32)
+    d = H.dev(drv, a)
+    c = drv.mem_alloc_d(be.fft_complex_shape(list(shape)), np.complex64)
+    be.fft_r2c(d, c)
+    assert H.normwise(c.cpu().numpy(), np.fft.rfftn(a.astype(np.float64))) \
+        < 1e-5, shape
+    back = drv.mem_alloc_d(shape, np.float32)
+    be.fft_c2r(c, back)
+    assert H.normwise(back.cpu().numpy() / a.size, a) < 1e-5, shape
+    dk = H.dev(drv, k)
+    be.fft_convolve_cached(d, dk)
+    assert H.normwise(d.cpu().numpy(), oracle.fft_convolve(a, k)) < 1e-5, shape
+
+
 def test_fft_convolve_matches_closed_form(drv):
     be = drv.backends().fft(np.float32)
     rng = np.random.default_rng(5)
