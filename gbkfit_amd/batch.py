@@ -38,7 +38,7 @@ class BDesc(_c.Structure):
                 ('pcube', _c.c_void_p), ('obs', _c.c_void_p),
                 ('err', _c.c_void_p), ('mask', _c.c_void_p),
                 ('err_scalar', _c.c_float), ('weight', _c.c_float),
-                ('max_batch', _c.c_int)]
+                ('max_batch', _c.c_int), ('path', _c.c_int)]
 
 
 N.register_signatures({
@@ -47,6 +47,7 @@ N.register_signatures({
     'gbkhip_batch_destroy': ([_c.c_void_p], _c.c_int),
     'gbkhip_batch_evaluate': ([_c.c_void_p, _c.c_void_p, _c.c_int,
                                _c.c_void_p, _c.c_void_p, _c.c_int], _c.c_int),
+    'gbkhip_batch_info': ([_c.c_void_p, _c.POINTER(_c.c_int)], _c.c_int),
     'gbkhip_batch_set_timing': ([_c.c_void_p, _c.c_int], _c.c_int),
     'gbkhip_batch_stage_times': ([_c.c_void_p, _c.POINTER(_c.c_double),
                                   _c.POINTER(_c.c_int)], _c.c_int),
@@ -153,7 +154,8 @@ class BatchEvaluator:
     """
 
     def __init__(self, driver, dmodel, gmodel, obs, error=None, mask=None,
-                 weight=1.0, max_batch=64, template_params=None):
+                 weight=1.0, max_batch=64, template_params=None,
+                 path='auto'):
         self.driver = driver
         comps = gmodel.components
         if len(comps) != 1 or not hasattr(comps[0].disk, 'traits') or \
@@ -225,6 +227,9 @@ class BatchEvaluator:
             d.mask = self._mask.data_ptr()
         d.weight = float(weight)
         d.max_batch = int(max_batch)
+        if path not in ('auto', 'padded'):
+            raise ValueError("path must be 'auto' or 'padded'")
+        d.path = 0 if path == 'auto' else 1
         self.desc = d
         self.size_lo = tuple(cube.size())
         self.size_hi = tuple(size_hi)
@@ -278,6 +283,14 @@ class BatchEvaluator:
         if want_cubes:
             return chi2, self._cubes_dev.cpu().numpy().copy()
         return chi2
+
+    def info(self):
+        """dict(fused=bool, fft_size=(x, y, z), nparts=int): whether the
+        spectral-first fused pipeline runs, and the FFT sizes in use."""
+        v = (_c.c_int * 5)()
+        N.call('gbkhip_batch_info', self.handle, v)
+        return dict(fused=bool(v[0]), fft_size=(v[1], v[2], v[3]),
+                    nparts=v[4])
 
     def set_timing(self, enable):
         N.call('gbkhip_batch_set_timing', self.handle, int(bool(enable)))

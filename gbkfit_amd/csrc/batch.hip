@@ -14,10 +14,12 @@
 // a given (chunk size, pointers) is captured once in a HIP graph and
 // replayed.
 #include "fft_plan.hpp"
+#include "fft_engine.hpp"
 #include "launch.hpp"
 #include "disk.hpp"
 
 #include <climits>
+#include <cmath>
 #include <cstring>
 
 namespace gbkhip {
@@ -38,13 +40,23 @@ __device__ __forceinline__ gbkhip_disk set_disk(const gbkhip_disk& t,
     d.ypos = p + L.ypos;
     d.posa = p + L.posa;
     d.incl = p + L.incl;
-    gbkhip_traits* tr[9] = {&d.rp, &d.rh, &d.vp, &d.vh, &d.dp, &d.dh, &d.zp,
-                            &d.sp, &d.wp};
-    for (int k = 0; k < 9; ++k) {
-        if (!tr[k]->uids) continue;
-        tr[k]->cvalues = p + L.cvalues[k];
-        tr[k]->pvalues = p + L.pvalues[k];
-    }
+    // one statement per slot: no member addresses in a runtime-indexed
+    // array, so the struct stays in registers
+    auto bind = [&](gbkhip_traits& t, int k) {
+        if (t.uids) {
+            t.cvalues = p + L.cvalues[k];
+            t.pvalues = p + L.pvalues[k];
+        }
+    };
+    bind(d.rp, 0);
+    bind(d.rh, 1);
+    bind(d.vp, 2);
+    bind(d.vh, 3);
+    bind(d.dp, 4);
+    bind(d.dh, 5);
+    bind(d.zp, 6);
+    bind(d.sp, 7);
+    bind(d.wp, 8);
     return d;
 }
 
@@ -191,6 +203,306 @@ __global__ void batch_chi2_final(const double* __restrict__ partials,
     if (threadIdx.x == 0) out[set0 + set] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
+// ---------------------------------------------------------------------------
+// Spectral-first fused pipeline (thin disk, separable PSF x LSF kernel)
+//
+// The convolution is a product of three 1-D transforms, so their order is
+// free.  Taking the spectral axis first lets each spaxel's line go straight
+// into the z transform, and the inverse z transform
+// hand the convolved line straight to downscale + chi^2: the hi-res cube
+// never reaches HBM.  Per chunk of sets, with S the (nz/2+1, ny, nx)
+// complex work spectrum of each set:
+//   K0 batch_lines_thin  (flux, v, sigma) of each spaxel's line
+//   K1 fz_deposit_r2c    lines of 2^lshift spaxels into an LDS tile, real
+//                        FFT along z, half spectrum -> S
+//   K2 fx_lines (y)      forward along y
+//   K3 fx_rows_conv      forward along x, times Lhat[kz] Phat[ky, kx] (the
+//                        1/N scale folded in), inverse along x; only the x
+//                        range read by the downscale is stored
+//   K4 fx_lines (y)      inverse along y over those x; only the needed y
+//                        range is stored
+//   K5 fz_c2r_down_chi2  inverse real FFT along z of a tile of spaxel
+//                        columns, block-mean downscale, residual, FP64
+//                        partial sum of r^2 per tile
+// Inputs beyond S_i + 2 edge_i along each axis (S_i the supersampled data
+// size) reach no stored output through a kernel of half-width edge_i, so
+// they are zero and the FFT sizes are the smallest 5-smooth ones above it.
+// ---------------------------------------------------------------------------
+// Line of sight of each spaxel (thin disk): (flux, v, sigma) of the line it
+// deposits, flux 0 when it deposits none or lies beyond the needed region.
+__global__ __launch_bounds__(256) void batch_lines_thin(
+    gbkhip_disk tmpl, SetLayout lay, const float* __restrict__ params,
+    int pstride, gbkhip_grid g, int need_x, int need_y, int nx, int ny,
+    float4* __restrict__ lines, long long lstride)
+{
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    const int set = blockIdx.z;
+    if (x >= nx || y >= ny) return;
+    float4 out = make_float4(0, 0, 1, 0);
+    if (x < need_x && y < need_y) {
+        const gbkhip_disk d = set_disk(tmpl, lay,
+                                       params + (size_t)set * pstride);
+        const bool need_theta = disk_needs_theta(d);
+        const NodeGeom ng = node_geom(d, 0);
+        const VoxelLine V = smdisk_voxel(d, g, ng, x, y, 0, need_theta, false);
+        if (V.on) out = make_float4(V.orvalue, V.v, V.s, 0);
+    }
+    lines[set * lstride + (size_t)y * nx + x] = out;
+}
+
+// Spectral lines of L spaxels into an LDS tile, real FFT along z, half
+// spectrum -> S.
+template <int LSH, int T>
+__global__ __launch_bounds__(T) void fz_deposit_r2c(
+    const float4* __restrict__ lines, long long lstride, gbkhip_grid g,
+    FAxis a, const float2* __restrict__ tw, const float2* __restrict__ tpz,
+    int need_z, int nx, int ny, float2* __restrict__ S, long long sstride)
+{
+    extern __shared__ float2 lds[];
+    constexpr int L = 1 << LSH, LS = L + 1;
+    const int n = a.n;
+    float2* s_tw = lds;
+    float2* buf = lds + n;
+    float2* s_post = buf + n * LS;      // tpz staged in LDS
+    const int x0 = blockIdx.x << LSH, y = blockIdx.y, set = blockIdx.z;
+    for (int i = threadIdx.x; i < n; i += T) s_tw[i] = tw[i];
+    for (int i = threadIdx.x; i <= n; i += T) s_post[i] = tpz[i];
+    for (int i = threadIdx.x; i < n * LS; i += T) buf[i] = make_float2(0, 0);
+    __syncthreads();
+    // spaxel l of the tile; the T / L threads of a spaxel split its channels
+    const int l = threadIdx.x & (L - 1);
+    const int part = threadIdx.x >> LSH;
+    constexpr int nparts = T >> LSH;
+    const int x = x0 + l;
+    if (x < nx) {
+        const float4 V = lines[set * lstride + (size_t)y * nx + x];
+        if (V.x != 0.0f) {
+            int k0, k1;
+            line_window(g, V.y, V.z, k0, k1);
+            k1 = min(k1, need_z - 1);
+            const LineEval e = line_eval(g, V.x, V.y, V.z);
+            float* fb = reinterpret_cast<float*>(buf);
+            for (int k = k0 + part; k <= k1; k += nparts)
+                fb[2 * ((k >> 1) * LS + l) + (k & 1)] = e.at(k);
+        }
+    }
+    __syncthreads();
+    // z[m] = x[2m] + i x[2m+1]; complex FFT of length n = nz/2, then
+    // X[k] = (Z[k] + conj Z[n-k]) / 2 + W^k (Z[k] - conj Z[n-k]) / (2i)
+    fast_fft<LSH, T>(buf, a, s_tw, -1.0f);
+    float2* out = S + set * sstride + (size_t)y * nx + x0;
+    const size_t plane = (size_t)nx * ny;
+    const int cnt = (n + 1) << LSH;
+    for (int i = threadIdx.x; i < cnt; i += T) {
+        const int ll = i & (L - 1), k = i >> LSH;
+        if (x0 + ll >= nx) continue;
+        const float2 zk = buf[(k == n ? 0 : k) * LS + ll];
+        const float2 zm = conjf2(buf[(k == 0 ? 0 : n - k) * LS + ll]);
+        const float2 fe = make_float2(0.5f * (zk.x + zm.x),
+                                      0.5f * (zk.y + zm.y));
+        const float2 dd = csub(zk, zm);
+        const float2 fo = make_float2(0.5f * dd.y, -0.5f * dd.x);
+        out[k * plane + ll] = cadd(fe, cmul(s_post[k], fo));
+    }
+}
+
+// Rows of S along x: forward FFT, kernel spectrum, inverse FFT.  Row r is
+// plane kz = r / ny, line ky = r % ny; element i of the block's contiguous
+// tile is (row i / n, x = i % n).
+template <int LSH, int T>
+__global__ __launch_bounds__(T) void fx_rows_conv(
+    float2* __restrict__ S, long long sstride, int nrows, int ny,
+    unsigned ny_m, FAxis a, unsigned n_m, const float2* __restrict__ tw,
+    const float2* __restrict__ phat, const float2* __restrict__ lhat,
+    int x_lo, int x_cnt, unsigned xcnt_m)
+{
+    extern __shared__ float2 lds[];
+    constexpr int L = 1 << LSH, LS = L + 1;
+    const int n = a.n;
+    float2* s_tw = lds;
+    float2* buf = lds + n;
+    const int r0 = blockIdx.x << LSH;
+    const int nl = min(L, nrows - r0);
+    float2* rows = S + blockIdx.y * sstride + (size_t)r0 * n;
+    for (int i = threadIdx.x; i < n; i += T) s_tw[i] = tw[i];
+    const int tile = n << LSH, valid = n * nl;
+    float2 r[kFastE];
+#pragma unroll
+    for (int c = 0; c < kFastE; ++c) {
+        const int i = threadIdx.x + c * T;
+        r[c] = i < valid ? rows[i] : make_float2(0, 0);
+    }
+#pragma unroll
+    for (int c = 0; c < kFastE; ++c) {
+        const int i = threadIdx.x + c * T;
+        if (i < tile) {
+            const int l = (int)fdiv((unsigned)i, n_m), m = i - l * n;
+            buf[m * LS + l] = r[c];
+        }
+    }
+    __syncthreads();
+    fast_fft<LSH, T>(buf, a, s_tw, -1.0f);
+#pragma unroll
+    for (int c = 0; c < kFastE; ++c) {
+        const int i = threadIdx.x + c * T;
+        r[c] = make_float2(0, 0);
+        if (i < valid) {
+            const int l = (int)fdiv((unsigned)i, n_m), m = i - l * n;
+            const int row = r0 + l;
+            const int kz = (int)fdiv((unsigned)row, ny_m);
+            r[c] = cmul(lhat[kz], phat[(size_t)(row - kz * ny) * n + m]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < kFastE; ++c) {
+        const int i = threadIdx.x + c * T;
+        if (i < valid) {
+            const int l = (int)fdiv((unsigned)i, n_m), m = i - l * n;
+            buf[m * LS + l] = cmul(buf[m * LS + l], r[c]);
+        }
+    }
+    __syncthreads();
+    fast_fft<LSH, T>(buf, a, s_tw, 1.0f);
+    const int cnt = nl * x_cnt;
+    for (int i = threadIdx.x; i < cnt; i += T) {
+        const int l = (int)fdiv((unsigned)i, xcnt_m), m = x_lo + i - l * x_cnt;
+        rows[(size_t)l * n + m] = buf[m * LS + l];
+    }
+}
+
+// Inverse real FFT along z of a tile of spaxel columns, then block-mean
+// downscale, residual and FP64 partial sum of r^2.  The tile is kx x ky
+// low-res spaxels = tx x ty hi-res columns (tx = kx sx, ty = ky sy), column
+// (lx, ly) on line l = ly * tx + lx.
+template <int LSH, int T>
+__global__ __launch_bounds__(T) void fz_c2r_down_chi2(
+    const float2* __restrict__ S, long long sstride, int nx, int ny, FAxis a,
+    const float2* __restrict__ tw, const float2* __restrict__ tpz, int x_lo,
+    int y_lo, int kx, int ky, int sx, int sy, int sz, int ez, int nxl,
+    int nyl, int nzl, int tiles_x, DataArgs da, float* __restrict__ lo_out,
+    long long lstride, double* __restrict__ partials, int nparts)
+{
+    extern __shared__ float2 lds[];
+    __shared__ double red[T / 64];
+    constexpr int L = 1 << LSH, LS = L + 1;
+    const int n = a.n;
+    const int tx = kx * sx, ty = ky * sy;
+    float2* s_tw = lds;
+    float2* buf = lds + n;
+    float2* s_post = buf + n * LS;      // tpz staged in LDS
+    const int set = blockIdx.y;
+    const int X0 = (blockIdx.x % tiles_x) * kx, Y0 = (blockIdx.x / tiles_x) * ky;
+    const float2* src = S + set * sstride;
+    const size_t plane = (size_t)nx * ny;
+    for (int i = threadIdx.x; i < n; i += T) s_tw[i] = tw[i];
+    for (int i = threadIdx.x; i <= n; i += T) s_post[i] = tpz[i];
+    __syncthreads();
+    // Z[k] = (X[k] + conj X[n-k]) + i W^-k (X[k] - conj X[n-k]), k < n
+    float2 xa[kFastE], xb[kFastE];
+#pragma unroll
+    for (int c = 0; c < kFastE; ++c) {
+        const int i = threadIdx.x + c * T;
+        const int l = i & (L - 1), k = i >> LSH;
+        const int lx = l % tx, ly = l / tx;
+        xa[c] = xb[c] = make_float2(0, 0);
+        if (k < n && ly < ty && X0 + lx / sx < nxl && Y0 + ly / sy < nyl) {
+            const size_t col = (size_t)(y_lo + Y0 * sy + ly) * nx +
+                               (x_lo + X0 * sx + lx);
+            xa[c] = src[k * plane + col];
+            xb[c] = src[(n - k) * plane + col];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < kFastE; ++c) {
+        const int i = threadIdx.x + c * T;
+        const int l = i & (L - 1), k = i >> LSH;
+        if (k < n) {
+            const float2 xk = xa[c];
+            const float2 xm = conjf2(xb[c]);
+            const float2 fe = cadd(xk, xm);
+            const float2 fo = cmul(csub(xk, xm), conjf2(s_post[k]));
+            buf[k * LS + l] = cadd(fe, make_float2(-fo.y, fo.x));
+        }
+    }
+    __syncthreads();
+    fast_fft<LSH, T>(buf, a, s_tw, 1.0f);
+    const float* fb = reinterpret_cast<const float*>(buf);
+    const float nf = 1.0f / (float)(sx * sy * sz);
+    const int per_z = kx * ky;
+    const int nvox = per_z * nzl;
+    double acc = 0.0;
+    // voxels v = threadIdx.x + j T, kV at a time: their data loads are all
+    // issued before the first is used
+    constexpr int kV = 8;
+    for (int v0 = 0; v0 < nvox; v0 += kV * T) {
+        long long gi[kV];
+        float ob[kV], er[kV], mk[kV];
+#pragma unroll
+        for (int j = 0; j < kV; ++j) {
+            const int v = v0 + threadIdx.x + j * T;
+            gi[j] = -1;
+            if (v < nvox) {
+                const int zl = v / per_z, rem = v - zl * per_z;
+                const int lyo = rem / kx, lxo = rem - lyo * kx;
+                const int X = X0 + lxo, Y = Y0 + lyo;
+                if (X < nxl && Y < nyl)
+                    gi[j] = ((long long)zl * nyl + Y) * nxl + X;
+            }
+            ob[j] = gi[j] >= 0 ? da.obs[gi[j]] : 0.0f;
+            er[j] = gi[j] >= 0 && da.err ? da.err[gi[j]] : da.err_scalar;
+            mk[j] = gi[j] >= 0 && da.mask ? da.mask[gi[j]] : 1.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < kV; ++j) {
+            if (gi[j] < 0) continue;
+            const int v = v0 + threadIdx.x + j * T;
+            const int zl = v / per_z, rem = v - zl * per_z;
+            const int lyo = rem / kx, lxo = rem - lyo * kx;
+            float sum = 0.0f;
+            for (int dz = 0; dz < sz; ++dz) {
+                const int z = ez + zl * sz + dz;
+                for (int dy = 0; dy < sy; ++dy)
+                    for (int dx = 0; dx < sx; ++dx) {
+                        const int l = (lyo * sy + dy) * tx + lxo * sx + dx;
+                        sum += fb[2 * ((z >> 1) * LS + l) + (z & 1)];
+                    }
+            }
+            const float m = sum * nf;
+            if (lo_out) lo_out[set * lstride + gi[j]] = m;
+            // objective.hpp:7-33 with mdl_w = mdl_m = none
+            float r = m - ob[j];
+            r /= er[j];
+            if (da.mask) r *= mk[j];
+            r *= da.weight;
+            acc += (double)r * (double)r;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < T / 64; ++w) t += red[w];
+        partials[(size_t)set * nparts + blockIdx.x] = t;
+    }
+}
+
+struct Fused {
+    bool on = false;
+    int nx = 0, ny = 0, nz = 0, nzh = 0;   // FFT sizes (nz even)
+    int need[3] = {0, 0, 0};               // S_i + 2 edge_i
+    int xlo = 0, xcnt = 0, ylo = 0, ycnt = 0;
+    FastShape fz, fy, fx;                  // z columns, y lines, x rows
+    DevBuf twz, twy, twx, tpz;
+    DevBuf phat, lhat;
+    DevBuf S;
+    long long sstride = 0;
+    DevBuf lines;                          // per spaxel (flux, v, sigma)
+    long long lstride = 0;
+    int kx = 1, ky = 1, tiles_x = 1, tiles_y = 1;
+};
+
 struct Batch {
     Context* ctx;
     int device = 0;
@@ -201,6 +513,7 @@ struct Batch {
     DevBuf ints;                                      // device copy of them
     DevBuf rnodes;
     std::shared_ptr<FFTPlan> plan;
+    Fused fused;            // spectral-first pipeline, when it applies
     DevBuf spectrum;        // kernel spectrum (nz, ny, nxh) complex
     DevBuf hi;              // chunk x padded cube
     DevBuf spec;            // chunk x complex work spectrum
@@ -295,12 +608,261 @@ static void mark(Batch& b, int chunk, int i)
         GBK_HIP(hipEventRecord(b.events[4 * chunk + i], b.ctx->stream));
 }
 
+// Smallest 2^a 3^b 5^c >= n (even when asked).
+static int smooth5_at_least(int n, bool even)
+{
+    for (int m = std::max(n, 2);; ++m) {
+        if (even && (m & 1)) continue;
+        int r = m;
+        for (int p : {2, 3, 5})
+            while (r % p == 0) r /= p;
+        if (r == 1) return m;
+    }
+}
+
+// Decides whether the spectral-first pipeline applies and prepares it:
+// thin disk, convolution present, and a PSF x LSF cube that factors as
+// lsf(z) psf(y, x) with (up to FP32 rounding of the product) and whose
+// taps beyond the DCube edge carry no weight.  Kernel spectra are computed
+// on the host in FP64 from the taps within the edge:
+//   Phat[ky][kx] = sum psf(dy, dx) e^(-2 pi i (ky dy / ny + kx dx / nx))
+//   Lhat[kz]     = sum lsf(dz) e^(-2 pi i kz dz / nz)
+// with 1 / (sum pcube * nx ny nz) folded into Phat.
+static void plan_fused(Context& c, Batch& B)
+{
+    const gbkhip_batch_desc& D = B.desc;
+    Fused& F = B.fused;
+    F.on = false;
+    if (!B.thin || !D.has_conv || D.path == 1) return;
+    const int Nx = D.size_hi[0], Ny = D.size_hi[1], Nz = D.size_hi[2];
+    const size_t N = (size_t)Nx * Ny * Nz;
+    std::vector<float> pc(N);
+    GBK_HIP(hipMemcpy(pc.data(), D.pcube, N * sizeof(float),
+                      hipMemcpyDeviceToHost));
+    std::vector<double> psf((size_t)Nx * Ny, 0.0), lsf(Nz, 0.0);
+    double total = 0.0, amax = 0.0;
+    for (int z = 0; z < Nz; ++z)
+        for (size_t i = 0; i < (size_t)Nx * Ny; ++i) {
+            const double v = pc[(size_t)z * Nx * Ny + i];
+            psf[i] += v;
+            lsf[z] += v;
+            total += v;
+            amax = std::max(amax, std::fabs(v));
+        }
+    if (!(total > 0.0) || !std::isfinite(total)) return;
+    double worst = 0.0;
+    for (int z = 0; z < Nz; ++z)
+        for (size_t i = 0; i < (size_t)Nx * Ny; ++i)
+            worst = std::max(worst,
+                             std::fabs(pc[(size_t)z * Nx * Ny + i] -
+                                       lsf[z] * psf[i] / total));
+    if (worst > 1e-6 * amax) return;
+    // taps beyond the edge: at most 1e-9 of the absolute weight
+    auto dist = [](int i, int n) { return std::min(i, n - i); };
+    const int ex = D.edge_hi[0], ey = D.edge_hi[1], ez = D.edge_hi[2];
+    double in_p = 0, out_p = 0, in_l = 0, out_l = 0;
+    for (int y = 0; y < Ny; ++y)
+        for (int x = 0; x < Nx; ++x) {
+            const double a = std::fabs(psf[(size_t)y * Nx + x]);
+            (dist(x, Nx) <= ex && dist(y, Ny) <= ey ? in_p : out_p) += a;
+        }
+    for (int z = 0; z < Nz; ++z)
+        (dist(z, Nz) <= ez ? in_l : out_l) += std::fabs(lsf[z]);
+    if (out_p > 1e-9 * in_p || out_l > 1e-9 * in_l) return;
+    for (int i = 0; i < 3; ++i)
+        F.need[i] = D.size_lo[i] * D.scale[i] + 2 * D.edge_hi[i];
+    // FFT sizes with engine shapes: x rows, y lines, z columns of 64 or 32
+    int nx = smooth5_at_least(F.need[0], false);
+    int ny = smooth5_at_least(F.need[1], false);
+    int nz = smooth5_at_least(F.need[2], true);
+    FastShape fx = fast_shape_rows(nx), fy = fast_shape(ny);
+    FastShape fz = fast_shape_lines(nz / 2, 6);
+    if (!fz.ok) fz = fast_shape_lines(nz / 2, 5);
+    if (!fx.ok || !fy.ok || !fz.ok) return;
+    const int L = 1 << fz.lshift;
+    const int sxy = D.scale[0] * D.scale[1];
+    if (sxy > L) return;
+    const int nzh = nz / 2 + 1;
+    if ((long long)nzh * ny * ny >= (1LL << 32) ||
+        (long long)nzh * nx * nx >= (1LL << 32))
+        return;
+    F.nx = nx;
+    F.ny = ny;
+    F.nz = nz;
+    F.nzh = nzh;
+    F.fx = fx;
+    F.fy = fy;
+    F.fz = fz;
+    F.xlo = ex;
+    F.xcnt = D.size_lo[0] * D.scale[0];
+    F.ylo = ey;
+    F.ycnt = D.size_lo[1] * D.scale[1];
+    F.kx = std::max(1, std::min(L / sxy, D.size_lo[0]));
+    F.ky = 1;
+    F.tiles_x = (D.size_lo[0] + F.kx - 1) / F.kx;
+    F.tiles_y = (D.size_lo[1] + F.ky - 1) / F.ky;
+    // kernel spectra
+    const double scale = 1.0 / (total * (double)nx * ny * nz);
+    const int wx = 2 * ex + 1, wy = 2 * ey + 1;
+    std::vector<double> A((size_t)wy * nx * 2, 0.0);
+    for (int dy = -ey; dy <= ey; ++dy) {
+        const int iy = (dy % Ny + Ny) % Ny;
+        for (int kx = 0; kx < nx; ++kx) {
+            double re = 0, im = 0;
+            for (int dx = -ex; dx <= ex; ++dx) {
+                const int ix = (dx % Nx + Nx) % Nx;
+                const double v = psf[(size_t)iy * Nx + ix];
+                if (v == 0.0) continue;
+                const long long m = (((long long)kx * dx) % nx + nx) % nx;
+                const double ang = -2.0 * M_PI * (double)m / nx;
+                re += v * std::cos(ang);
+                im += v * std::sin(ang);
+            }
+            A[2 * ((size_t)(dy + ey) * nx + kx)] = re;
+            A[2 * ((size_t)(dy + ey) * nx + kx) + 1] = im;
+        }
+    }
+    std::vector<float> ph((size_t)ny * nx * 2);
+    std::vector<double> cy(wy), sy(wy);
+    for (int ky = 0; ky < ny; ++ky) {
+        for (int dy = -ey; dy <= ey; ++dy) {
+            const long long m = (((long long)ky * dy) % ny + ny) % ny;
+            const double ang = -2.0 * M_PI * (double)m / ny;
+            cy[dy + ey] = std::cos(ang);
+            sy[dy + ey] = std::sin(ang);
+        }
+        for (int kx = 0; kx < nx; ++kx) {
+            double re = 0, im = 0;
+            for (int j = 0; j < wy; ++j) {
+                const double ar = A[2 * ((size_t)j * nx + kx)];
+                const double ai = A[2 * ((size_t)j * nx + kx) + 1];
+                re += ar * cy[j] - ai * sy[j];
+                im += ar * sy[j] + ai * cy[j];
+            }
+            ph[2 * ((size_t)ky * nx + kx)] = (float)(re * scale);
+            ph[2 * ((size_t)ky * nx + kx) + 1] = (float)(im * scale);
+        }
+    }
+    std::vector<float> lh((size_t)nzh * 2);
+    for (int kz = 0; kz < nzh; ++kz) {
+        double re = 0, im = 0;
+        for (int dz = -ez; dz <= ez; ++dz) {
+            const double v = lsf[(dz % Nz + Nz) % Nz];
+            const long long m = (((long long)kz * dz) % nz + nz) % nz;
+            const double ang = -2.0 * M_PI * (double)m / nz;
+            re += v * std::cos(ang);
+            im += v * std::sin(ang);
+        }
+        lh[2 * kz] = (float)re;
+        lh[2 * kz + 1] = (float)im;
+    }
+    (void)wx;
+    F.phat.alloc(ph.size() * sizeof(float));
+    GBK_HIP(hipMemcpy(F.phat.ptr, ph.data(), ph.size() * sizeof(float),
+                      hipMemcpyHostToDevice));
+    F.lhat.alloc(lh.size() * sizeof(float));
+    GBK_HIP(hipMemcpy(F.lhat.ptr, lh.data(), lh.size() * sizeof(float),
+                      hipMemcpyHostToDevice));
+    upload_twiddles(F.twz, nz / 2, nz / 2);
+    upload_twiddles(F.tpz, nz, nzh);
+    upload_twiddles(F.twy, ny, ny);
+    upload_twiddles(F.twx, nx, nx);
+    F.sstride = (long long)nzh * ny * nx;
+    F.S.alloc((size_t)D.max_batch * F.sstride * sizeof(float2));
+    F.lstride = (long long)ny * nx;
+    F.lines.alloc((size_t)D.max_batch * F.lstride * sizeof(float4));
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const void* ks[] = {GBK_FAST_KERNELS(fz_deposit_r2c),
+                            GBK_FAST_KERNELS(fx_rows_conv),
+                            GBK_FAST_KERNELS(fz_c2r_down_chi2)};
+        for (const void* k : ks)
+            (void)hipFuncSetAttribute(
+                k, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+    });
+    fft_prepare_attributes();
+    (void)c;
+    F.on = true;
+}
+
+static void run_chunk_fused(Batch& b, const float* params, int set0,
+                            int nsets, double* chi2, float* cubes, int ci)
+{
+    Context& c = *b.ctx;
+    const gbkhip_batch_desc& D = b.desc;
+    const Fused& F = b.fused;
+    const float* P = params + (size_t)set0 * D.pstride;
+    float2* S = F.S.as<float2>();
+    const long long plane = (long long)F.ny * F.nx;
+    {
+        float4* lines = F.lines.as<float4>();
+        dim3 g0(ceil_div(F.nx, 64), ceil_div(F.ny, 4), nsets);
+        hipLaunchKernelGGL(batch_lines_thin, g0, dim3(64, 4), 0, c.stream,
+                           b.tmpl, b.lay, P, D.pstride, D.grid, F.need[0],
+                           F.need[1], F.nx, F.ny, lines, F.lstride);
+        GBK_HIP(hipGetLastError());
+        dim3 grid(ceil_div(F.nx, 1 << F.fz.lshift), F.ny, nsets);
+        GBK_FAST_DISPATCH(F.fz, fz_deposit_r2c, grid,
+                          F.fz.lds + (F.fz.a.n + 1) * sizeof(float2),
+                          c.stream, lines, F.lstride, D.grid, F.fz.a,
+                          F.twz.as<float2>(), F.tpz.as<float2>(), F.need[2],
+                          F.nx, F.ny, S, F.sstride);
+        GBK_HIP(hipGetLastError());
+    }
+    mark(b, ci, 1);
+    launch_fx_lines(c, F.fy, F.twy.as<float2>(), S, S, nullptr,
+                    F.nzh * F.nx, F.nx, 0, plane, F.nx, 0, F.ny, 0, 1.0f,
+                    nsets, F.sstride, 0);
+    {
+        const int nrows = F.nzh * F.ny;
+        dim3 grid(ceil_div(nrows, 1 << F.fx.lshift), nsets);
+        GBK_FAST_DISPATCH(F.fx, fx_rows_conv, grid, F.fx.lds, c.stream, S,
+                          F.sstride, nrows, F.ny, div_magic((unsigned)F.ny),
+                          F.fx.a, div_magic((unsigned)F.nx),
+                          F.twx.as<float2>(), F.phat.as<float2>(),
+                          F.lhat.as<float2>(), F.xlo, F.xcnt,
+                          div_magic((unsigned)F.xcnt));
+        GBK_HIP(hipGetLastError());
+    }
+    launch_fx_lines(c, F.fy, F.twy.as<float2>(), S, S, nullptr,
+                    F.nzh * F.xcnt, F.xcnt, F.xlo, plane, F.nx, F.ylo,
+                    F.ylo + F.ycnt, 1, 1.0f, nsets, F.sstride, 0);
+    mark(b, ci, 2);
+    DataArgs da;
+    da.obs = D.obs;
+    da.err = D.err;
+    da.mask = D.mask;
+    da.err_scalar = D.err_scalar;
+    da.weight = D.weight;
+    {
+        dim3 grid(F.tiles_x * F.tiles_y, nsets);
+        GBK_FAST_DISPATCH(F.fz, fz_c2r_down_chi2, grid,
+                          F.fz.lds + (F.fz.a.n + 1) * sizeof(float2),
+                          c.stream, S, F.sstride, F.nx, F.ny, F.fz.a,
+                          F.twz.as<float2>(), F.tpz.as<float2>(), F.xlo,
+                          F.ylo, F.kx, F.ky, D.scale[0], D.scale[1],
+                          D.scale[2], D.edge_hi[2], D.size_lo[0],
+                          D.size_lo[1], D.size_lo[2], F.tiles_x, da,
+                          cubes ? cubes + (size_t)set0 * b.n_lo : nullptr,
+                          b.n_lo, b.partials.as<double>(), b.nparts);
+        GBK_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(batch_chi2_final, dim3(nsets), dim3(256), 0, c.stream,
+                       b.partials.as<double>(), b.nparts, chi2, set0);
+    GBK_HIP(hipGetLastError());
+    mark(b, ci, 3);
+}
+
 static void run_chunk(Batch& b, const float* params, int set0, int nsets,
                       double* chi2, float* cubes)
 {
     Context& c = *b.ctx;
     const int ci = set0 / b.desc.max_batch;
     mark(b, ci, 0);
+    if (b.fused.on) {
+        run_chunk_fused(b, params, set0, nsets, chi2, cubes, ci);
+        return;
+    }
     const gbkhip_batch_desc& D = b.desc;
     const gbkhip_grid& g = D.grid;
     const float* P = params + (size_t)set0 * D.pstride;
@@ -400,6 +962,17 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
             B.n_lo = (long long)D.size_lo[0] * D.size_lo[1] * D.size_lo[2];
             B.nparts = (int)std::max<long long>(
                 1, std::min<long long>(512, (B.n_lo + 4095) / 4096));
+            GBK_CHECK(D.path == 0 || D.path == 1, GBKHIP_ERR_ARG,
+                      "path must be 0 (automatic) or 1 (padded-cube FFT)");
+            plan_fused(*c, B);
+            if (B.fused.on) {
+                B.nparts = B.fused.tiles_x * B.fused.tiles_y;
+                B.partials.alloc((size_t)D.max_batch * B.nparts *
+                                 sizeof(double));
+                GBK_HIP(hipStreamSynchronize(c->stream));
+                *out = b;
+                return;
+            }
             B.hi.alloc((size_t)D.max_batch * B.n_hi * sizeof(float));
             B.partials.alloc((size_t)D.max_batch * B.nparts * sizeof(double));
             if (D.has_conv) {
@@ -485,6 +1058,20 @@ int gbkhip_batch_evaluate(gbkhip_batch* b, const float* params, int nsets,
             it = B.graphs.emplace(key, exec).first;
         }
         GBK_HIP(hipGraphLaunch(it->second, c.stream));
+    });
+}
+
+int gbkhip_batch_info(gbkhip_batch* b, int* info)
+{
+    return guarded([&] {
+        GBK_CHECK(b && info, GBKHIP_ERR_ARG, "null argument");
+        const Batch& B = b->impl;
+        const Fused& F = B.fused;
+        info[0] = F.on ? 1 : 0;
+        info[1] = F.on ? F.nx : B.desc.size_hi[0];
+        info[2] = F.on ? F.ny : B.desc.size_hi[1];
+        info[3] = F.on ? F.nz : B.desc.size_hi[2];
+        info[4] = B.nparts;
     });
 }
 

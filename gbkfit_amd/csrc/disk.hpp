@@ -336,7 +336,9 @@ __device__ __forceinline__ void p_traits(const gbkhip_traits& t,
 {
     const float* c = t.cvalues;
     const float* q = t.pvalues;
-    for (int i = 0; i < t.n && i < kTraitMax; ++i) {
+#pragma unroll
+    for (int i = 0; i < kTraitMax; ++i) {
+        if (i >= t.n) break;
         out[i] = trait_eval<KIND>(t.uids[i], c, q, p);
         c += t.ccounts[i];
         q += t.pcounts[i];
@@ -348,7 +350,9 @@ __device__ __forceinline__ void rh_traits(const gbkhip_traits& t,
 {
     const float* c = t.cvalues;
     const float* q = t.pvalues;
-    for (int i = 0; i < t.n && i < kTraitMax; ++i) {
+#pragma unroll
+    for (int i = 0; i < kTraitMax; ++i) {
+        if (i >= t.n) break;
         out[i] = rh_trait(t.uids[i], c, q, p, z);
         c += t.ccounts[i];
         q += t.pcounts[i];
@@ -357,8 +361,9 @@ __device__ __forceinline__ void rh_traits(const gbkhip_traits& t,
 
 __device__ __forceinline__ void one_traits(const gbkhip_traits& t, float* out)
 {
-    for (int i = 0; i < t.n && i < kTraitMax; ++i)
-        out[i] = t.uids[i] == 1 ? 1.0f : __builtin_nanf("");
+#pragma unroll
+    for (int i = 0; i < kTraitMax; ++i)
+        if (i < t.n) out[i] = t.uids[i] == 1 ? 1.0f : __builtin_nanf("");
 }
 
 // True when any trait of the disk needs the angle theta itself (the
@@ -462,7 +467,7 @@ struct VoxelLine {
 // gmodel_smdisk_evaluate_spaxel (gmodels.hpp:645-958) up to, but not
 // including, the spectral deposit.  `need_theta` is wave-uniform; `ng` holds
 // the per-model geometry of non-loose, non-tilted disks.
-__device__ inline VoxelLine smdisk_voxel(const gbkhip_disk& d,
+__device__ __forceinline__ VoxelLine smdisk_voxel(const gbkhip_disk& d,
                                          const gbkhip_grid& g,
                                          const NodeGeom& ng, int x, int y,
                                          int z, bool need_theta, bool want_w)
@@ -529,20 +534,25 @@ __device__ inline VoxelLine smdisk_voxel(const gbkhip_disk& d,
     if (d.sp.uids) {
         float sv = 0;
         p_traits<TK_SP>(d.sp, P, pt);
-        for (int i = 0; i < d.sp.n && i < kTraitMax; ++i) sv += pt[i];
+#pragma unroll
+        for (int i = 0; i < kTraitMax; ++i)
+            if (i < d.sp.n) sv += pt[i];
         if (!sv) return L;
     }
     if (d.zp.uids) {
         float zv = 0;
         p_traits<TK_ZP>(d.zp, P, pt);
-        for (int i = 0; i < d.zp.n && i < kTraitMax; ++i) zv += pt[i];
+#pragma unroll
+        for (int i = 0; i < kTraitMax; ++i)
+            if (i < d.zp.n) zv += pt[i];
         zr += zv;
     }
     float rvalue = 0;
     if (d.rp.uids) p_traits<TK_RP>(d.rp, P, pt);
     if (d.rh.uids) rh_traits(d.rh, P, fabsf(zr), ht);
-    for (int i = 0; i < d.rp.n && i < kTraitMax; ++i)
-        rvalue += pt[i] * (thin ? 1.0f : ht[i]);
+#pragma unroll
+    for (int i = 0; i < kTraitMax; ++i)
+        if (i < d.rp.n) rvalue += pt[i] * (thin ? 1.0f : ht[i]);
     if (!rvalue) return L;
     if (thin) rvalue /= G.ci;
     else rvalue *= g.spat_step[2];
@@ -550,21 +560,26 @@ __device__ inline VoxelLine smdisk_voxel(const gbkhip_disk& d,
     float vvalue = 0;
     if (d.vp.uids) p_traits<TK_VP>(d.vp, P, pt);
     if (d.vh.uids) one_traits(d.vh, ht);
-    for (int i = 0; i < d.vp.n && i < kTraitMax; ++i)
-        vvalue += pt[i] * (thin ? 1.0f : ht[i]);
+#pragma unroll
+    for (int i = 0; i < kTraitMax; ++i)
+        if (i < d.vp.n) vvalue += pt[i] * (thin ? 1.0f : ht[i]);
     vvalue += vsysi;
 
     float dvalue = 0;
     if (d.dp.uids) p_traits<TK_DP>(d.dp, P, pt);
     if (d.dh.uids) one_traits(d.dh, ht);
-    for (int i = 0; i < d.dp.n && i < kTraitMax; ++i)
-        dvalue += pt[i] * (thin ? 1.0f : ht[i]);
+#pragma unroll
+    for (int i = 0; i < kTraitMax; ++i)
+        if (i < d.dp.n) dvalue += pt[i] * (thin ? 1.0f : ht[i]);
     dvalue = fabsf(dvalue);
 
     float wvalue = 1.0f;
-    if (d.wp.uids && want_w)     // wp_trait_axis_range returns 1
-        for (int i = 0; i < d.wp.n && i < kTraitMax; ++i)
-            wvalue *= d.wp.uids[i] == 1 ? 1.0f : __builtin_nanf("");
+    if (d.wp.uids && want_w) {   // wp_trait_axis_range returns 1
+#pragma unroll
+        for (int i = 0; i < kTraitMax; ++i)
+            if (i < d.wp.n)
+                wvalue *= d.wp.uids[i] == 1 ? 1.0f : __builtin_nanf("");
+    }
 
     float orvalue = rvalue;
     if (d.opacity) {

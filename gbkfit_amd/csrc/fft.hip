@@ -21,6 +21,7 @@
 //   * twiddles exp(-2 pi i m / n) are generated in FP64 on the host and
 //     rounded to FP32 once per plan; the pass stages them in LDS.
 #include "fft_plan.hpp"
+#include "fft_engine.hpp"
 
 #include <cmath>
 #include <vector>
@@ -50,7 +51,7 @@ static Axis make_axis(int n)
     return a;
 }
 
-static void upload_twiddles(DevBuf& buf, int n, int count)
+void upload_twiddles(DevBuf& buf, int n, int count)
 {
     std::vector<float> h(2 * (size_t)count);
     for (int m = 0; m < count; ++m) {
@@ -85,20 +86,21 @@ static bool fast_radices(int n, std::vector<int>& R)
 
 // Magic multiplier m with floor(x / d) = umulhi(x, m) whenever x * d < 2^32
 // (m = ceil(2^32 / d); d = 1 is flagged by m = 0).
-static unsigned div_magic(unsigned d)
+unsigned div_magic(unsigned d)
 {
     return d <= 1 ? 0u
                   : (unsigned)((((unsigned long long)1 << 32) + d - 1) / d);
 }
 
-constexpr int kFastThreads = 256;
 constexpr size_t kFastLdsTarget = 48 * 1024;
 
-static FastShape fast_shape(int n)
+// Engine shape for n-point lines, 2^lshift lines per block: 256 threads when
+// the tile holds at most 16 elements per thread, else 512.
+FastShape fast_shape_lines(int n, int lshift)
 {
     FastShape f;
     std::vector<int> R;
-    if (n < 2 || !fast_radices(n, R)) return f;
+    if (n < 2 || lshift < 0 || lshift > 6 || !fast_radices(n, R)) return f;
     f.a.n = n;
     f.a.nst = (int)R.size();
     int ns = 1;
@@ -110,28 +112,44 @@ static FastShape fast_shape(int n)
         f.a.nsm[s] = div_magic((unsigned)ns);
         ns *= R[s];
     }
-    // most lines per block (best coalescing) whose tile fits the LDS target
-    // and whose elements per thread stay within the register buckets (32
-    // complex values per thread keeps the kernels near 150 VGPRs, no spills)
-    int lshift = -1;
-    for (int ls = 5; ls >= 2; --ls) {
-        const size_t bytes = (size_t)n * ((1 << ls) + 2) * sizeof(float2);
-        const int e = ((n << ls) + kFastThreads - 1) / kFastThreads;
-        if ((bytes <= kFastLdsTarget || (ls == 2 && bytes <= 96 * 1024)) &&
-            e <= 32) {
-            lshift = ls;
-            break;
-        }
-    }
-    if (lshift < 0) return f;
+    const long long tile = (long long)n << lshift;
     const size_t lds = (size_t)n * ((1 << lshift) + 2) * sizeof(float2);
-    const int e = ((n << lshift) + kFastThreads - 1) / kFastThreads;
-    const int emax = e <= 8 ? 8 : e <= 16 ? 16 : e <= 24 ? 24 : 32;
+    if (lds > 96 * 1024) return f;
+    if (tile <= (long long)kFastE * 256) f.threads = 256;
+    else if (tile <= (long long)kFastE * 512 && lshift >= 2) f.threads = 512;
+    else return f;
     f.ok = true;
     f.lshift = lshift;
-    f.emax = emax;
     f.lds = lds;
     return f;
+}
+
+// Strided lines: the most lines per block (the block reads 8 L contiguous
+// bytes per element) whose tile fits the LDS target.
+FastShape fast_shape(int n)
+{
+    for (int ls = 5; ls >= 2; --ls) {
+        const size_t bytes = (size_t)n * ((1 << ls) + 2) * sizeof(float2);
+        if (bytes <= kFastLdsTarget || (ls == 2 && bytes <= 96 * 1024)) {
+            FastShape f = fast_shape_lines(n, ls);
+            if (f.ok) return f;
+        }
+    }
+    return FastShape();
+}
+
+// Contiguous rows: the rows per block do not affect coalescing, so take the
+// most rows that keep 256 threads at 16 elements each.
+FastShape fast_shape_rows(int n)
+{
+    for (int ls = 5; ls >= 0; --ls) {
+        const size_t bytes = (size_t)n * ((1 << ls) + 2) * sizeof(float2);
+        if (((long long)n << ls) <= kFastE * 256 && bytes <= kFastLdsTarget) {
+            FastShape f = fast_shape_lines(n, ls);
+            if (f.ok) return f;
+        }
+    }
+    return fast_shape(n);
 }
 
 std::shared_ptr<FFTPlan> get_plan(Context& c, int nx, int ny, int nz)
@@ -157,7 +175,7 @@ std::shared_ptr<FFTPlan> get_plan(Context& c, int nx, int ny, int nz)
     upload_twiddles(p->twy, ny, ny);
     upload_twiddles(p->twz, nz, nz);
     upload_twiddles(p->twpost, nx, nx / 2 + 1);
-    if (p->even) p->fx = fast_shape(rn);
+    if (p->even) p->fx = fast_shape_rows(rn);
     p->fy = fast_shape(ny);
     p->fz = fast_shape(nz);
     c.plans[key] = p;
@@ -179,38 +197,6 @@ static AxisArg axis_arg(const Axis& a)
     r.nst = a.nst;
     for (int i = 0; i < kMaxStages; ++i) r.radix[i] = a.radix[i];
     return r;
-}
-
-__device__ __forceinline__ float2 cmul(float2 a, float2 b)
-{
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ float2 cadd(float2 a, float2 b)
-{ return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b)
-{ return make_float2(a.x - b.x, a.y - b.y); }
-// multiply by (sign * i)
-__device__ __forceinline__ float2 mul_si(float2 a, float sign)
-{ return make_float2(-sign * a.y, sign * a.x); }
-__device__ __forceinline__ float2 conjf2(float2 a)
-{ return make_float2(a.x, -a.y); }
-
-// twiddle exp(sign * 2 pi i m / n) from the forward table
-__device__ __forceinline__ float2 twid(const float2* tw, int m, float sign)
-{
-    const float2 t = tw[m];
-    return sign < 0 ? t : make_float2(t.x, -t.y);
-}
-
-__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2,
-                                     float2& a3, float sign)
-{
-    const float2 s02 = cadd(a0, a2), d02 = csub(a0, a2);
-    const float2 s13 = cadd(a1, a3), d13 = mul_si(csub(a1, a3), sign);
-    a0 = cadd(s02, s13);
-    a2 = csub(s02, s13);
-    a1 = cadd(d02, d13);
-    a3 = csub(d02, d13);
 }
 
 // One Stockham stage of radix r over `lines` lines of length n stored with
@@ -481,268 +467,70 @@ __global__ __launch_bounds__(256) void fft_lines(
     }
 }
 
-// ---------------------------------------------------------------------------
-// Register-tiled engine (2/3/5-smooth lengths)
-//
-// A block transforms L = 2^lshift lines held in one LDS tile laid out
-// [m][L+1] (element m of line l at m*(L+1) + l).  Threads walk butterflies
-// with the line index fastest, so the lanes of a wave touch consecutive LDS
-// words and share one twiddle.  Each Stockham stage reads its butterflies
-// into registers (compile-time radix, statically indexed arrays), syncs, and
-// writes them back in autosort order into the same tile.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ unsigned fdiv(unsigned x, unsigned magic)
-{
-    return magic ? __umulhi(x, magic) : x;
-}
-
-template <int R> struct Bfly;
-
-template <> struct Bfly<2> {
-    __device__ static constexpr int perm(int q) { return q; }
-    __device__ static void run(float2* v, float)
-    {
-        const float2 a = v[0], b = v[1];
-        v[0] = cadd(a, b);
-        v[1] = csub(a, b);
-    }
-};
-template <> struct Bfly<3> {
-    __device__ static constexpr int perm(int q) { return q; }
-    __device__ static void run(float2* v, float sign)
-    {
-        const float2 s = cadd(v[1], v[2]);
-        const float2 d = mul_si(csub(v[1], v[2]),
-                                sign * 0.866025403784438647f);
-        const float2 t = make_float2(v[0].x - 0.5f * s.x, v[0].y - 0.5f * s.y);
-        v[0] = cadd(v[0], s);
-        v[1] = cadd(t, d);
-        v[2] = csub(t, d);
-    }
-};
-template <> struct Bfly<4> {
-    __device__ static constexpr int perm(int q) { return q; }
-    __device__ static void run(float2* v, float sign)
-    {
-        dft4(v[0], v[1], v[2], v[3], sign);
-    }
-};
-template <> struct Bfly<5> {
-    __device__ static constexpr int perm(int q) { return q; }
-    __device__ static void run(float2* v, float sign)
-    {
-        // cos/sin of 2 pi / 5 and 4 pi / 5
-        const float c1 = 0.309016994374947424f;
-        const float c2 = -0.809016994374947424f;
-        const float s1 = 0.951056516295153572f * sign;
-        const float s2 = 0.587785252292473129f * sign;
-        const float2 b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]);
-        const float2 d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
-        const float2 t1 = make_float2(v[0].x + c1 * b1.x + c2 * b2.x,
-                                      v[0].y + c1 * b1.y + c2 * b2.y);
-        const float2 t2 = make_float2(v[0].x + c2 * b1.x + c1 * b2.x,
-                                      v[0].y + c2 * b1.y + c1 * b2.y);
-        // u1 = s1 d1 + s2 d2, u2 = s2 d1 - s1 d2, each times i
-        const float2 u1 = make_float2(-(s1 * d1.y + s2 * d2.y),
-                                      s1 * d1.x + s2 * d2.x);
-        const float2 u2 = make_float2(-(s2 * d1.y - s1 * d2.y),
-                                      s2 * d1.x - s1 * d2.x);
-        v[0] = cadd(v[0], cadd(b1, b2));
-        v[1] = cadd(t1, u1);
-        v[4] = csub(t1, u1);
-        v[2] = cadd(t2, u2);
-        v[3] = csub(t2, u2);
-    }
-};
-template <> struct Bfly<8> {
-    __device__ static constexpr int perm(int q) { return q; }
-    __device__ static void run(float2* v, float sign)
-    {
-        float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
-        float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
-        dft4(e0, e1, e2, e3, sign);
-        dft4(o0, o1, o2, o3, sign);
-        const float h = 0.707106781186547524f;
-        // W8^1 = h (1 + sign i), W8^2 = sign i, W8^3 = h (-1 + sign i)
-        o1 = make_float2(h * (o1.x - sign * o1.y), h * (o1.y + sign * o1.x));
-        o2 = mul_si(o2, sign);
-        o3 = make_float2(h * (-o3.x - sign * o3.y), h * (-o3.y + sign * o3.x));
-        v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
-        v[1] = cadd(e1, o1); v[5] = csub(e1, o1);
-        v[2] = cadd(e2, o2); v[6] = csub(e2, o2);
-        v[3] = cadd(e3, o3); v[7] = csub(e3, o3);
-    }
-};
-template <> struct Bfly<16> {
-    // 16 = 4 x 4: index n = n1 + 4 n2 in, k = 4 k1 + k2 out;
-    // X[4 k1 + k2] = sum_n1 W4^(n1 k1) W16^(n1 k2) sum_n2 x[n1 + 4 n2] W4^(n2 k2)
-    // computed in place: X[4 k1 + k2] ends in v[4 k2 + k1].
-    __device__ static constexpr int perm(int q) { return 4 * (q % 4) + q / 4; }
-    __device__ static void run(float2* v, float sign)
-    {
-        // cos and sin of 2 pi m / 16, m = 0..9
-        constexpr float C[10] = {1.0f, 0.923879532511286756f,
-                                 0.707106781186547524f, 0.382683432365089772f,
-                                 0.0f, -0.382683432365089772f,
-                                 -0.707106781186547524f, -0.923879532511286756f,
-                                 -1.0f, -0.923879532511286756f};
-        constexpr float S[10] = {0.0f, 0.382683432365089772f,
-                                 0.707106781186547524f, 0.923879532511286756f,
-                                 1.0f, 0.923879532511286756f,
-                                 0.707106781186547524f, 0.382683432365089772f,
-                                 0.0f, -0.382683432365089772f};
-#pragma unroll
-        for (int n1 = 0; n1 < 4; ++n1)
-            dft4(v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12], sign);
-        // v[n1 + 4 k2] now holds Y[n1][k2]
-#pragma unroll
-        for (int n1 = 1; n1 < 4; ++n1)
-#pragma unroll
-            for (int k2 = 1; k2 < 4; ++k2) {
-                const int m = n1 * k2;
-                v[n1 + 4 * k2] = cmul(v[n1 + 4 * k2],
-                                      make_float2(C[m], sign * S[m]));
-            }
-#pragma unroll
-        for (int k2 = 0; k2 < 4; ++k2)
-            dft4(v[4 * k2], v[4 * k2 + 1], v[4 * k2 + 2], v[4 * k2 + 3], sign);
-    }
-};
-
-// One Stockham stage of radix R over the tile; CAP = butterflies per thread.
-// The per-stage constants come from the stage table (not from n and R), so
-// the compiler cannot hoist one radix case's addresses out of the stage loop
-// and keep all cases' live at once.
-template <int R, int CAP>
-__device__ __forceinline__ void fstage(float2* buf, int lshift, int nb, int ns,
-                                       int twstep, unsigned nsm,
-                                       const float2* tw, float sign)
-{
-    constexpr int T = kFastThreads;
-    const int LS = (1 << lshift) + 1;
-    const int lmask = (1 << lshift) - 1;
-    const int total = nb << lshift;
-    float2 v[CAP][R];
-#pragma unroll
-    for (int c = 0; c < CAP; ++c) {
-        const int b = threadIdx.x + c * T;
-        if (b < total) {
-            const int l = b & lmask, j = b >> lshift;
-            const int k = j - ns * (int)fdiv((unsigned)j, nsm);
-#pragma unroll
-            for (int t = 0; t < R; ++t) v[c][t] = buf[(j + t * nb) * LS + l];
-            if (k) {
-#pragma unroll
-                for (int t = 1; t < R; ++t)
-                    v[c][t] = cmul(v[c][t], twid(tw, t * k * twstep, sign));
-            }
-            Bfly<R>::run(v[c], sign);
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < CAP; ++c) {
-        const int b = threadIdx.x + c * T;
-        if (b < total) {
-            const int l = b & lmask, j = b >> lshift;
-            const int k = j - ns * (int)fdiv((unsigned)j, nsm);
-            const int base = (j - k) * R + k;
-#pragma unroll
-            for (int q = 0; q < R; ++q)
-                buf[(base + q * ns) * LS + l] = v[c][Bfly<R>::perm(q)];
-        }
-    }
-    __syncthreads();
-}
-
-// Whole transform of the tile (the tile must be complete on entry; it is
-// complete again on return).
-template <int EMAX>
-__device__ void fast_fft(float2* buf, const FAxis& a, int lshift,
-                         const float2* tw, float sign)
-{
-#pragma unroll 1
-    for (int s = 0; s < a.nst; ++s) {
-        const int nb = a.nb[s], ns = a.ns[s], tws = a.tws[s];
-        const unsigned nsm = a.nsm[s];
-        switch (a.R[s]) {
-        case 2: fstage<2, (EMAX + 1) / 2>(buf, lshift, nb, ns, tws, nsm, tw, sign); break;
-        case 3: fstage<3, (EMAX + 2) / 3>(buf, lshift, nb, ns, tws, nsm, tw, sign); break;
-        case 4: fstage<4, (EMAX + 3) / 4>(buf, lshift, nb, ns, tws, nsm, tw, sign); break;
-        case 5: fstage<5, (EMAX + 4) / 5>(buf, lshift, nb, ns, tws, nsm, tw, sign); break;
-        case 8: fstage<8, (EMAX + 7) / 8>(buf, lshift, nb, ns, tws, nsm, tw, sign); break;
-        default: fstage<16, (EMAX + 15) / 16>(buf, lshift, nb, ns, tws, nsm, tw, sign); break;
-        }
-    }
-}
-
-// Strided line pass along y or z.  Line q (of nlines) starts at
-// (q / inner) * ostride + q % inner; its elements are `stride` apart.
-// mode 0: forward, 1: inverse, 2: forward * K * scale then inverse.
-template <int EMAX>
-__global__ __launch_bounds__(256) void fx_lines(
+// Strided line pass.  Line q (of nlines) starts at
+// base0 + (q / inner) * ostride + q % inner; its elements are `stride` apart.
+// mode 0: forward, 1: inverse, 2: forward * K * scale then inverse.  Only
+// elements m in [st_lo, st_hi) are stored.  Thread (l, m0) = (tid % L,
+// tid / L) moves elements m0 + c T/L of line l.
+template <int LSH, int T>
+__global__ __launch_bounds__(T) void fx_lines(
     const float2* __restrict__ src, float2* __restrict__ dst,
     const float2* __restrict__ K, int nlines, int inner, unsigned inner_m,
-    long long ostride, long long stride, FAxis a, const float2* tw,
-    int lshift, int mode, float scale, long long bstride, long long kbstride)
+    long long base0, long long ostride, long long stride, FAxis a,
+    const float2* tw, int mode, float scale, int st_lo, int st_hi,
+    long long bstride, long long kbstride)
 {
     extern __shared__ float2 lds[];
-    constexpr int T = kFastThreads;
+    constexpr int L = 1 << LSH, LS = L + 1, JS = T >> LSH;
     const int n = a.n;
-    const int LS = (1 << lshift) + 1;
     float2* s_tw = lds;
     float2* buf = lds + n;
     src += blockIdx.y * bstride;
     dst += blockIdx.y * bstride;
     if (K) K += blockIdx.y * kbstride;
     for (int i = threadIdx.x; i < n; i += T) s_tw[i] = tw[i];
-    const int l = threadIdx.x & ((1 << lshift) - 1);
-    const int q = (blockIdx.x << lshift) + l;
+    const int l = threadIdx.x & (L - 1);
+    const int m0 = threadIdx.x >> LSH;
+    const int q = (blockIdx.x << LSH) + l;
     const bool valid = q < nlines;
     const int qo = inner >= nlines ? 0 : (int)fdiv((unsigned)q, inner_m);
-    const long long base = (long long)qo * ostride + (q - qo * inner);
-    // a thread owns elements m0 + c * mstep (c < EMAX) of line l; all its
-    // global loads are issued before the first LDS write so that EMAX loads
-    // per lane are in flight
-    const int m0 = threadIdx.x >> lshift, mstep = T >> lshift;
-    float2 r[EMAX];
+    const long long base = base0 + (long long)qo * ostride + (q - qo * inner) +
+                           (long long)m0 * stride;
+    const long long cstride = (long long)JS * stride;
+    const int la = m0 * LS + l;
+    // all global loads of a thread are issued before the first LDS write
+    float2 r[kFastE];
 #pragma unroll
-    for (int c = 0; c < EMAX; ++c) {
-        const int m = m0 + c * mstep;
-        r[c] = valid && m < n ? src[base + (long long)m * stride]
-                              : make_float2(0, 0);
-    }
+    for (int c = 0; c < kFastE; ++c)
+        r[c] = valid && m0 + c * JS < n ? src[base + c * cstride]
+                                        : make_float2(0, 0);
 #pragma unroll
-    for (int c = 0; c < EMAX; ++c) {
-        const int m = m0 + c * mstep;
-        if (m < n) buf[m * LS + l] = r[c];
-    }
+    for (int c = 0; c < kFastE; ++c)
+        if (m0 + c * JS < n) buf[la + c * JS * LS] = r[c];
     __syncthreads();
-    fast_fft<EMAX>(buf, a, lshift, s_tw, mode == 1 ? 1.0f : -1.0f);
+    fast_fft<LSH, T>(buf, a, s_tw, mode == 1 ? 1.0f : -1.0f);
     if (mode == 2) {
 #pragma unroll
-        for (int c = 0; c < EMAX; ++c) {
-            const int m = m0 + c * mstep;
-            r[c] = valid && m < n ? K[base + (long long)m * stride]
-                                  : make_float2(0, 0);
-        }
+        for (int c = 0; c < kFastE; ++c)
+            r[c] = valid && m0 + c * JS < n ? K[base + c * cstride]
+                                            : make_float2(0, 0);
 #pragma unroll
-        for (int c = 0; c < EMAX; ++c) {
-            const int m = m0 + c * mstep;
-            if (m < n) {
-                const float2 v = buf[m * LS + l], kv = r[c];
-                buf[m * LS + l] = make_float2((v.x * kv.x - v.y * kv.y) * scale,
-                                              (v.x * kv.y + v.y * kv.x) * scale);
+        for (int c = 0; c < kFastE; ++c)
+            if (m0 + c * JS < n) {
+                const float2 v = buf[la + c * JS * LS], kv = r[c];
+                buf[la + c * JS * LS] =
+                    make_float2((v.x * kv.x - v.y * kv.y) * scale,
+                                (v.x * kv.y + v.y * kv.x) * scale);
             }
-        }
         __syncthreads();
-        fast_fft<EMAX>(buf, a, lshift, s_tw, 1.0f);
+        fast_fft<LSH, T>(buf, a, s_tw, 1.0f);
     }
     if (valid) {
 #pragma unroll
-        for (int c = 0; c < EMAX; ++c) {
-            const int m = m0 + c * mstep;
-            if (m < n) dst[base + (long long)m * stride] = buf[m * LS + l];
+        for (int c = 0; c < kFastE; ++c) {
+            const int m = m0 + c * JS;
+            if (m < n && m >= st_lo && m < st_hi)
+                dst[base + c * cstride] = buf[la + c * JS * LS];
         }
     }
 }
@@ -750,35 +538,37 @@ __global__ __launch_bounds__(256) void fx_lines(
 // Row pass, real -> half spectrum, nx even: a complex FFT of length n = nx/2
 // over z[m] = x[2m] + i x[2m+1], then
 // X[k] = (Z[k] + conj Z[n-k]) / 2 + W^k (Z[k] - conj Z[n-k]) / (2i).
-template <int EMAX>
-__global__ __launch_bounds__(256) void fx_rows_r2c(
+// Element i of the block's contiguous tile is (row i / n, m = i % n).
+template <int LSH, int T>
+__global__ __launch_bounds__(T) void fx_rows_r2c(
     const float* __restrict__ in, float2* __restrict__ out, int nrows,
     FAxis a, unsigned n_m, unsigned nxh_m, const float2* tw,
-    const float2* __restrict__ twpost, int lshift, long long bstride_r,
+    const float2* __restrict__ twpost, long long bstride_r,
     long long bstride_c)
 {
     extern __shared__ float2 lds[];
-    constexpr int T = kFastThreads;
+    constexpr int L = 1 << LSH, LS = L + 1;
     const int n = a.n, nxh = n + 1;
-    const int LS = (1 << lshift) + 1;
     float2* s_tw = lds;
     float2* buf = lds + n;
-    const int r0 = blockIdx.x << lshift;
-    const int nl = min(1 << lshift, nrows - r0);
+    float2* s_post = buf + n * LS;      // twpost staged in LDS
+    const int r0 = blockIdx.x << LSH;
+    const int nl = min(L, nrows - r0);
     const float2* in2 = reinterpret_cast<const float2*>(
                             in + blockIdx.y * bstride_r) + (size_t)r0 * n;
     out += blockIdx.y * bstride_c + (size_t)r0 * nxh;
     for (int i = threadIdx.x; i < n; i += T) s_tw[i] = tw[i];
-    const int tile = n << lshift;
+    for (int i = threadIdx.x; i < nxh; i += T) s_post[i] = twpost[i];
+    const int tile = n << LSH;
     const int valid = n * nl;
-    float2 r[EMAX];
+    float2 r[kFastE];
 #pragma unroll
-    for (int c = 0; c < EMAX; ++c) {
+    for (int c = 0; c < kFastE; ++c) {
         const int i = threadIdx.x + c * T;
         r[c] = i < valid ? in2[i] : make_float2(0, 0);
     }
 #pragma unroll
-    for (int c = 0; c < EMAX; ++c) {
+    for (int c = 0; c < kFastE; ++c) {
         const int i = threadIdx.x + c * T;
         if (i < tile) {
             const int l = (int)fdiv((unsigned)i, n_m), m = i - l * n;
@@ -786,7 +576,7 @@ __global__ __launch_bounds__(256) void fx_rows_r2c(
         }
     }
     __syncthreads();
-    fast_fft<EMAX>(buf, a, lshift, s_tw, -1.0f);
+    fast_fft<LSH, T>(buf, a, s_tw, -1.0f);
     const int cnt = nl * nxh;
     for (int i = threadIdx.x; i < cnt; i += T) {
         const int l = (int)fdiv((unsigned)i, nxh_m), k = i - l * nxh;
@@ -796,35 +586,37 @@ __global__ __launch_bounds__(256) void fx_rows_r2c(
                                       0.5f * (zk.y + zm.y));
         const float2 dd = csub(zk, zm);
         const float2 fo = make_float2(0.5f * dd.y, -0.5f * dd.x);
-        out[i] = cadd(fe, cmul(twpost[k], fo));
+        out[i] = cadd(fe, cmul(s_post[k], fo));
     }
 }
 
 // Row pass, half spectrum -> real, nx even (mirror of fx_rows_r2c):
 // Z[k] = (X[k] + conj X[n-k]) + i W^-k (X[k] - conj X[n-k]).
-template <int EMAX>
-__global__ __launch_bounds__(256) void fx_rows_c2r(
+template <int LSH, int T>
+__global__ __launch_bounds__(T) void fx_rows_c2r(
     const float2* __restrict__ in, float* __restrict__ out, int nrows,
     FAxis a, unsigned n_m, const float2* tw,
-    const float2* __restrict__ twpost, int lshift, long long bstride_c,
+    const float2* __restrict__ twpost, long long bstride_c,
     long long bstride_r)
 {
     extern __shared__ float2 lds[];
-    constexpr int T = kFastThreads;
+    constexpr int L = 1 << LSH, LS = L + 1;
     const int n = a.n, nxh = n + 1;
-    const int LS = (1 << lshift) + 1;
     float2* s_tw = lds;
     float2* buf = lds + n;
-    const int r0 = blockIdx.x << lshift;
-    const int nl = min(1 << lshift, nrows - r0);
+    float2* s_post = buf + n * LS;      // twpost staged in LDS
+    const int r0 = blockIdx.x << LSH;
+    const int nl = min(L, nrows - r0);
     in += blockIdx.y * bstride_c + (size_t)r0 * nxh;
     float2* out2 = reinterpret_cast<float2*>(out + blockIdx.y * bstride_r) +
                    (size_t)r0 * n;
     for (int i = threadIdx.x; i < n; i += T) s_tw[i] = tw[i];
-    const int tile = n << lshift;
-    float2 xa[EMAX], xb[EMAX];
+    for (int i = threadIdx.x; i < nxh; i += T) s_post[i] = twpost[i];
+    __syncthreads();
+    const int tile = n << LSH;
+    float2 xa[kFastE], xb[kFastE];
 #pragma unroll
-    for (int c = 0; c < EMAX; ++c) {
+    for (int c = 0; c < kFastE; ++c) {
         const int i = threadIdx.x + c * T;
         const int l = (int)fdiv((unsigned)i, n_m), k = i - l * n;
         xa[c] = xb[c] = make_float2(0, 0);
@@ -835,19 +627,19 @@ __global__ __launch_bounds__(256) void fx_rows_c2r(
         }
     }
 #pragma unroll
-    for (int c = 0; c < EMAX; ++c) {
+    for (int c = 0; c < kFastE; ++c) {
         const int i = threadIdx.x + c * T;
         if (i < tile) {
             const int l = (int)fdiv((unsigned)i, n_m), k = i - l * n;
             const float2 xk = xa[c];
             const float2 xm = conjf2(xb[c]);
             const float2 fe = cadd(xk, xm);
-            const float2 fo = cmul(csub(xk, xm), conjf2(twpost[k]));
+            const float2 fo = cmul(csub(xk, xm), conjf2(s_post[k]));
             buf[k * LS + l] = cadd(fe, make_float2(-fo.y, fo.x));
         }
     }
     __syncthreads();
-    fast_fft<EMAX>(buf, a, lshift, s_tw, 1.0f);
+    fast_fft<LSH, T>(buf, a, s_tw, 1.0f);
     const int cnt = nl * n;
     for (int i = threadIdx.x; i < cnt; i += T) {
         const int l = (int)fdiv((unsigned)i, n_m), m = i - l * n;
@@ -877,13 +669,23 @@ constexpr size_t kLdsBudget = 64 * 1024;
 
 static int pad_ld(int n) { return n + (n % 2 == 0 ? 1 : 0); }
 
-#define GBK_FAST_DISPATCH(EMAX, KERNEL, ...)                                 \
-    switch (EMAX) {                                                          \
-    case 8: hipLaunchKernelGGL(KERNEL<8>, __VA_ARGS__); break;               \
-    case 16: hipLaunchKernelGGL(KERNEL<16>, __VA_ARGS__); break;             \
-    case 24: hipLaunchKernelGGL(KERNEL<24>, __VA_ARGS__); break;             \
-    default: hipLaunchKernelGGL(KERNEL<32>, __VA_ARGS__); break;             \
-    }
+void launch_fx_lines(Context& c, const FastShape& f, const float2* tw,
+                     const float2* src, float2* dst, const float2* K,
+                     int nlines, int inner, long long base0,
+                     long long ostride, long long stride, int st_lo,
+                     int st_hi, int mode, float scale, int batch,
+                     long long bstride, long long kbstride)
+{
+    GBK_CHECK(f.ok && nlines > 0 && inner > 0 &&
+              (long long)nlines * inner < (1LL << 32) && st_lo >= 0 &&
+              st_hi <= f.a.n, GBKHIP_ERR_INTERNAL, "bad line pass shape");
+    const unsigned inner_m = inner >= nlines ? 0u : div_magic((unsigned)inner);
+    dim3 grid(ceil_div(nlines, 1 << f.lshift), batch);
+    GBK_FAST_DISPATCH(f, fx_lines, grid, f.lds, c.stream, src, dst, K,
+                      nlines, inner, inner_m, base0, ostride, stride, f.a, tw,
+                      mode, scale, st_lo, st_hi, bstride, kbstride);
+    GBK_HIP(hipGetLastError());
+}
 
 static void launch_rows_fast(Context& c, const FFTPlan& p, bool forward,
                              const void* in, void* out, int batch,
@@ -894,18 +696,17 @@ static void launch_rows_fast(Context& c, const FFTPlan& p, bool forward,
     const int n = f.a.n;
     const unsigned n_m = div_magic((unsigned)n);
     const unsigned nxh_m = div_magic((unsigned)(n + 1));
-    const size_t lds = f.lds;
+    const size_t lds = f.lds + (size_t)(n + 1) * sizeof(float2);
     dim3 grid(ceil_div(nrows, 1 << f.lshift), batch);
     if (forward) {
-        GBK_FAST_DISPATCH(f.emax, fx_rows_r2c, grid, dim3(kFastThreads), lds,
-                          c.stream, (const float*)in, (float2*)out, nrows,
-                          f.a, n_m, nxh_m, p.twx.as<float2>(),
-                          p.twpost.as<float2>(), f.lshift, br, bc);
+        GBK_FAST_DISPATCH(f, fx_rows_r2c, grid, lds, c.stream,
+                          (const float*)in, (float2*)out, nrows, f.a, n_m,
+                          nxh_m, p.twx.as<float2>(), p.twpost.as<float2>(),
+                          br, bc);
     } else {
-        GBK_FAST_DISPATCH(f.emax, fx_rows_c2r, grid, dim3(kFastThreads), lds,
-                          c.stream, (const float2*)in, (float*)out, nrows,
-                          f.a, n_m, p.twx.as<float2>(),
-                          p.twpost.as<float2>(), f.lshift, bc, br);
+        GBK_FAST_DISPATCH(f, fx_rows_c2r, grid, lds, c.stream,
+                          (const float2*)in, (float*)out, nrows, f.a, n_m,
+                          p.twx.as<float2>(), p.twpost.as<float2>(), bc, br);
     }
     GBK_HIP(hipGetLastError());
 }
@@ -963,11 +764,9 @@ static void launch_lines(Context& c, const FFTPlan& p, int axis, int mode,
         const unsigned inner_m = axis == 1 ? div_magic((unsigned)inner) : 0u;
         const long long os = axis == 1 ? ostride : 0;
         dim3 grid(ceil_div(nlines, 1 << f.lshift), batch);
-        GBK_FAST_DISPATCH(f.emax, fx_lines, grid, dim3(kFastThreads), f.lds,
-                          c.stream, src, dst, K, (int)nlines, inner, inner_m,
-                          os, stride, f.a, tw, f.lshift, mode, scale, bstride,
-                          kbstride);
-        GBK_HIP(hipGetLastError());
+        launch_fx_lines(c, f, tw, src, dst, K, (int)nlines, inner, 0, os,
+                        stride, 0, f.a.n, mode, scale, batch, bstride,
+                        kbstride);
         return;
     }
     const int n = A.n;
@@ -997,13 +796,9 @@ void fft_prepare_attributes()
         (void)hipFuncSetAttribute((const void*)fft_lines,
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
-        const void* fast[] = {
-            (const void*)fx_lines<8>, (const void*)fx_lines<16>,
-            (const void*)fx_lines<24>, (const void*)fx_lines<32>,
-            (const void*)fx_rows_r2c<8>, (const void*)fx_rows_r2c<16>,
-            (const void*)fx_rows_r2c<24>, (const void*)fx_rows_r2c<32>,
-            (const void*)fx_rows_c2r<8>, (const void*)fx_rows_c2r<16>,
-            (const void*)fx_rows_c2r<24>, (const void*)fx_rows_c2r<32>};
+        const void* fast[] = {GBK_FAST_KERNELS(fx_lines),
+                              GBK_FAST_KERNELS(fx_rows_r2c),
+                              GBK_FAST_KERNELS(fx_rows_c2r)};
         for (const void* k : fast)
             (void)hipFuncSetAttribute(k,
                                       hipFuncAttributeMaxDynamicSharedMemorySize,

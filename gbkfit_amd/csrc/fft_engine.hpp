@@ -1,0 +1,286 @@
+// Device side of the register-tiled FFT engine, shared by the FFT passes
+// and the fused batched pipeline.
+#pragma once
+
+#include "fft_plan.hpp"
+
+namespace gbkhip {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b)
+{ return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b)
+{ return make_float2(a.x - b.x, a.y - b.y); }
+// multiply by (sign * i)
+__device__ __forceinline__ float2 mul_si(float2 a, float sign)
+{ return make_float2(-sign * a.y, sign * a.x); }
+__device__ __forceinline__ float2 conjf2(float2 a)
+{ return make_float2(a.x, -a.y); }
+
+// twiddle exp(sign * 2 pi i m / n) from the forward table
+__device__ __forceinline__ float2 twid(const float2* tw, int m, float sign)
+{
+    const float2 t = tw[m];
+    return sign < 0 ? t : make_float2(t.x, -t.y);
+}
+
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2,
+                                     float2& a3, float sign)
+{
+    const float2 s02 = cadd(a0, a2), d02 = csub(a0, a2);
+    const float2 s13 = cadd(a1, a3), d13 = mul_si(csub(a1, a3), sign);
+    a0 = cadd(s02, s13);
+    a2 = csub(s02, s13);
+    a1 = cadd(d02, d13);
+    a3 = csub(d02, d13);
+}
+
+// ---------------------------------------------------------------------------
+// Register-tiled engine (2/3/5-smooth lengths)
+//
+// A block transforms L = 2^lshift lines held in one LDS tile laid out
+// [m][L+1] (element m of line l at m*(L+1) + l).  Threads walk butterflies
+// with the line index fastest, so the lanes of a wave touch consecutive LDS
+// words and share one twiddle.  Each Stockham stage reads its butterflies
+// into registers (compile-time radix, statically indexed arrays), syncs, and
+// writes them back in autosort order into the same tile.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned fdiv(unsigned x, unsigned magic)
+{
+    return magic ? __umulhi(x, magic) : x;
+}
+
+template <int R> struct Bfly;
+
+template <> struct Bfly<2> {
+    __device__ static constexpr int perm(int q) { return q; }
+    __device__ static void run(float2* v, float)
+    {
+        const float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+template <> struct Bfly<3> {
+    __device__ static constexpr int perm(int q) { return q; }
+    __device__ static void run(float2* v, float sign)
+    {
+        const float2 s = cadd(v[1], v[2]);
+        const float2 d = mul_si(csub(v[1], v[2]),
+                                sign * 0.866025403784438647f);
+        const float2 t = make_float2(v[0].x - 0.5f * s.x, v[0].y - 0.5f * s.y);
+        v[0] = cadd(v[0], s);
+        v[1] = cadd(t, d);
+        v[2] = csub(t, d);
+    }
+};
+template <> struct Bfly<4> {
+    __device__ static constexpr int perm(int q) { return q; }
+    __device__ static void run(float2* v, float sign)
+    {
+        dft4(v[0], v[1], v[2], v[3], sign);
+    }
+};
+template <> struct Bfly<5> {
+    __device__ static constexpr int perm(int q) { return q; }
+    __device__ static void run(float2* v, float sign)
+    {
+        // cos/sin of 2 pi / 5 and 4 pi / 5
+        const float c1 = 0.309016994374947424f;
+        const float c2 = -0.809016994374947424f;
+        const float s1 = 0.951056516295153572f * sign;
+        const float s2 = 0.587785252292473129f * sign;
+        const float2 b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]);
+        const float2 d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+        const float2 t1 = make_float2(v[0].x + c1 * b1.x + c2 * b2.x,
+                                      v[0].y + c1 * b1.y + c2 * b2.y);
+        const float2 t2 = make_float2(v[0].x + c2 * b1.x + c1 * b2.x,
+                                      v[0].y + c2 * b1.y + c1 * b2.y);
+        // u1 = s1 d1 + s2 d2, u2 = s2 d1 - s1 d2, each times i
+        const float2 u1 = make_float2(-(s1 * d1.y + s2 * d2.y),
+                                      s1 * d1.x + s2 * d2.x);
+        const float2 u2 = make_float2(-(s2 * d1.y - s1 * d2.y),
+                                      s2 * d1.x - s1 * d2.x);
+        v[0] = cadd(v[0], cadd(b1, b2));
+        v[1] = cadd(t1, u1);
+        v[4] = csub(t1, u1);
+        v[2] = cadd(t2, u2);
+        v[3] = csub(t2, u2);
+    }
+};
+template <> struct Bfly<8> {
+    __device__ static constexpr int perm(int q) { return q; }
+    __device__ static void run(float2* v, float sign)
+    {
+        float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+        float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+        dft4(e0, e1, e2, e3, sign);
+        dft4(o0, o1, o2, o3, sign);
+        const float h = 0.707106781186547524f;
+        // W8^1 = h (1 + sign i), W8^2 = sign i, W8^3 = h (-1 + sign i)
+        o1 = make_float2(h * (o1.x - sign * o1.y), h * (o1.y + sign * o1.x));
+        o2 = mul_si(o2, sign);
+        o3 = make_float2(h * (-o3.x - sign * o3.y), h * (-o3.y + sign * o3.x));
+        v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+        v[1] = cadd(e1, o1); v[5] = csub(e1, o1);
+        v[2] = cadd(e2, o2); v[6] = csub(e2, o2);
+        v[3] = cadd(e3, o3); v[7] = csub(e3, o3);
+    }
+};
+template <> struct Bfly<16> {
+    // 16 = 4 x 4: index n = n1 + 4 n2 in, k = 4 k1 + k2 out;
+    // X[4 k1 + k2] = sum_n1 W4^(n1 k1) W16^(n1 k2) sum_n2 x[n1 + 4 n2] W4^(n2 k2)
+    // computed in place: X[4 k1 + k2] ends in v[4 k2 + k1].
+    __device__ static constexpr int perm(int q) { return 4 * (q % 4) + q / 4; }
+    __device__ static void run(float2* v, float sign)
+    {
+        // cos and sin of 2 pi m / 16, m = 0..9
+        constexpr float C[10] = {1.0f, 0.923879532511286756f,
+                                 0.707106781186547524f, 0.382683432365089772f,
+                                 0.0f, -0.382683432365089772f,
+                                 -0.707106781186547524f, -0.923879532511286756f,
+                                 -1.0f, -0.923879532511286756f};
+        constexpr float S[10] = {0.0f, 0.382683432365089772f,
+                                 0.707106781186547524f, 0.923879532511286756f,
+                                 1.0f, 0.923879532511286756f,
+                                 0.707106781186547524f, 0.382683432365089772f,
+                                 0.0f, -0.382683432365089772f};
+#pragma unroll
+        for (int n1 = 0; n1 < 4; ++n1)
+            dft4(v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12], sign);
+        // v[n1 + 4 k2] now holds Y[n1][k2]
+#pragma unroll
+        for (int n1 = 1; n1 < 4; ++n1)
+#pragma unroll
+            for (int k2 = 1; k2 < 4; ++k2) {
+                const int m = n1 * k2;
+                v[n1 + 4 * k2] = cmul(v[n1 + 4 * k2],
+                                      make_float2(C[m], sign * S[m]));
+            }
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2)
+            dft4(v[4 * k2], v[4 * k2 + 1], v[4 * k2 + 2], v[4 * k2 + 3], sign);
+    }
+};
+
+// Elements per thread: a tile of n x L complex values is spread over the T
+// threads of a block with n L <= kFastE T.
+constexpr int kFastE = 16;
+
+// One Stockham stage of radix R.  Thread (l, j0) = (tid % L, tid / L)
+// handles butterflies j = j0 + c JS (JS = T / L) of line l, so its LDS
+// addresses differ from slot to slot by compile-time offsets and from
+// element to element by uniform strides; k = j mod ns is carried from slot
+// to slot instead of divided out.
+template <int R, int LSH, int T>
+__device__ __forceinline__ void fstage(float2* buf, int a0, int l, int j0,
+                                       int nb, int ns, int twstep,
+                                       unsigned nsm, const float2* tw,
+                                       float sign)
+{
+    constexpr int LS = (1 << LSH) + 1;
+    constexpr int JS = T >> LSH;
+    constexpr int CAP = (kFastE + R - 1) / R;
+    const int rstride = nb * LS;              // uniform
+    const int wstride = ns * LS;              // uniform
+    const int jm = JS - ns * (int)fdiv((unsigned)JS, nsm);   // JS mod ns
+    const int jmt = jm * twstep, wrapt = ns * twstep;
+    int k = j0 - ns * (int)fdiv((unsigned)j0, nsm);
+    int kt = k * twstep;
+    float2 v[CAP][R];
+    int ks[CAP];
+#pragma unroll
+    for (int c = 0; c < CAP; ++c) {
+        ks[c] = k;
+        if (j0 + c * JS < nb) {
+            const int ra = a0 + c * JS * LS;
+#pragma unroll
+            for (int t = 0; t < R; ++t) v[c][t] = buf[ra + t * rstride];
+            if (k) {
+#pragma unroll
+                for (int t = 1; t < R; ++t)
+                    v[c][t] = cmul(v[c][t], twid(tw, t * kt, sign));
+            }
+            Bfly<R>::run(v[c], sign);
+        }
+        k += jm;
+        kt += jmt;
+        if (k >= ns) {
+            k -= ns;
+            kt -= wrapt;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < CAP; ++c) {
+        const int j = j0 + c * JS;
+        if (j < nb) {
+            const int wa = ((j - ks[c]) * R + ks[c]) * LS + l;
+#pragma unroll
+            for (int q = 0; q < R; ++q)
+                buf[wa + q * wstride] = v[c][Bfly<R>::perm(q)];
+        }
+    }
+    __syncthreads();
+}
+
+// Whole transform of the tile [m][L+1] (complete on entry and on return).
+template <int LSH, int T>
+__device__ void fast_fft(float2* buf, const FAxis& a, const float2* tw,
+                         float sign)
+{
+    constexpr int LS = (1 << LSH) + 1;
+    const int l = threadIdx.x & ((1 << LSH) - 1);
+    const int j0 = threadIdx.x >> LSH;
+    const int a0 = j0 * LS + l;
+#pragma unroll 1
+    for (int s = 0; s < a.nst; ++s) {
+        const int nb = a.nb[s], ns = a.ns[s], tws = a.tws[s];
+        const unsigned nsm = a.nsm[s];
+        switch (a.R[s]) {
+        case 2: fstage<2, LSH, T>(buf, a0, l, j0, nb, ns, tws, nsm, tw, sign); break;
+        case 3: fstage<3, LSH, T>(buf, a0, l, j0, nb, ns, tws, nsm, tw, sign); break;
+        case 4: fstage<4, LSH, T>(buf, a0, l, j0, nb, ns, tws, nsm, tw, sign); break;
+        case 5: fstage<5, LSH, T>(buf, a0, l, j0, nb, ns, tws, nsm, tw, sign); break;
+        case 8: fstage<8, LSH, T>(buf, a0, l, j0, nb, ns, tws, nsm, tw, sign); break;
+        default: fstage<16, LSH, T>(buf, a0, l, j0, nb, ns, tws, nsm, tw, sign); break;
+        }
+    }
+}
+
+// Launch of KERNEL<LSH, T> for the shape's runtime (lshift, threads).
+#define GBK_FAST_CASE(KERNEL, L_, T_, ...)                                   \
+    case (L_) * 2 + ((T_) == 512):                                           \
+        hipLaunchKernelGGL((KERNEL<L_, T_>), __VA_ARGS__);                   \
+        break;
+#define GBK_FAST_DISPATCH(SHAPE, KERNEL, GRID, LDS, STREAM, ...)             \
+    switch ((SHAPE).lshift * 2 + ((SHAPE).threads == 512)) {                 \
+    GBK_FAST_CASE(KERNEL, 0, 256, GRID, dim3(256), LDS, STREAM, __VA_ARGS__) \
+    GBK_FAST_CASE(KERNEL, 1, 256, GRID, dim3(256), LDS, STREAM, __VA_ARGS__) \
+    GBK_FAST_CASE(KERNEL, 2, 256, GRID, dim3(256), LDS, STREAM, __VA_ARGS__) \
+    GBK_FAST_CASE(KERNEL, 3, 256, GRID, dim3(256), LDS, STREAM, __VA_ARGS__) \
+    GBK_FAST_CASE(KERNEL, 4, 256, GRID, dim3(256), LDS, STREAM, __VA_ARGS__) \
+    GBK_FAST_CASE(KERNEL, 5, 256, GRID, dim3(256), LDS, STREAM, __VA_ARGS__) \
+    GBK_FAST_CASE(KERNEL, 6, 256, GRID, dim3(256), LDS, STREAM, __VA_ARGS__) \
+    GBK_FAST_CASE(KERNEL, 2, 512, GRID, dim3(512), LDS, STREAM, __VA_ARGS__) \
+    GBK_FAST_CASE(KERNEL, 3, 512, GRID, dim3(512), LDS, STREAM, __VA_ARGS__) \
+    GBK_FAST_CASE(KERNEL, 4, 512, GRID, dim3(512), LDS, STREAM, __VA_ARGS__) \
+    GBK_FAST_CASE(KERNEL, 5, 512, GRID, dim3(512), LDS, STREAM, __VA_ARGS__) \
+    GBK_FAST_CASE(KERNEL, 6, 512, GRID, dim3(512), LDS, STREAM, __VA_ARGS__) \
+    default:                                                                 \
+        throw Error{GBKHIP_ERR_INTERNAL, "no kernel for this FFT shape"};    \
+    }
+
+// Every (lshift, threads) pair the dispatch above instantiates.
+#define GBK_FAST_KERNELS(KERNEL)                                             \
+    (const void*)KERNEL<0, 256>, (const void*)KERNEL<1, 256>,                \
+    (const void*)KERNEL<2, 256>, (const void*)KERNEL<3, 256>,                \
+    (const void*)KERNEL<4, 256>, (const void*)KERNEL<5, 256>,                \
+    (const void*)KERNEL<6, 256>, (const void*)KERNEL<2, 512>,                \
+    (const void*)KERNEL<3, 512>, (const void*)KERNEL<4, 512>,                \
+    (const void*)KERNEL<5, 512>, (const void*)KERNEL<6, 512>
+
+}  // namespace gbkhip

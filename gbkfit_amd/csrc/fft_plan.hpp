@@ -28,12 +28,13 @@ struct FAxis {
 };
 
 // Launch shape of one axis on the register-tiled engine: lshift = log2 of
-// the lines per block, emax = elements held per thread (template bucket).
+// the lines per block, threads = block size (256 or 512), lds = bytes of
+// the twiddle table plus the [n][L+1] tile.
 struct FastShape {
     bool ok = false;
     FAxis a;
     int lshift = 0;
-    int emax = 0;
+    int threads = 256;
     size_t lds = 0;
 };
 

@@ -50,5 +50,18 @@ void fft_prepare_attributes();
 void fft_convolve_with(Context& c, const FFTPlan& p, float* data_r,
                        float2* spec, const float2* K, int batch, long long br,
                        long long bc, long long kbc);
+// register-tiled engine: shapes, twiddles and the strided line pass
+struct FastShape;
+FastShape fast_shape(int n);
+FastShape fast_shape_lines(int n, int lshift);
+FastShape fast_shape_rows(int n);
+unsigned div_magic(unsigned d);
+void upload_twiddles(DevBuf& buf, int n, int count);
+void launch_fx_lines(Context& c, const FastShape& f, const float2* tw,
+                     const float2* src, float2* dst, const float2* K,
+                     int nlines, int inner, long long base0,
+                     long long ostride, long long stride, int st_lo,
+                     int st_hi, int mode, float scale, int batch,
+                     long long bstride, long long kbstride);
 
 }  // namespace gbkhip

@@ -218,6 +218,9 @@ typedef struct {
     float err_scalar;        /* error used when err is NULL (1: none) */
     float weight;
     int max_batch;           /* sets evaluated per chunk (buffer sizing) */
+    int path;                /* 0: automatic (spectral-first fused pipeline
+                                when the disk is thin and the PSF x LSF cube
+                                is separable), 1: always FFT the padded cube */
 } gbkhip_batch_desc;
 
 typedef struct gbkhip_batch gbkhip_batch;
@@ -239,6 +242,9 @@ int gbkhip_batch_evaluate(gbkhip_batch* batch, const float* params,
  * of every chunk; stage_times() waits for them and returns the summed stage
  * durations (ms[3], milliseconds) of the last evaluate() and its chunk
  * count. */
+/* info[0] = 1 when the spectral-first fused pipeline runs, info[1..3] = the
+ * FFT sizes (x, y, z) in use, info[4] = chi^2 partial sums per set. */
+int gbkhip_batch_info(gbkhip_batch* batch, int* info);
 int gbkhip_batch_set_timing(gbkhip_batch* batch, int enable);
 int gbkhip_batch_stage_times(gbkhip_batch* batch, double* ms, int* chunks);
 

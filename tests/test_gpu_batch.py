@@ -77,3 +77,53 @@ def test_composition_independence(setup):
     for i in (0, 3, 5):
         alone = ev.evaluate([params[i]], use_graph=False)
         assert alone[0] == full[i]
+
+
+def test_fused_pipeline_selected(setup):
+    """Thin disk + separable PSF x LSF: the spectral-first pipeline runs on
+    the smallest 5-smooth sizes >= S + 2 edge (256 + 40, 256 + 40, 96 + 14)."""
+    _, ev, _, _, _ = setup
+    info = ev.info()
+    assert info['fused']
+    assert info['fft_size'] == (300, 300, 120)
+
+
+def test_fused_matches_padded_path(setup):
+    """Spectral-first fused pipeline vs FFT of the whole padded cube."""
+    from gbkfit_amd.batch import BatchEvaluator
+    drv, ev, sets, mock, model = setup
+    ref = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock,
+                         error=0.01, max_batch=4, path='padded')
+    assert not ref.info()['fused']
+    params = [P.gparams(p) for p in sets]
+    c_f, cubes_f = ev.evaluate(params, want_cubes=True, use_graph=False)
+    c_p, cubes_p = ref.evaluate(params, want_cubes=True, use_graph=False)
+    for i in range(len(sets)):
+        assert abs(c_f[i] - c_p[i]) <= 1e-5 * c_p[i], (i, c_f[i], c_p[i])
+        assert H.normwise(cubes_f[i], cubes_p[i]) < 1e-5, i
+    ref.close()
+
+
+def test_thick_disk_uses_padded_path():
+    """A thick disk runs the padded-cube path; chi^2 equals the plain path."""
+    from gbkfit_amd.batch import BatchEvaluator
+    from gbkfit_amd.objective import Data, Objective
+    drv = H.driver()
+    p = dict(P.c1_params(), thin=False, size=(32, 32, 24))
+    model, _ = P.build_product(drv, p)
+    ref, _, _ = P.oracle_cube(p)
+    rng = np.random.default_rng(3)
+    mock = (ref + rng.normal(0, 0.01, ref.shape)).astype(np.float32)
+    ev = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock,
+                        error=0.01, max_batch=2)
+    assert not ev.info()['fused']
+    sets = [P.gparams(dict(p, incl=i, posa=a)) for i, a in
+            ((40, 10), (60, 200), (75, 300))]
+    chi2 = ev.evaluate(sets, use_graph=False)
+    obj = Objective([dict(scube=Data(mock, error=np.full(mock.shape, 0.01,
+                                                          np.float32)))],
+                    [model])
+    for i, s in enumerate(sets):
+        plain = -2 * obj.log_likelihood(s)[0]
+        assert abs(plain - chi2[i]) <= 1e-5 * chi2[i], (i, plain, chi2[i])
+    ev.close()
