@@ -253,20 +253,22 @@ __global__ __launch_bounds__(256) void batch_lines_thin(
 
 // Spectral lines of L spaxels into an LDS tile, real FFT along z, half
 // spectrum -> S.
-template <int LSH, int T>
+template <int LSH, int T, int N = 0>
 __global__ __launch_bounds__(T) void fz_deposit_r2c(
     const float4* __restrict__ lines, long long lstride, gbkhip_grid g,
-    FAxis a, const float2* __restrict__ tw, const float2* __restrict__ tpz,
+    FAxis a, const float2* __restrict__ tw,
+    const float2* __restrict__ tws, const float2* __restrict__ tpz,
     int need_z, int nx, int ny, float2* __restrict__ S, long long sstride)
 {
     extern __shared__ float2 lds[];
     constexpr int L = 1 << LSH, LS = L + 1;
-    const int n = a.n;
+    const int n = N ? N : a.n;
     float2* s_tw = lds;
-    float2* buf = lds + n;
+    float2* buf = lds + tw_entries<N>(n);
     float2* s_post = buf + n * LS;      // tpz staged in LDS
     const int x0 = blockIdx.x << LSH, y = blockIdx.y, set = blockIdx.z;
-    for (int i = threadIdx.x; i < n; i += T) s_tw[i] = tw[i];
+    for (int i = threadIdx.x; i < tw_entries<N>(n); i += T)
+        s_tw[i] = N ? tws[i] : tw[i];
     for (int i = threadIdx.x; i <= n; i += T) s_post[i] = tpz[i];
     for (int i = threadIdx.x; i < n * LS; i += T) buf[i] = make_float2(0, 0);
     __syncthreads();
@@ -290,7 +292,7 @@ __global__ __launch_bounds__(T) void fz_deposit_r2c(
     __syncthreads();
     // z[m] = x[2m] + i x[2m+1]; complex FFT of length n = nz/2, then
     // X[k] = (Z[k] + conj Z[n-k]) / 2 + W^k (Z[k] - conj Z[n-k]) / (2i)
-    fast_fft<LSH, T>(buf, a, s_tw, -1.0f);
+    tile_fft<LSH, T, N>(buf, a, s_tw, -1.0f);
     float2* out = S + set * sstride + (size_t)y * nx + x0;
     const size_t plane = (size_t)nx * ny;
     const int cnt = (n + 1) << LSH;
@@ -310,22 +312,24 @@ __global__ __launch_bounds__(T) void fz_deposit_r2c(
 // Rows of S along x: forward FFT, kernel spectrum, inverse FFT.  Row r is
 // plane kz = r / ny, line ky = r % ny; element i of the block's contiguous
 // tile is (row i / n, x = i % n).
-template <int LSH, int T>
+template <int LSH, int T, int N = 0>
 __global__ __launch_bounds__(T) void fx_rows_conv(
     float2* __restrict__ S, long long sstride, int nrows, int ny,
     unsigned ny_m, FAxis a, unsigned n_m, const float2* __restrict__ tw,
+    const float2* __restrict__ tws,
     const float2* __restrict__ phat, const float2* __restrict__ lhat,
     int x_lo, int x_cnt, unsigned xcnt_m)
 {
     extern __shared__ float2 lds[];
     constexpr int L = 1 << LSH, LS = L + 1;
-    const int n = a.n;
+    const int n = N ? N : a.n;
     float2* s_tw = lds;
-    float2* buf = lds + n;
+    float2* buf = lds + tw_entries<N>(n);
     const int r0 = blockIdx.x << LSH;
     const int nl = min(L, nrows - r0);
     float2* rows = S + blockIdx.y * sstride + (size_t)r0 * n;
-    for (int i = threadIdx.x; i < n; i += T) s_tw[i] = tw[i];
+    for (int i = threadIdx.x; i < tw_entries<N>(n); i += T)
+        s_tw[i] = N ? tws[i] : tw[i];
     const int tile = n << LSH, valid = n * nl;
     float2 r[kFastE];
 #pragma unroll
@@ -342,7 +346,7 @@ __global__ __launch_bounds__(T) void fx_rows_conv(
         }
     }
     __syncthreads();
-    fast_fft<LSH, T>(buf, a, s_tw, -1.0f);
+    tile_fft<LSH, T, N>(buf, a, s_tw, -1.0f);
 #pragma unroll
     for (int c = 0; c < kFastE; ++c) {
         const int i = threadIdx.x + c * T;
@@ -363,7 +367,7 @@ __global__ __launch_bounds__(T) void fx_rows_conv(
         }
     }
     __syncthreads();
-    fast_fft<LSH, T>(buf, a, s_tw, 1.0f);
+    tile_fft<LSH, T, N>(buf, a, s_tw, 1.0f);
     const int cnt = nl * x_cnt;
     for (int i = threadIdx.x; i < cnt; i += T) {
         const int l = (int)fdiv((unsigned)i, xcnt_m), m = x_lo + i - l * x_cnt;
@@ -375,27 +379,30 @@ __global__ __launch_bounds__(T) void fx_rows_conv(
 // downscale, residual and FP64 partial sum of r^2.  The tile is kx x ky
 // low-res spaxels = tx x ty hi-res columns (tx = kx sx, ty = ky sy), column
 // (lx, ly) on line l = ly * tx + lx.
-template <int LSH, int T>
+template <int LSH, int T, int N = 0>
 __global__ __launch_bounds__(T) void fz_c2r_down_chi2(
     const float2* __restrict__ S, long long sstride, int nx, int ny, FAxis a,
-    const float2* __restrict__ tw, const float2* __restrict__ tpz, int x_lo,
+    const float2* __restrict__ tw,
+    const float2* __restrict__ tws, const float2* __restrict__ tpz, int x_lo,
     int y_lo, int kx, int ky, int sx, int sy, int sz, int ez, int nxl,
-    int nyl, int nzl, int tiles_x, DataArgs da, float* __restrict__ lo_out,
-    long long lstride, double* __restrict__ partials, int nparts)
+    int nyl, int nzl, int tiles_x, unsigned pz_m, unsigned kx_m, DataArgs da,
+    float* __restrict__ lo_out, long long lstride,
+    double* __restrict__ partials, int nparts)
 {
     extern __shared__ float2 lds[];
     __shared__ double red[T / 64];
     constexpr int L = 1 << LSH, LS = L + 1;
-    const int n = a.n;
+    const int n = N ? N : a.n;
     const int tx = kx * sx, ty = ky * sy;
     float2* s_tw = lds;
-    float2* buf = lds + n;
+    float2* buf = lds + tw_entries<N>(n);
     float2* s_post = buf + n * LS;      // tpz staged in LDS
     const int set = blockIdx.y;
     const int X0 = (blockIdx.x % tiles_x) * kx, Y0 = (blockIdx.x / tiles_x) * ky;
     const float2* src = S + set * sstride;
     const size_t plane = (size_t)nx * ny;
-    for (int i = threadIdx.x; i < n; i += T) s_tw[i] = tw[i];
+    for (int i = threadIdx.x; i < tw_entries<N>(n); i += T)
+        s_tw[i] = N ? tws[i] : tw[i];
     for (int i = threadIdx.x; i <= n; i += T) s_post[i] = tpz[i];
     __syncthreads();
     // Z[k] = (X[k] + conj X[n-k]) + i W^-k (X[k] - conj X[n-k]), k < n
@@ -426,7 +433,7 @@ __global__ __launch_bounds__(T) void fz_c2r_down_chi2(
         }
     }
     __syncthreads();
-    fast_fft<LSH, T>(buf, a, s_tw, 1.0f);
+    tile_fft<LSH, T, N>(buf, a, s_tw, 1.0f);
     const float* fb = reinterpret_cast<const float*>(buf);
     const float nf = 1.0f / (float)(sx * sy * sz);
     const int per_z = kx * ky;
@@ -437,17 +444,25 @@ __global__ __launch_bounds__(T) void fz_c2r_down_chi2(
     constexpr int kV = 8;
     for (int v0 = 0; v0 < nvox; v0 += kV * T) {
         long long gi[kV];
+        int lb[kV], zb[kV];
         float ob[kV], er[kV], mk[kV];
 #pragma unroll
         for (int j = 0; j < kV; ++j) {
             const int v = v0 + threadIdx.x + j * T;
             gi[j] = -1;
+            lb[j] = zb[j] = 0;
             if (v < nvox) {
-                const int zl = v / per_z, rem = v - zl * per_z;
-                const int lyo = rem / kx, lxo = rem - lyo * kx;
+                // v = (zl * ky + lyo) * kx + lxo
+                const int zl = (int)fdiv((unsigned)v, pz_m);
+                const int rem = v - zl * per_z;
+                const int lyo = (int)fdiv((unsigned)rem, kx_m);
+                const int lxo = rem - lyo * kx;
                 const int X = X0 + lxo, Y = Y0 + lyo;
-                if (X < nxl && Y < nyl)
+                if (X < nxl && Y < nyl) {
                     gi[j] = ((long long)zl * nyl + Y) * nxl + X;
+                    lb[j] = lyo * sy * tx + lxo * sx;
+                    zb[j] = ez + zl * sz;
+                }
             }
             ob[j] = gi[j] >= 0 ? da.obs[gi[j]] : 0.0f;
             er[j] = gi[j] >= 0 && da.err ? da.err[gi[j]] : da.err_scalar;
@@ -456,17 +471,13 @@ __global__ __launch_bounds__(T) void fz_c2r_down_chi2(
 #pragma unroll
         for (int j = 0; j < kV; ++j) {
             if (gi[j] < 0) continue;
-            const int v = v0 + threadIdx.x + j * T;
-            const int zl = v / per_z, rem = v - zl * per_z;
-            const int lyo = rem / kx, lxo = rem - lyo * kx;
             float sum = 0.0f;
             for (int dz = 0; dz < sz; ++dz) {
-                const int z = ez + zl * sz + dz;
+                const int z = zb[j] + dz;
+                const int zo = 2 * ((z >> 1) * LS) + (z & 1);
                 for (int dy = 0; dy < sy; ++dy)
-                    for (int dx = 0; dx < sx; ++dx) {
-                        const int l = (lyo * sy + dy) * tx + lxo * sx + dx;
-                        sum += fb[2 * ((z >> 1) * LS + l) + (z & 1)];
-                    }
+                    for (int dx = 0; dx < sx; ++dx)
+                        sum += fb[zo + 2 * (lb[j] + dy * tx + dx)];
             }
             const float m = sum * nf;
             if (lo_out) lo_out[set * lstride + gi[j]] = m;
@@ -495,6 +506,7 @@ struct Fused {
     int xlo = 0, xcnt = 0, ylo = 0, ycnt = 0;
     FastShape fz, fy, fx;                  // z columns, y lines, x rows
     DevBuf twz, twy, twx, tpz;
+    DevBuf twsz, twsy, twsx;               // per-stage tables
     DevBuf phat, lhat;
     DevBuf S;
     long long sstride = 0;
@@ -767,6 +779,9 @@ static void plan_fused(Context& c, Batch& B)
     upload_twiddles(F.tpz, nz, nzh);
     upload_twiddles(F.twy, ny, ny);
     upload_twiddles(F.twx, nx, nx);
+    upload_stage_twiddles(F.twsz, nz / 2);
+    upload_stage_twiddles(F.twsy, ny);
+    upload_stage_twiddles(F.twsx, nx);
     F.sstride = (long long)nzh * ny * nx;
     F.S.alloc((size_t)D.max_batch * F.sstride * sizeof(float2));
     F.lstride = (long long)ny * nx;
@@ -802,29 +817,32 @@ static void run_chunk_fused(Batch& b, const float* params, int set0,
                            F.need[1], F.nx, F.ny, lines, F.lstride);
         GBK_HIP(hipGetLastError());
         dim3 grid(ceil_div(F.nx, 1 << F.fz.lshift), F.ny, nsets);
-        GBK_FAST_DISPATCH(F.fz, fz_deposit_r2c, grid,
-                          F.fz.lds + (F.fz.a.n + 1) * sizeof(float2),
-                          c.stream, lines, F.lstride, D.grid, F.fz.a,
-                          F.twz.as<float2>(), F.tpz.as<float2>(), F.need[2],
-                          F.nx, F.ny, S, F.sstride);
+        GBK_DISPATCH(GBK_SPEC_Z, F.fz, fz_deposit_r2c, grid,
+                     F.fz.lds + (F.fz.a.n + 1) * sizeof(float2), c.stream,
+                     lines, F.lstride, D.grid, F.fz.a, F.twz.as<float2>(),
+                     F.twsz.as<float2>(), F.tpz.as<float2>(), F.need[2],
+                     F.nx, F.ny, S, F.sstride);
         GBK_HIP(hipGetLastError());
     }
     mark(b, ci, 1);
-    launch_fx_lines(c, F.fy, F.twy.as<float2>(), S, S, nullptr,
+    launch_fx_lines(c, F.fy, F.twy.as<float2>(), F.twsy.as<float2>(), S, S,
+                    nullptr,
                     F.nzh * F.nx, F.nx, 0, plane, F.nx, 0, F.ny, 0, 1.0f,
                     nsets, F.sstride, 0);
     {
         const int nrows = F.nzh * F.ny;
         dim3 grid(ceil_div(nrows, 1 << F.fx.lshift), nsets);
-        GBK_FAST_DISPATCH(F.fx, fx_rows_conv, grid, F.fx.lds, c.stream, S,
-                          F.sstride, nrows, F.ny, div_magic((unsigned)F.ny),
-                          F.fx.a, div_magic((unsigned)F.nx),
-                          F.twx.as<float2>(), F.phat.as<float2>(),
+        GBK_DISPATCH(GBK_SPEC_ROWS, F.fx, fx_rows_conv, grid, F.fx.lds,
+                     c.stream, S, F.sstride, nrows, F.ny,
+                     div_magic((unsigned)F.ny), F.fx.a,
+                     div_magic((unsigned)F.nx), F.twx.as<float2>(),
+                     F.twsx.as<float2>(), F.phat.as<float2>(),
                           F.lhat.as<float2>(), F.xlo, F.xcnt,
                           div_magic((unsigned)F.xcnt));
         GBK_HIP(hipGetLastError());
     }
-    launch_fx_lines(c, F.fy, F.twy.as<float2>(), S, S, nullptr,
+    launch_fx_lines(c, F.fy, F.twy.as<float2>(), F.twsy.as<float2>(), S, S,
+                    nullptr,
                     F.nzh * F.xcnt, F.xcnt, F.xlo, plane, F.nx, F.ylo,
                     F.ylo + F.ycnt, 1, 1.0f, nsets, F.sstride, 0);
     mark(b, ci, 2);
@@ -836,13 +854,15 @@ static void run_chunk_fused(Batch& b, const float* params, int set0,
     da.weight = D.weight;
     {
         dim3 grid(F.tiles_x * F.tiles_y, nsets);
-        GBK_FAST_DISPATCH(F.fz, fz_c2r_down_chi2, grid,
-                          F.fz.lds + (F.fz.a.n + 1) * sizeof(float2),
-                          c.stream, S, F.sstride, F.nx, F.ny, F.fz.a,
-                          F.twz.as<float2>(), F.tpz.as<float2>(), F.xlo,
+        GBK_DISPATCH(GBK_SPEC_Z, F.fz, fz_c2r_down_chi2, grid,
+                     F.fz.lds + (F.fz.a.n + 1) * sizeof(float2), c.stream, S,
+                     F.sstride, F.nx, F.ny, F.fz.a, F.twz.as<float2>(),
+                     F.twsz.as<float2>(), F.tpz.as<float2>(), F.xlo,
                           F.ylo, F.kx, F.ky, D.scale[0], D.scale[1],
                           D.scale[2], D.edge_hi[2], D.size_lo[0],
-                          D.size_lo[1], D.size_lo[2], F.tiles_x, da,
+                          D.size_lo[1], D.size_lo[2], F.tiles_x,
+                          div_magic((unsigned)(F.kx * F.ky)),
+                          div_magic((unsigned)F.kx), da,
                           cubes ? cubes + (size_t)set0 * b.n_lo : nullptr,
                           b.n_lo, b.partials.as<double>(), b.nparts);
         GBK_HIP(hipGetLastError());

@@ -51,6 +51,27 @@ static Axis make_axis(int n)
     return a;
 }
 
+// Per-stage twiddle table of the compile-time plan of length n:
+// [stage][k][t - 1] = exp(-2 pi i t k / (NS R)), FP64 rounded once to FP32.
+void upload_stage_twiddles(DevBuf& buf, int n)
+{
+    const CPlan P = cplan(n);
+    std::vector<float> h(2 * (size_t)std::max(P.tw_total, 1), 0.0f);
+    for (int s = 0; s < P.nst; ++s)
+        for (int k = 0; k < P.NS[s]; ++k)
+            for (int t = 1; t < P.R[s]; ++t) {
+                const double ang = -2.0 * M_PI * (double)t * k /
+                                   ((double)P.NS[s] * P.R[s]);
+                const size_t i = P.tw_off[s] + (size_t)k * (P.R[s] - 1) +
+                                 (t - 1);
+                h[2 * i] = (float)std::cos(ang);
+                h[2 * i + 1] = (float)std::sin(ang);
+            }
+    buf.alloc(h.size() * sizeof(float));
+    GBK_HIP(hipMemcpy(buf.ptr, h.data(), h.size() * sizeof(float),
+                      hipMemcpyHostToDevice));
+}
+
 void upload_twiddles(DevBuf& buf, int n, int count)
 {
     std::vector<float> h(2 * (size_t)count);
@@ -113,7 +134,8 @@ FastShape fast_shape_lines(int n, int lshift)
         ns *= R[s];
     }
     const long long tile = (long long)n << lshift;
-    const size_t lds = (size_t)n * ((1 << lshift) + 2) * sizeof(float2);
+    const size_t lds = ((size_t)std::max(n, cplan(n).tw_total) +
+                        (size_t)n * ((1 << lshift) + 1)) * sizeof(float2);
     if (lds > 96 * 1024) return f;
     if (tile <= (long long)kFastE * 256) f.threads = 256;
     else if (tile <= (long long)kFastE * 512 && lshift >= 2) f.threads = 512;
@@ -178,6 +200,9 @@ std::shared_ptr<FFTPlan> get_plan(Context& c, int nx, int ny, int nz)
     if (p->even) p->fx = fast_shape_rows(rn);
     p->fy = fast_shape(ny);
     p->fz = fast_shape(nz);
+    if (p->fx.ok) upload_stage_twiddles(p->twsx, rn);
+    if (p->fy.ok) upload_stage_twiddles(p->twsy, ny);
+    if (p->fz.ok) upload_stage_twiddles(p->twsz, nz);
     c.plans[key] = p;
     return p;
 }
@@ -472,23 +497,24 @@ __global__ __launch_bounds__(256) void fft_lines(
 // mode 0: forward, 1: inverse, 2: forward * K * scale then inverse.  Only
 // elements m in [st_lo, st_hi) are stored.  Thread (l, m0) = (tid % L,
 // tid / L) moves elements m0 + c T/L of line l.
-template <int LSH, int T>
+template <int LSH, int T, int N = 0>
 __global__ __launch_bounds__(T) void fx_lines(
     const float2* __restrict__ src, float2* __restrict__ dst,
     const float2* __restrict__ K, int nlines, int inner, unsigned inner_m,
     long long base0, long long ostride, long long stride, FAxis a,
-    const float2* tw, int mode, float scale, int st_lo, int st_hi,
+    const float2* tw, const float2* tws, int mode, float scale, int st_lo, int st_hi,
     long long bstride, long long kbstride)
 {
     extern __shared__ float2 lds[];
     constexpr int L = 1 << LSH, LS = L + 1, JS = T >> LSH;
-    const int n = a.n;
+    const int n = N ? N : a.n;
     float2* s_tw = lds;
-    float2* buf = lds + n;
+    float2* buf = lds + tw_entries<N>(n);
     src += blockIdx.y * bstride;
     dst += blockIdx.y * bstride;
     if (K) K += blockIdx.y * kbstride;
-    for (int i = threadIdx.x; i < n; i += T) s_tw[i] = tw[i];
+    for (int i = threadIdx.x; i < tw_entries<N>(n); i += T)
+        s_tw[i] = N ? tws[i] : tw[i];
     const int l = threadIdx.x & (L - 1);
     const int m0 = threadIdx.x >> LSH;
     const int q = (blockIdx.x << LSH) + l;
@@ -508,7 +534,7 @@ __global__ __launch_bounds__(T) void fx_lines(
     for (int c = 0; c < kFastE; ++c)
         if (m0 + c * JS < n) buf[la + c * JS * LS] = r[c];
     __syncthreads();
-    fast_fft<LSH, T>(buf, a, s_tw, mode == 1 ? 1.0f : -1.0f);
+    tile_fft<LSH, T, N>(buf, a, s_tw, mode == 1 ? 1.0f : -1.0f);
     if (mode == 2) {
 #pragma unroll
         for (int c = 0; c < kFastE; ++c)
@@ -523,7 +549,7 @@ __global__ __launch_bounds__(T) void fx_lines(
                                 (v.x * kv.y + v.y * kv.x) * scale);
             }
         __syncthreads();
-        fast_fft<LSH, T>(buf, a, s_tw, 1.0f);
+        tile_fft<LSH, T, N>(buf, a, s_tw, 1.0f);
     }
     if (valid) {
 #pragma unroll
@@ -539,25 +565,26 @@ __global__ __launch_bounds__(T) void fx_lines(
 // over z[m] = x[2m] + i x[2m+1], then
 // X[k] = (Z[k] + conj Z[n-k]) / 2 + W^k (Z[k] - conj Z[n-k]) / (2i).
 // Element i of the block's contiguous tile is (row i / n, m = i % n).
-template <int LSH, int T>
+template <int LSH, int T, int N = 0>
 __global__ __launch_bounds__(T) void fx_rows_r2c(
     const float* __restrict__ in, float2* __restrict__ out, int nrows,
-    FAxis a, unsigned n_m, unsigned nxh_m, const float2* tw,
+    FAxis a, unsigned n_m, unsigned nxh_m, const float2* tw, const float2* tws,
     const float2* __restrict__ twpost, long long bstride_r,
     long long bstride_c)
 {
     extern __shared__ float2 lds[];
     constexpr int L = 1 << LSH, LS = L + 1;
-    const int n = a.n, nxh = n + 1;
+    const int n = N ? N : a.n, nxh = n + 1;
     float2* s_tw = lds;
-    float2* buf = lds + n;
+    float2* buf = lds + tw_entries<N>(n);
     float2* s_post = buf + n * LS;      // twpost staged in LDS
     const int r0 = blockIdx.x << LSH;
     const int nl = min(L, nrows - r0);
     const float2* in2 = reinterpret_cast<const float2*>(
                             in + blockIdx.y * bstride_r) + (size_t)r0 * n;
     out += blockIdx.y * bstride_c + (size_t)r0 * nxh;
-    for (int i = threadIdx.x; i < n; i += T) s_tw[i] = tw[i];
+    for (int i = threadIdx.x; i < tw_entries<N>(n); i += T)
+        s_tw[i] = N ? tws[i] : tw[i];
     for (int i = threadIdx.x; i < nxh; i += T) s_post[i] = twpost[i];
     const int tile = n << LSH;
     const int valid = n * nl;
@@ -576,7 +603,7 @@ __global__ __launch_bounds__(T) void fx_rows_r2c(
         }
     }
     __syncthreads();
-    fast_fft<LSH, T>(buf, a, s_tw, -1.0f);
+    tile_fft<LSH, T, N>(buf, a, s_tw, -1.0f);
     const int cnt = nl * nxh;
     for (int i = threadIdx.x; i < cnt; i += T) {
         const int l = (int)fdiv((unsigned)i, nxh_m), k = i - l * nxh;
@@ -592,25 +619,26 @@ __global__ __launch_bounds__(T) void fx_rows_r2c(
 
 // Row pass, half spectrum -> real, nx even (mirror of fx_rows_r2c):
 // Z[k] = (X[k] + conj X[n-k]) + i W^-k (X[k] - conj X[n-k]).
-template <int LSH, int T>
+template <int LSH, int T, int N = 0>
 __global__ __launch_bounds__(T) void fx_rows_c2r(
     const float2* __restrict__ in, float* __restrict__ out, int nrows,
-    FAxis a, unsigned n_m, const float2* tw,
+    FAxis a, unsigned n_m, const float2* tw, const float2* tws,
     const float2* __restrict__ twpost, long long bstride_c,
     long long bstride_r)
 {
     extern __shared__ float2 lds[];
     constexpr int L = 1 << LSH, LS = L + 1;
-    const int n = a.n, nxh = n + 1;
+    const int n = N ? N : a.n, nxh = n + 1;
     float2* s_tw = lds;
-    float2* buf = lds + n;
+    float2* buf = lds + tw_entries<N>(n);
     float2* s_post = buf + n * LS;      // twpost staged in LDS
     const int r0 = blockIdx.x << LSH;
     const int nl = min(L, nrows - r0);
     in += blockIdx.y * bstride_c + (size_t)r0 * nxh;
     float2* out2 = reinterpret_cast<float2*>(out + blockIdx.y * bstride_r) +
                    (size_t)r0 * n;
-    for (int i = threadIdx.x; i < n; i += T) s_tw[i] = tw[i];
+    for (int i = threadIdx.x; i < tw_entries<N>(n); i += T)
+        s_tw[i] = N ? tws[i] : tw[i];
     for (int i = threadIdx.x; i < nxh; i += T) s_post[i] = twpost[i];
     __syncthreads();
     const int tile = n << LSH;
@@ -639,7 +667,7 @@ __global__ __launch_bounds__(T) void fx_rows_c2r(
         }
     }
     __syncthreads();
-    fast_fft<LSH, T>(buf, a, s_tw, 1.0f);
+    tile_fft<LSH, T, N>(buf, a, s_tw, 1.0f);
     const int cnt = nl * n;
     for (int i = threadIdx.x; i < cnt; i += T) {
         const int l = (int)fdiv((unsigned)i, n_m), m = i - l * n;
@@ -670,7 +698,8 @@ constexpr size_t kLdsBudget = 64 * 1024;
 static int pad_ld(int n) { return n + (n % 2 == 0 ? 1 : 0); }
 
 void launch_fx_lines(Context& c, const FastShape& f, const float2* tw,
-                     const float2* src, float2* dst, const float2* K,
+                     const float2* tws, const float2* src, float2* dst,
+                     const float2* K,
                      int nlines, int inner, long long base0,
                      long long ostride, long long stride, int st_lo,
                      int st_hi, int mode, float scale, int batch,
@@ -681,9 +710,9 @@ void launch_fx_lines(Context& c, const FastShape& f, const float2* tw,
               st_hi <= f.a.n, GBKHIP_ERR_INTERNAL, "bad line pass shape");
     const unsigned inner_m = inner >= nlines ? 0u : div_magic((unsigned)inner);
     dim3 grid(ceil_div(nlines, 1 << f.lshift), batch);
-    GBK_FAST_DISPATCH(f, fx_lines, grid, f.lds, c.stream, src, dst, K,
-                      nlines, inner, inner_m, base0, ostride, stride, f.a, tw,
-                      mode, scale, st_lo, st_hi, bstride, kbstride);
+    GBK_DISPATCH(GBK_SPEC_LINES, f, fx_lines, grid, f.lds, c.stream, src,
+                 dst, K, nlines, inner, inner_m, base0, ostride, stride, f.a,
+                 tw, tws, mode, scale, st_lo, st_hi, bstride, kbstride);
     GBK_HIP(hipGetLastError());
 }
 
@@ -699,14 +728,15 @@ static void launch_rows_fast(Context& c, const FFTPlan& p, bool forward,
     const size_t lds = f.lds + (size_t)(n + 1) * sizeof(float2);
     dim3 grid(ceil_div(nrows, 1 << f.lshift), batch);
     if (forward) {
-        GBK_FAST_DISPATCH(f, fx_rows_r2c, grid, lds, c.stream,
-                          (const float*)in, (float2*)out, nrows, f.a, n_m,
-                          nxh_m, p.twx.as<float2>(), p.twpost.as<float2>(),
-                          br, bc);
+        GBK_DISPATCH(GBK_SPEC_ROWS, f, fx_rows_r2c, grid, lds, c.stream,
+                     (const float*)in, (float2*)out, nrows, f.a, n_m, nxh_m,
+                     p.twx.as<float2>(), p.twsx.as<float2>(),
+                     p.twpost.as<float2>(), br, bc);
     } else {
-        GBK_FAST_DISPATCH(f, fx_rows_c2r, grid, lds, c.stream,
-                          (const float2*)in, (float*)out, nrows, f.a, n_m,
-                          p.twx.as<float2>(), p.twpost.as<float2>(), bc, br);
+        GBK_DISPATCH(GBK_SPEC_ROWS, f, fx_rows_c2r, grid, lds, c.stream,
+                     (const float2*)in, (float*)out, nrows, f.a, n_m,
+                     p.twx.as<float2>(), p.twsx.as<float2>(),
+                     p.twpost.as<float2>(), bc, br);
     }
     GBK_HIP(hipGetLastError());
 }
@@ -764,8 +794,10 @@ static void launch_lines(Context& c, const FFTPlan& p, int axis, int mode,
         const unsigned inner_m = axis == 1 ? div_magic((unsigned)inner) : 0u;
         const long long os = axis == 1 ? ostride : 0;
         dim3 grid(ceil_div(nlines, 1 << f.lshift), batch);
-        launch_fx_lines(c, f, tw, src, dst, K, (int)nlines, inner, 0, os,
-                        stride, 0, f.a.n, mode, scale, batch, bstride,
+        const float2* tws = axis == 1 ? p.twsy.as<float2>()
+                                      : p.twsz.as<float2>();
+        launch_fx_lines(c, f, tw, tws, src, dst, K, (int)nlines, inner, 0,
+                        os, stride, 0, f.a.n, mode, scale, batch, bstride,
                         kbstride);
         return;
     }

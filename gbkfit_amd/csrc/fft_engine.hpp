@@ -251,10 +251,161 @@ __device__ void fast_fft(float2* buf, const FAxis& a, const float2* tw,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Compile-time plans: for the lengths in the specialisation lists below the
+// radices, strides and slot counts are constants, LDS offsets become
+// instruction immediates and each stage reads its twiddles from a table
+// laid out [k][t - 1], tw_s[k][t - 1] = exp(-2 pi i t k / (NS_s R_s)).
+// ---------------------------------------------------------------------------
+struct CPlan {
+    int nst = 0;
+    int R[kFMax] = {};
+    int NS[kFMax] = {};
+    int tw_off[kFMax] = {};
+    int tw_total = 0;
+};
+
+// Same radix choice as the runtime planner (fast_radices in fft.hip).
+__host__ __device__ constexpr CPlan cplan(int n)
+{
+    CPlan P;
+    int m = n, a2 = 0, a3 = 0, a5 = 0;
+    while (m % 2 == 0) { m /= 2; ++a2; }
+    while (m % 3 == 0) { m /= 3; ++a3; }
+    while (m % 5 == 0) { m /= 5; ++a5; }
+    if (m != 1) return P;
+    const int s2 = (a2 + 3) / 4;
+    for (int s = 0; s < s2; ++s) {
+        const int bits = a2 / s2 + (s < a2 % s2 ? 1 : 0);
+        P.R[P.nst++] = 1 << bits;
+    }
+    for (int i = 0; i < a3; ++i) P.R[P.nst++] = 3;
+    for (int i = 0; i < a5; ++i) P.R[P.nst++] = 5;
+    int ns = 1, off = 0;
+    for (int s = 0; s < P.nst; ++s) {
+        P.NS[s] = ns;
+        P.tw_off[s] = off;
+        off += ns * (P.R[s] - 1);
+        ns *= P.R[s];
+    }
+    P.tw_total = off;
+    return P;
+}
+
+// The plan of length N as a compile-time constant (no object at run time).
+template <int N> struct CP {
+    static constexpr CPlan P = cplan(N);
+};
+
+template <int N, int LSH, int T, int S>
+__device__ __forceinline__ void cstage(float2* buf, int a0, int j0,
+                                       const float2* twst, float sign)
+{
+    constexpr int R = CP<N>::P.R[S], NS = CP<N>::P.NS[S], NB = N / R;
+    constexpr int LS = (1 << LSH) + 1, JS = T >> LSH;
+    constexpr int CAP = (NB + JS - 1) / JS;
+    constexpr int JM = JS % NS;
+    constexpr int TO = CP<N>::P.tw_off[S];
+    int k = NS == 1 ? 0 : j0 % NS;
+    float2 v[CAP][R];
+    int ks[CAP];
+#pragma unroll
+    for (int c = 0; c < CAP; ++c) {
+        ks[c] = k;
+        if ((c + 1) * JS <= NB || j0 + c * JS < NB) {
+#pragma unroll
+            for (int t = 0; t < R; ++t)
+                v[c][t] = buf[a0 + (c * JS + t * NB) * LS];
+            if (NS > 1) {
+                const float2* w = twst + TO + k * (R - 1);
+#pragma unroll
+                for (int t = 1; t < R; ++t) {
+                    const float2 wt = w[t - 1];
+                    v[c][t] = cmul(v[c][t], sign < 0 ? wt : conjf2(wt));
+                }
+            }
+            Bfly<R>::run(v[c], sign);
+        }
+        if (NS > 1) {
+            k += JM;
+            if (k >= NS) k -= NS;
+        }
+    }
+    __syncthreads();
+    // write index (j - k) R + k with j = j0 + c JS
+    const int w0 = a0 + j0 * (R - 1) * LS;
+#pragma unroll
+    for (int c = 0; c < CAP; ++c) {
+        if ((c + 1) * JS <= NB || j0 + c * JS < NB) {
+            const int wa = w0 + c * JS * R * LS - ks[c] * (R - 1) * LS;
+#pragma unroll
+            for (int q = 0; q < R; ++q)
+                buf[wa + q * NS * LS] = v[c][Bfly<R>::perm(q)];
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int LSH, int T, int S = 0>
+__device__ __forceinline__ void cfft_stages(float2* buf, int a0, int j0,
+                                            const float2* twst, float sign)
+{
+    if constexpr (S < CP<N>::P.nst) {
+        cstage<N, LSH, T, S>(buf, a0, j0, twst, sign);
+        cfft_stages<N, LSH, T, S + 1>(buf, a0, j0, twst, sign);
+    }
+}
+
+// Whole transform of the tile: the runtime engine (N = 0, twiddles `tw` =
+// exp(-2 pi i m / n)) or the compile-time one (twiddles = stage table).
+template <int LSH, int T, int N>
+__device__ __forceinline__ void tile_fft(float2* buf, const FAxis& a,
+                                         const float2* tw, float sign)
+{
+    if constexpr (N == 0) {
+        fast_fft<LSH, T>(buf, a, tw, sign);
+    } else {
+        constexpr int LS = (1 << LSH) + 1;
+        const int l = threadIdx.x & ((1 << LSH) - 1);
+        const int j0 = threadIdx.x >> LSH;
+        cfft_stages<N, LSH, T>(buf, j0 * LS + l, j0, tw, sign);
+    }
+}
+
+// Twiddle entries a kernel stages in LDS (n = N when N > 0).
+template <int N>
+__device__ __forceinline__ constexpr int tw_entries(int n)
+{
+    if constexpr (N == 0) return n;
+    else return CP<N>::P.tw_total > N ? CP<N>::P.tw_total : N;
+}
+
+// Specialised shapes (n, lshift, threads) per kernel family: the sizes of
+// the benchmark workloads (fused pipeline 300^2 x 120, padded cubes
+// 320^2 x 128 and 192^2 x 96).
+#define GBK_SPEC_LINES(X, ...)                                               \
+    X(300, 4, 512, __VA_ARGS__) X(320, 4, 512, __VA_ARGS__)                  \
+    X(128, 5, 256, __VA_ARGS__) X(192, 4, 256, __VA_ARGS__)                  \
+    X(96, 5, 256, __VA_ARGS__)
+#define GBK_SPEC_ROWS(X, ...)                                                \
+    X(160, 4, 256, __VA_ARGS__) X(96, 5, 256, __VA_ARGS__)                   \
+    X(300, 3, 256, __VA_ARGS__)
+#define GBK_SPEC_Z(X, ...) X(60, 6, 256, __VA_ARGS__)
+
+#define GBK_SPEC_LAUNCH(N_, L_, T_, SHAPE, KERNEL, GRID, LDS, STREAM, ...)   \
+    if ((SHAPE).a.n == (N_) && (SHAPE).lshift == (L_) &&                     \
+        (SHAPE).threads == (T_))                                             \
+        hipLaunchKernelGGL((KERNEL<L_, T_, N_>), GRID, dim3(T_), LDS,        \
+                           STREAM, __VA_ARGS__);                             \
+    else
+#define GBK_SPEC_MATCH(N_, L_, T_, SHAPE)                                    \
+    || ((SHAPE).a.n == (N_) && (SHAPE).lshift == (L_) &&                     \
+        (SHAPE).threads == (T_))
+
 // Launch of KERNEL<LSH, T> for the shape's runtime (lshift, threads).
 #define GBK_FAST_CASE(KERNEL, L_, T_, ...)                                   \
     case (L_) * 2 + ((T_) == 512):                                           \
-        hipLaunchKernelGGL((KERNEL<L_, T_>), __VA_ARGS__);                   \
+        hipLaunchKernelGGL((KERNEL<L_, T_, 0>), __VA_ARGS__);                \
         break;
 #define GBK_FAST_DISPATCH(SHAPE, KERNEL, GRID, LDS, STREAM, ...)             \
     switch ((SHAPE).lshift * 2 + ((SHAPE).threads == 512)) {                 \
@@ -276,11 +427,17 @@ __device__ void fast_fft(float2* buf, const FAxis& a, const float2* tw,
 
 // Every (lshift, threads) pair the dispatch above instantiates.
 #define GBK_FAST_KERNELS(KERNEL)                                             \
-    (const void*)KERNEL<0, 256>, (const void*)KERNEL<1, 256>,                \
-    (const void*)KERNEL<2, 256>, (const void*)KERNEL<3, 256>,                \
-    (const void*)KERNEL<4, 256>, (const void*)KERNEL<5, 256>,                \
-    (const void*)KERNEL<6, 256>, (const void*)KERNEL<2, 512>,                \
-    (const void*)KERNEL<3, 512>, (const void*)KERNEL<4, 512>,                \
-    (const void*)KERNEL<5, 512>, (const void*)KERNEL<6, 512>
+    (const void*)KERNEL<0, 256, 0>, (const void*)KERNEL<1, 256, 0>,          \
+    (const void*)KERNEL<2, 256, 0>, (const void*)KERNEL<3, 256, 0>,          \
+    (const void*)KERNEL<4, 256, 0>, (const void*)KERNEL<5, 256, 0>,          \
+    (const void*)KERNEL<6, 256, 0>, (const void*)KERNEL<2, 512, 0>,          \
+    (const void*)KERNEL<3, 512, 0>, (const void*)KERNEL<4, 512, 0>,          \
+    (const void*)KERNEL<5, 512, 0>, (const void*)KERNEL<6, 512, 0>
+
+// Specialised launch when (n, lshift, threads) is on LIST, else the runtime
+// engine.
+#define GBK_DISPATCH(LIST, SHAPE, KERNEL, GRID, LDS, STREAM, ...)            \
+    LIST(GBK_SPEC_LAUNCH, SHAPE, KERNEL, GRID, LDS, STREAM, __VA_ARGS__)     \
+    { GBK_FAST_DISPATCH(SHAPE, KERNEL, GRID, LDS, STREAM, __VA_ARGS__) }
 
 }  // namespace gbkhip

@@ -45,6 +45,7 @@ struct FFTPlan {
     Axis ay, az;
     FastShape fx, fy, fz;         // register-tiled engine, when applicable
     DevBuf twx, twy, twz;         // twiddle tables of the three transforms
+    DevBuf twsx, twsy, twsz;      // per-stage tables (compile-time plans)
     DevBuf twpost;                // exp(-2 pi i k / nx), k = 0..nx/2
 };
 

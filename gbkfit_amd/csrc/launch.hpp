@@ -57,8 +57,10 @@ FastShape fast_shape_lines(int n, int lshift);
 FastShape fast_shape_rows(int n);
 unsigned div_magic(unsigned d);
 void upload_twiddles(DevBuf& buf, int n, int count);
+void upload_stage_twiddles(DevBuf& buf, int n);
 void launch_fx_lines(Context& c, const FastShape& f, const float2* tw,
-                     const float2* src, float2* dst, const float2* K,
+                     const float2* tws, const float2* src, float2* dst,
+                     const float2* K,
                      int nlines, int inner, long long base0,
                      long long ostride, long long stride, int st_lo,
                      int st_hi, int mode, float scale, int batch,
