@@ -7,11 +7,19 @@ A step evaluates `nsets` parameter sets of a thin smooth disk (20 ring
 nodes at 1.5 px, exponential 1/8 px, arctan rotation curve, uniform
 dispersion; per set incl U(20,80), PA U(0,360), vmax U(100,300),
 rt U(2,20), sigma U(5,50)) on a 128x128x96 cube, 2x spatial supersampling,
-PSF FWHM 3 px, LSF FWHM 2 channels (10 km/s), reference FFT padding
-(320x320x128), each against a seeded mock cube (noise sigma 0.01) -> one
-FP64 chi^2 per set back on the host.  The timed region covers packing the
-parameter block, the upload, the batched native evaluation (one HIP graph
-replay per chunk sequence) and the download of the chi^2 vector.
+PSF FWHM 3 px, LSF FWHM 2 channels (10 km/s), reference DCube padding
+(model grid 320x320x128), each against a seeded mock cube (noise sigma
+0.01) -> one FP64 chi^2 per set back on the host.  The batched path runs the
+spectral-first fused pipeline on 300x300x120 FFTs (the smallest 5-smooth
+sizes that contain every output-relevant input; identical result in the
+downscaled region).  The timed region covers packing the parameter block,
+the upload, the batched native evaluation (one HIP graph replay per chunk
+sequence) and the download of the chi^2 vector.
+
+A second timed region repeats the steps with direct launches and HIP events
+after every kernel (on the kernels' own stream): the per-kernel times give
+the roofline of the dominant kernel, algorithmic bytes per launch over its
+average launch time.
 
 The plain path is the drop-in call sequence of the reference
 (Objective.log_likelihood per set: pack + upload trait arrays, smdisk,
@@ -138,6 +146,44 @@ def cpu_baseline(seconds, thick=False):
                        f"is absent)")
 
 
+def algorithmic_bytes(ev, info):
+    """Compulsory HBM bytes per parameter set of each kernel (or stage)."""
+    nxl, nyl, nzl = ev.size_lo
+    sx, sy, _ = ev.desc.scale
+    X, Y = nxl * sx, nyl * sy
+    n_lo = nxl * nyl * nzl
+    if info['fused']:
+        nx, ny, nz = info['fft_size']
+        nzh = nz // 2 + 1
+        plane = 8 * nzh * ny * nx            # one complex half spectrum
+        return {
+            'batch_lines_thin': 16 * nx * ny,
+            'fz_deposit_r2c': 16 * nx * ny + plane,
+            'fx_lines (y)': 2 * plane,
+            'fx_rows_conv': plane + 8 * nzh * ny * X,
+            'fx_lines (inverse y)': 8 * nzh * ny * X + 8 * nzh * Y * X,
+            'fz_c2r_down_chi2': 8 * nzh * Y * X + 4 * n_lo,
+        }
+    n_pad = int(np.prod(ev.size_hi))
+    return {'smdisk': 4 * n_pad, 'fft_convolution': 12 * n_pad,
+            'downscale_chi2': 4 * X * Y * nzl + 4 * n_lo}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration),
+    or None when no summary for it is present."""
+    name = kernel.split(' ')[0]
+    path = os.path.join(ROOT, 'profiles', f'{name}_c3_traffic.txt')
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith('traffic_bytes_per_launch:'):
+                return float(line.split(':', 1)[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -150,7 +196,21 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-graph', action='store_true')
     ap.add_argument('--thick', action='store_true')
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='only copy a 1 GiB device buffer 3 times (known '
+                         'bytes for scaling FETCH_SIZE / WRITE_SIZE)')
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 1 << 28
+        x = torch.ones(n, dtype=torch.float32, device='cuda')
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps(dict(copy_bytes_read=4 * n, copy_bytes_written=4 * n,
+                              launches=3)))
+        return
     if args.gpus != 1:
         print(json.dumps(dict(error="this benchmark runs on one GPU; "
                               "multi-GPU runs are not part of this build")))
@@ -200,16 +260,20 @@ def main():
     step()
     torch.cuda.synchronize()
     stage_ms = np.zeros(3)
+    kern_ms = {}
     chunks = 0
     t1 = time.perf_counter()
     for _ in range(args.steps):
         chi2_t = step()
         ms, chunks = ev.stage_times()
         stage_ms += ms
+        for k, v in ev.kernel_times().items():
+            kern_ms[k] = kern_ms.get(k, 0.0) + v
     torch.cuda.synchronize()
     t_timed = time.perf_counter() - t1
     ev.set_timing(False)
     assert np.array_equal(chi2_t, chi2)
+    info = ev.info()
 
     # plain drop-in path, one parameter set per call
     obj = Objective([dict(scube=Data(mock, error=np.full(mock.shape, 0.01,
@@ -229,14 +293,24 @@ def main():
     t_plain = time.perf_counter() - tp
     plain_value = len(plain_sets) / t_plain
 
-    # roofline of the dominant stage (batched FFT convolution): algorithmic
-    # bytes per set = 12 * N_pad (read the cube, read the half spectrum,
-    # write the cube), per launch = sets per chunk
-    n_pad = int(np.prod(ev.size_hi))
+    # roofline of the dominant kernel: its algorithmic bytes per launch (the
+    # compulsory reads and writes of its inputs and outputs for the sets of
+    # one chunk) over its average launch time from the HIP events above
+    launches = args.steps * max(chunks, 1)
     per_chunk = args.nsets / max(chunks, 1)
-    conv_ms = stage_ms[1] / (args.steps * max(chunks, 1))
-    conv_bytes = 12.0 * n_pad * per_chunk
-    achieved = conv_bytes / (conv_ms * 1e-3) / 1e9 if conv_ms > 0 else None
+    algo = algorithmic_bytes(ev, info)
+    kernels = {}
+    for k, t in kern_ms.items():
+        avg_ms = t / launches
+        b = algo.get(k)
+        kernels[k] = dict(avg_ms_per_launch=avg_ms)
+        if b is not None and avg_ms > 0:
+            gbs = b * per_chunk / (avg_ms * 1e-3) / 1e9
+            kernels[k].update(bytes_per_set=b, achieved_gbs=gbs,
+                              frac=gbs / HBM_PEAK_GBS)
+    dom = max(kern_ms, key=kern_ms.get)
+    achieved = kernels[dom].get('achieved_gbs')
+    traffic = measured_traffic(dom)
 
     cpu = None
     if not args.no_cpu:
@@ -252,11 +326,13 @@ def main():
         scaling='weak', vs_baseline=None, dtype='fp32', data='synthetic',
         config=dict(workload=(
             f"smdisk {'thick' if args.thick else 'thin'} 128x128x96, 2x "
-            f"spatial supersampling, PSF FWHM 3 px + LSF FWHM 2 ch FFT "
-            f"convolution (padded 320x320x128), chi^2 vs seeded mock; "
-            f"{args.nsets} parameter sets per step"),
+            f"spatial supersampling, PSF FWHM 3 px + LSF FWHM 2 ch, chi^2 vs "
+            f"seeded mock; {args.nsets} parameter sets per step"),
             nsets=args.nsets, chunk=args.chunk, graph=use_graph,
-            padding='reference'),
+            padding='reference', dcube_size_hi=list(ev.size_hi),
+            pipeline='spectral-first fused' if info['fused']
+            else 'padded-cube FFT',
+            fft_size=list(info['fft_size'])),
         plain=dict(value=plain_value, unit='evals/s',
                    sets=len(plain_sets), **percentiles(plat)),
         batched_over_plain=value / plain_value,
@@ -266,11 +342,12 @@ def main():
         stages_ms_per_step=dict(smdisk=stage_ms[0] / args.steps,
                                 fft_convolution=stage_ms[1] / args.steps,
                                 downscale_chi2=stage_ms[2] / args.steps),
+        kernels=kernels,
         roofline=dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS,
                       unit='GB/s',
                       frac=(achieved / HBM_PEAK_GBS) if achieved else None,
-                      traffic=None,
-                      kernel='batched FFT convolution stage (5 kernels)'),
+                      traffic=traffic, kernel=dom,
+                      sets_per_launch=per_chunk),
         cpu_baseline=cpu,
         host=dict(cpu=platform.processor() or platform.machine(),
                   nproc=os.cpu_count()),

@@ -48,6 +48,8 @@ N.register_signatures({
     'gbkhip_batch_evaluate': ([_c.c_void_p, _c.c_void_p, _c.c_int,
                                _c.c_void_p, _c.c_void_p, _c.c_int], _c.c_int),
     'gbkhip_batch_info': ([_c.c_void_p, _c.POINTER(_c.c_int)], _c.c_int),
+    'gbkhip_batch_kernel_times': ([_c.c_void_p, _c.POINTER(_c.c_double),
+                                   _c.c_int, _c.POINTER(_c.c_int)], _c.c_int),
     'gbkhip_batch_set_timing': ([_c.c_void_p, _c.c_int], _c.c_int),
     'gbkhip_batch_stage_times': ([_c.c_void_p, _c.POINTER(_c.c_double),
                                   _c.POINTER(_c.c_int)], _c.c_int),
@@ -302,6 +304,20 @@ class BatchEvaluator:
         n = _c.c_int()
         N.call('gbkhip_batch_stage_times', self.handle, ms, _c.byref(n))
         return tuple(ms), n.value
+
+    FUSED_KERNELS = ('batch_lines_thin', 'fz_deposit_r2c', 'fx_lines (y)',
+                     'fx_rows_conv', 'fx_lines (inverse y)',
+                     'fz_c2r_down_chi2', 'batch_chi2_final')
+    PADDED_STAGES = ('smdisk', 'fft_convolution', 'downscale_chi2')
+
+    def kernel_times(self):
+        """{kernel: milliseconds} of the last timed evaluation, summed over
+        its chunks (stages on the padded-cube path)."""
+        ms = (_c.c_double * 8)()
+        n = _c.c_int()
+        N.call('gbkhip_batch_kernel_times', self.handle, ms, 8, _c.byref(n))
+        names = self.FUSED_KERNELS if n.value == 7 else self.PADDED_STAGES
+        return {k: ms[i] for i, k in enumerate(names[:n.value])}
 
     def close(self):
         if getattr(self, 'handle', None) and not N.shutting_down():

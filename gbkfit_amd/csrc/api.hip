@@ -332,6 +332,9 @@ int gbkhip_mcdisk_evaluate(gbkhip_ctx* ctx, float cflux, int nclouds,
         check_ptr(hasordint, "hasordint");
         GBK_CHECK(ncloudscsum_len > 0, GBKHIP_ERR_ARG,
                   "empty cloud cumulative sum");
+        // cloud i of trait t samples height trait t (gmodels.hpp:406-416)
+        GBK_CHECK(disk->rp.n > 0 && disk->rh.n >= disk->rp.n, GBKHIP_ERR_ARG,
+                  "mcdisk needs one density height trait per polar trait");
         std::lock_guard<std::mutex> lk(c->mu);
         launch_mcdisk(*c, cflux, nclouds, ncloudscsum, ncloudscsum_len,
                       hasordint, normalise(*disk), *grid, *out, seed);

@@ -258,7 +258,8 @@ __global__ __launch_bounds__(T) void fz_deposit_r2c(
     const float4* __restrict__ lines, long long lstride, gbkhip_grid g,
     FAxis a, const float2* __restrict__ tw,
     const float2* __restrict__ tws, const float2* __restrict__ tpz,
-    int need_z, int nx, int ny, float2* __restrict__ S, long long sstride)
+    int need_z, int nx, int ny, int px, int shift, float2* __restrict__ S,
+    long long sstride)
 {
     extern __shared__ float2 lds[];
     constexpr int L = 1 << LSH, LS = L + 1;
@@ -276,8 +277,9 @@ __global__ __launch_bounds__(T) void fz_deposit_r2c(
     const int l = threadIdx.x & (L - 1);
     const int part = threadIdx.x >> LSH;
     constexpr int nparts = T >> LSH;
-    const int x = x0 + l;
-    if (x < nx) {
+    const int c = x0 + l;                // FFT column
+    if (c < nx) {
+        const int x = c >= shift ? c - shift : c - shift + nx;   // spaxel
         const float4 V = lines[set * lstride + (size_t)y * nx + x];
         if (V.x != 0.0f) {
             int k0, k1;
@@ -293,8 +295,8 @@ __global__ __launch_bounds__(T) void fz_deposit_r2c(
     // z[m] = x[2m] + i x[2m+1]; complex FFT of length n = nz/2, then
     // X[k] = (Z[k] + conj Z[n-k]) / 2 + W^k (Z[k] - conj Z[n-k]) / (2i)
     tile_fft<LSH, T, N>(buf, a, s_tw, -1.0f);
-    float2* out = S + set * sstride + (size_t)y * nx + x0;
-    const size_t plane = (size_t)nx * ny;
+    float2* out = S + set * sstride + (size_t)y * px + x0;
+    const size_t plane = (size_t)px * ny;
     const int cnt = (n + 1) << LSH;
     for (int i = threadIdx.x; i < cnt; i += T) {
         const int ll = i & (L - 1), k = i >> LSH;
@@ -318,7 +320,7 @@ __global__ __launch_bounds__(T) void fx_rows_conv(
     unsigned ny_m, FAxis a, unsigned n_m, const float2* __restrict__ tw,
     const float2* __restrict__ tws,
     const float2* __restrict__ phat, const float2* __restrict__ lhat,
-    int x_lo, int x_cnt, unsigned xcnt_m)
+    int x_lo, int x_cnt, unsigned xcnt_m, int px)
 {
     extern __shared__ float2 lds[];
     constexpr int L = 1 << LSH, LS = L + 1;
@@ -327,7 +329,7 @@ __global__ __launch_bounds__(T) void fx_rows_conv(
     float2* buf = lds + tw_entries<N>(n);
     const int r0 = blockIdx.x << LSH;
     const int nl = min(L, nrows - r0);
-    float2* rows = S + blockIdx.y * sstride + (size_t)r0 * n;
+    float2* rows = S + blockIdx.y * sstride + (size_t)r0 * px;
     for (int i = threadIdx.x; i < tw_entries<N>(n); i += T)
         s_tw[i] = N ? tws[i] : tw[i];
     const int tile = n << LSH, valid = n * nl;
@@ -335,7 +337,8 @@ __global__ __launch_bounds__(T) void fx_rows_conv(
 #pragma unroll
     for (int c = 0; c < kFastE; ++c) {
         const int i = threadIdx.x + c * T;
-        r[c] = i < valid ? rows[i] : make_float2(0, 0);
+        const int l = (int)fdiv((unsigned)i, n_m), m = i - l * n;
+        r[c] = i < valid ? rows[(size_t)l * px + m] : make_float2(0, 0);
     }
 #pragma unroll
     for (int c = 0; c < kFastE; ++c) {
@@ -371,7 +374,7 @@ __global__ __launch_bounds__(T) void fx_rows_conv(
     const int cnt = nl * x_cnt;
     for (int i = threadIdx.x; i < cnt; i += T) {
         const int l = (int)fdiv((unsigned)i, xcnt_m), m = x_lo + i - l * x_cnt;
-        rows[(size_t)l * n + m] = buf[m * LS + l];
+        rows[(size_t)l * px + m] = buf[m * LS + l];
     }
 }
 
@@ -381,7 +384,7 @@ __global__ __launch_bounds__(T) void fx_rows_conv(
 // (lx, ly) on line l = ly * tx + lx.
 template <int LSH, int T, int N = 0>
 __global__ __launch_bounds__(T) void fz_c2r_down_chi2(
-    const float2* __restrict__ S, long long sstride, int nx, int ny, FAxis a,
+    const float2* __restrict__ S, long long sstride, int px, int ny, FAxis a,
     const float2* __restrict__ tw,
     const float2* __restrict__ tws, const float2* __restrict__ tpz, int x_lo,
     int y_lo, int kx, int ky, int sx, int sy, int sz, int ez, int nxl,
@@ -400,12 +403,14 @@ __global__ __launch_bounds__(T) void fz_c2r_down_chi2(
     const int set = blockIdx.y;
     const int X0 = (blockIdx.x % tiles_x) * kx, Y0 = (blockIdx.x / tiles_x) * ky;
     const float2* src = S + set * sstride;
-    const size_t plane = (size_t)nx * ny;
+    const size_t plane = (size_t)px * ny;
     for (int i = threadIdx.x; i < tw_entries<N>(n); i += T)
         s_tw[i] = N ? tws[i] : tw[i];
     for (int i = threadIdx.x; i <= n; i += T) s_post[i] = tpz[i];
     __syncthreads();
-    // Z[k] = (X[k] + conj X[n-k]) + i W^-k (X[k] - conj X[n-k]), k < n
+    // pairs (k, n - k), k <= n/2, each plane read once:
+    //   Z[k] = (X[k] + conj X[n-k]) + i W^-k (X[k] - conj X[n-k]), k < n
+    const int h = n >> 1;
     float2 xa[kFastE], xb[kFastE];
 #pragma unroll
     for (int c = 0; c < kFastE; ++c) {
@@ -413,8 +418,8 @@ __global__ __launch_bounds__(T) void fz_c2r_down_chi2(
         const int l = i & (L - 1), k = i >> LSH;
         const int lx = l % tx, ly = l / tx;
         xa[c] = xb[c] = make_float2(0, 0);
-        if (k < n && ly < ty && X0 + lx / sx < nxl && Y0 + ly / sy < nyl) {
-            const size_t col = (size_t)(y_lo + Y0 * sy + ly) * nx +
+        if (k <= h && ly < ty && X0 + lx / sx < nxl && Y0 + ly / sy < nyl) {
+            const size_t col = (size_t)(y_lo + Y0 * sy + ly) * px +
                                (x_lo + X0 * sx + lx);
             xa[c] = src[k * plane + col];
             xb[c] = src[(n - k) * plane + col];
@@ -424,12 +429,16 @@ __global__ __launch_bounds__(T) void fz_c2r_down_chi2(
     for (int c = 0; c < kFastE; ++c) {
         const int i = threadIdx.x + c * T;
         const int l = i & (L - 1), k = i >> LSH;
-        if (k < n) {
-            const float2 xk = xa[c];
-            const float2 xm = conjf2(xb[c]);
-            const float2 fe = cadd(xk, xm);
-            const float2 fo = cmul(csub(xk, xm), conjf2(s_post[k]));
-            buf[k * LS + l] = cadd(fe, make_float2(-fo.y, fo.x));
+        if (k <= h) {
+            const float2 A = xa[c], B = xb[c];
+            const float2 ca = conjf2(A), cb = conjf2(B);
+            const float2 fo = cmul(csub(A, cb), conjf2(s_post[k]));
+            buf[k * LS + l] = cadd(cadd(A, cb), make_float2(-fo.y, fo.x));
+            if (k > 0 && n - k != k) {
+                const float2 go = cmul(csub(B, ca), conjf2(s_post[n - k]));
+                buf[(n - k) * LS + l] =
+                    cadd(cadd(B, ca), make_float2(-go.y, go.x));
+            }
         }
     }
     __syncthreads();
@@ -504,6 +513,11 @@ struct Fused {
     int nx = 0, ny = 0, nz = 0, nzh = 0;   // FFT sizes (nz even)
     int need[3] = {0, 0, 0};               // S_i + 2 edge_i
     int xlo = 0, xcnt = 0, ylo = 0, ycnt = 0;
+    // S rows are px >= nx complex apart (a multiple of 16: 128-byte aligned
+    // column tiles) and column c holds model spaxel (c - shift) mod nx, the
+    // shift putting the needed x range at a multiple of 16 (the convolution
+    // is circular, so a cyclic shift of its input shifts its output)
+    int px = 0, shift = 0;
     FastShape fz, fy, fx;                  // z columns, y lines, x rows
     DevBuf twz, twy, twx, tpz;
     DevBuf twsz, twsy, twsx;               // per-stage tables
@@ -605,9 +619,15 @@ static void build_layout(Batch& b)
     b.thin = D.traits[1].n == 0;
 }
 
+// Timing marks per chunk: the padded path records 4 (start, after smdisk,
+// after convolution, after chi^2), the fused one 8 (one after each kernel).
+constexpr int kMarks = 8;
+
+static int n_marks(const Batch& b) { return b.fused.on ? 8 : 4; }
+
 static void ensure_events(Batch& b, int chunks)
 {
-    while ((int)b.events.size() < 4 * chunks) {
+    while ((int)b.events.size() < kMarks * chunks) {
         hipEvent_t e;
         GBK_HIP(hipEventCreate(&e));
         b.events.push_back(e);
@@ -617,7 +637,7 @@ static void ensure_events(Batch& b, int chunks)
 static void mark(Batch& b, int chunk, int i)
 {
     if (b.timing)
-        GBK_HIP(hipEventRecord(b.events[4 * chunk + i], b.ctx->stream));
+        GBK_HIP(hipEventRecord(b.events[kMarks * chunk + i], b.ctx->stream));
 }
 
 // Smallest 2^a 3^b 5^c >= n (even when asked).
@@ -705,8 +725,11 @@ static void plan_fused(Context& c, Batch& B)
     F.fx = fx;
     F.fy = fy;
     F.fz = fz;
-    F.xlo = ex;
     F.xcnt = D.size_lo[0] * D.scale[0];
+    F.px = (nx + 15) / 16 * 16;
+    F.shift = (16 - ex % 16) % 16;
+    if (ex + F.shift + F.xcnt > nx) F.shift = 0;
+    F.xlo = ex + F.shift;
     F.ylo = ey;
     F.ycnt = D.size_lo[1] * D.scale[1];
     F.kx = std::max(1, std::min(L / sxy, D.size_lo[0]));
@@ -782,7 +805,7 @@ static void plan_fused(Context& c, Batch& B)
     upload_stage_twiddles(F.twsz, nz / 2);
     upload_stage_twiddles(F.twsy, ny);
     upload_stage_twiddles(F.twsx, nx);
-    F.sstride = (long long)nzh * ny * nx;
+    F.sstride = (long long)nzh * ny * F.px;
     F.S.alloc((size_t)D.max_batch * F.sstride * sizeof(float2));
     F.lstride = (long long)ny * nx;
     F.lines.alloc((size_t)D.max_batch * F.lstride * sizeof(float4));
@@ -808,7 +831,7 @@ static void run_chunk_fused(Batch& b, const float* params, int set0,
     const Fused& F = b.fused;
     const float* P = params + (size_t)set0 * D.pstride;
     float2* S = F.S.as<float2>();
-    const long long plane = (long long)F.ny * F.nx;
+    const long long plane = (long long)F.ny * F.px;
     {
         float4* lines = F.lines.as<float4>();
         dim3 g0(ceil_div(F.nx, 64), ceil_div(F.ny, 4), nsets);
@@ -816,19 +839,21 @@ static void run_chunk_fused(Batch& b, const float* params, int set0,
                            b.tmpl, b.lay, P, D.pstride, D.grid, F.need[0],
                            F.need[1], F.nx, F.ny, lines, F.lstride);
         GBK_HIP(hipGetLastError());
+        mark(b, ci, 1);
         dim3 grid(ceil_div(F.nx, 1 << F.fz.lshift), F.ny, nsets);
         GBK_DISPATCH(GBK_SPEC_Z, F.fz, fz_deposit_r2c, grid,
                      F.fz.lds + (F.fz.a.n + 1) * sizeof(float2), c.stream,
                      lines, F.lstride, D.grid, F.fz.a, F.twz.as<float2>(),
                      F.twsz.as<float2>(), F.tpz.as<float2>(), F.need[2],
-                     F.nx, F.ny, S, F.sstride);
+                     F.nx, F.ny, F.px, F.shift, S, F.sstride);
         GBK_HIP(hipGetLastError());
     }
-    mark(b, ci, 1);
+    mark(b, ci, 2);
     launch_fx_lines(c, F.fy, F.twy.as<float2>(), F.twsy.as<float2>(), S, S,
                     nullptr,
-                    F.nzh * F.nx, F.nx, 0, plane, F.nx, 0, F.ny, 0, 1.0f,
+                    F.nzh * F.nx, F.nx, 0, plane, F.px, 0, F.ny, 0, 1.0f,
                     nsets, F.sstride, 0);
+    mark(b, ci, 3);
     {
         const int nrows = F.nzh * F.ny;
         dim3 grid(ceil_div(nrows, 1 << F.fx.lshift), nsets);
@@ -838,14 +863,15 @@ static void run_chunk_fused(Batch& b, const float* params, int set0,
                      div_magic((unsigned)F.nx), F.twx.as<float2>(),
                      F.twsx.as<float2>(), F.phat.as<float2>(),
                           F.lhat.as<float2>(), F.xlo, F.xcnt,
-                          div_magic((unsigned)F.xcnt));
+                          div_magic((unsigned)F.xcnt), F.px);
         GBK_HIP(hipGetLastError());
     }
+    mark(b, ci, 4);
     launch_fx_lines(c, F.fy, F.twy.as<float2>(), F.twsy.as<float2>(), S, S,
                     nullptr,
-                    F.nzh * F.xcnt, F.xcnt, F.xlo, plane, F.nx, F.ylo,
+                    F.nzh * F.xcnt, F.xcnt, F.xlo, plane, F.px, F.ylo,
                     F.ylo + F.ycnt, 1, 1.0f, nsets, F.sstride, 0);
-    mark(b, ci, 2);
+    mark(b, ci, 5);
     DataArgs da;
     da.obs = D.obs;
     da.err = D.err;
@@ -856,7 +882,7 @@ static void run_chunk_fused(Batch& b, const float* params, int set0,
         dim3 grid(F.tiles_x * F.tiles_y, nsets);
         GBK_DISPATCH(GBK_SPEC_Z, F.fz, fz_c2r_down_chi2, grid,
                      F.fz.lds + (F.fz.a.n + 1) * sizeof(float2), c.stream, S,
-                     F.sstride, F.nx, F.ny, F.fz.a, F.twz.as<float2>(),
+                     F.sstride, F.px, F.ny, F.fz.a, F.twz.as<float2>(),
                      F.twsz.as<float2>(), F.tpz.as<float2>(), F.xlo,
                           F.ylo, F.kx, F.ky, D.scale[0], D.scale[1],
                           D.scale[2], D.edge_hi[2], D.size_lo[0],
@@ -867,10 +893,11 @@ static void run_chunk_fused(Batch& b, const float* params, int set0,
                           b.n_lo, b.partials.as<double>(), b.nparts);
         GBK_HIP(hipGetLastError());
     }
+    mark(b, ci, 6);
     hipLaunchKernelGGL(batch_chi2_final, dim3(nsets), dim3(256), 0, c.stream,
                        b.partials.as<double>(), b.nparts, chi2, set0);
     GBK_HIP(hipGetLastError());
-    mark(b, ci, 3);
+    mark(b, ci, 7);
 }
 
 static void run_chunk(Batch& b, const float* params, int set0, int nsets,
@@ -1110,13 +1137,40 @@ int gbkhip_batch_stage_times(gbkhip_batch* b, double* ms, int* chunks)
         Batch& B = b->impl;
         ms[0] = ms[1] = ms[2] = 0.0;
         *chunks = B.timed_chunks;
+        // stage boundaries among the marks
+        static const int kFusedBnd[4] = {0, 2, 5, 7};
+        static const int kPaddedBnd[4] = {0, 1, 2, 3};
+        const int* bnd = B.fused.on ? kFusedBnd : kPaddedBnd;
         for (int c = 0; c < B.timed_chunks; ++c) {
-            GBK_HIP(hipEventSynchronize(B.events[4 * c + 3]));
+            GBK_HIP(hipEventSynchronize(
+                B.events[kMarks * c + n_marks(B) - 1]));
             for (int s = 0; s < 3; ++s) {
                 float t = 0.0f;
-                GBK_HIP(hipEventElapsedTime(&t, B.events[4 * c + s],
-                                            B.events[4 * c + s + 1]));
+                GBK_HIP(hipEventElapsedTime(&t,
+                                            B.events[kMarks * c + bnd[s]],
+                                            B.events[kMarks * c + bnd[s + 1]]));
                 ms[s] += t;
+            }
+        }
+    });
+}
+
+int gbkhip_batch_kernel_times(gbkhip_batch* b, double* ms, int cap, int* n)
+{
+    return guarded([&] {
+        GBK_CHECK(b && ms && n, GBKHIP_ERR_ARG, "null argument");
+        Batch& B = b->impl;
+        const int k = n_marks(B) - 1;
+        GBK_CHECK(cap >= k, GBKHIP_ERR_ARG, "kernel_times: array too short");
+        *n = k;
+        for (int i = 0; i < k; ++i) ms[i] = 0.0;
+        for (int c = 0; c < B.timed_chunks; ++c) {
+            GBK_HIP(hipEventSynchronize(B.events[kMarks * c + k]));
+            for (int i = 0; i < k; ++i) {
+                float t = 0.0f;
+                GBK_HIP(hipEventElapsedTime(&t, B.events[kMarks * c + i],
+                                            B.events[kMarks * c + i + 1]));
+                ms[i] += t;
             }
         }
     });

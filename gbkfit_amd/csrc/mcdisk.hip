@@ -230,6 +230,11 @@ __global__ __launch_bounds__(256) void mcdisk_kernel(
         rot_lh_z(xr, yr, xn, yn, -sp, cp);
         xr += xposi;
         yr += yposi;
+        // A NaN position (the exponential height sampler of math.hpp:252-255
+        // returns NaN for |u| > 1/2) lands nowhere: on the host driver
+        // rint(NaN) converts to INT_MIN, outside the cube, while the GPU's
+        // conversion would give 0.
+        if (isnan(xr) || isnan(yr) || isnan(zn)) continue;
         const int x = (int)rintf((xr - g.spat_zero[0]) / g.spat_step[0]);
         const int y = (int)rintf((yr - g.spat_zero[1]) / g.spat_step[1]);
         const int z = (int)rintf((zn - g.spat_zero[2]) / g.spat_step[2]);

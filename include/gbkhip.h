@@ -247,6 +247,13 @@ int gbkhip_batch_evaluate(gbkhip_batch* batch, const float* params,
 int gbkhip_batch_info(gbkhip_batch* batch, int* info);
 int gbkhip_batch_set_timing(gbkhip_batch* batch, int enable);
 int gbkhip_batch_stage_times(gbkhip_batch* batch, double* ms, int* chunks);
+/* Per-kernel times of the last timed evaluate(), summed over its chunks:
+ * *n = 3 on the padded-cube path (smdisk, FFT convolution, downscale +
+ * chi^2) and 7 on the fused path (lines, deposit + z FFT, y FFT, x FFT
+ * convolution, inverse y FFT, inverse z FFT + downscale + chi^2, chi^2
+ * final).  cap = length of ms. */
+int gbkhip_batch_kernel_times(gbkhip_batch* batch, double* ms, int cap,
+                              int* n);
 
 #ifdef __cplusplus
 }

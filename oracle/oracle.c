@@ -828,6 +828,10 @@ static void mcdisk_cloud(double cflux, int ci, const int* csum,
     xn += xposi;
     yn += yposi;
 
+    /* NaN positions (exponential height sampler, math.hpp:252-255, for
+     * |u| > 1/2) land outside the cube on the host driver: rint(NaN) -> int
+     * gives INT_MIN on x86.  Made explicit here (the C conversion is UB). */
+    if (isnan(xn) || isnan(yn) || isnan(zn)) return;
     const double ax = (xn - g->spat_zero[0]) / g->spat_step[0];
     const double ay = (yn - g->spat_zero[1]) / g->spat_step[1];
     const double az = (zn - g->spat_zero[2]) / g->spat_step[2];
