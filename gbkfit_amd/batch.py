@@ -51,6 +51,11 @@ N.register_signatures({
     'gbkhip_batch_kernel_times': ([_c.c_void_p, _c.POINTER(_c.c_double),
                                    _c.c_int, _c.POINTER(_c.c_int)], _c.c_int),
     'gbkhip_batch_set_timing': ([_c.c_void_p, _c.c_int], _c.c_int),
+    'gbkhip_batch_set_stream': ([_c.c_void_p, _c.c_void_p], _c.c_int),
+    'gbkhip_batch_release': ([_c.c_void_p], _c.c_int),
+    'gbkhip_batch_memory': ([_c.c_void_p, _c.POINTER(_c.c_ulonglong),
+                             _c.POINTER(_c.c_ulonglong),
+                             _c.POINTER(_c.c_int)], _c.c_int),
     'gbkhip_batch_stage_times': ([_c.c_void_p, _c.POINTER(_c.c_double),
                                   _c.POINTER(_c.c_int)], _c.c_int),
 })
@@ -271,6 +276,9 @@ class BatchEvaluator:
             self._cubes_dev = t.empty((nsets,) + self.size_lo[::-1],
                                       dtype=t.float32,
                                       device=self.driver.device)
+        # evaluate on torch's current stream, where the uploads and the
+        # downloads of evaluate() are ordered
+        self.set_stream(t.cuda.current_stream())
         N.call('gbkhip_batch_evaluate', self.handle,
                self._params_dev.data_ptr(), int(nsets),
                self._chi2_dev.data_ptr(),
@@ -304,6 +312,24 @@ class BatchEvaluator:
         n = _c.c_int()
         N.call('gbkhip_batch_stage_times', self.handle, ms, _c.byref(n))
         return tuple(ms), n.value
+
+    def set_stream(self, stream):
+        """Stream of later evaluations (a torch.cuda.Stream, a raw handle or
+        None for the context's stream)."""
+        h = getattr(stream, 'cuda_stream', stream)
+        N.call('gbkhip_batch_set_stream', self.handle,
+               _c.c_void_p(int(h) if h else 0))
+
+    def release(self):
+        """Frees the work buffers; the next evaluation allocates them."""
+        N.call('gbkhip_batch_release', self.handle)
+
+    def memory(self):
+        """dict(resident=bytes, work=bytes, allocated=bool)."""
+        r, w, a = _c.c_ulonglong(), _c.c_ulonglong(), _c.c_int()
+        N.call('gbkhip_batch_memory', self.handle, _c.byref(r), _c.byref(w),
+               _c.byref(a))
+        return dict(resident=r.value, work=w.value, allocated=bool(a.value))
 
     FUSED_KERNELS = ('batch_lines_thin', 'fz_deposit_r2c', 'fx_lines (y)',
                      'fx_rows_conv', 'fx_lines (inverse y)',

@@ -547,6 +547,8 @@ struct Batch {
     int nparts = 0;
     long long n_hi = 0, n_c = 0, n_lo = 0;
     bool thin = true;
+    bool work = false;      // work buffers allocated
+    hipStream_t stream = nullptr;   // evaluate() stream; null: the context's
     // graph cache: (params, nsets, chi2, cubes) -> executable graph
     std::map<std::tuple<uintptr_t, int, uintptr_t, uintptr_t>,
              hipGraphExec_t> graphs;
@@ -617,6 +619,56 @@ static void build_layout(Batch& b)
     b.tmpl.rnodes = b.rnodes.as<float>();
     b.tmpl.opacity = nullptr;
     b.thin = D.traits[1].n == 0;
+}
+
+// Work buffers: everything sized by max_batch (per-set spectra or padded
+// cubes, line parameters, chi^2 partials).  They can be released between
+// evaluations and are re-allocated by the next one; the resident state
+// (layout, kernel spectra, plans, twiddles) stays.
+static size_t work_bytes(const Batch& B)
+{
+    const size_t mb = (size_t)B.desc.max_batch;
+    size_t n = mb * (size_t)B.nparts * sizeof(double);
+    if (B.fused.on) {
+        n += mb * (size_t)B.fused.sstride * sizeof(float2);
+        n += mb * (size_t)B.fused.lstride * sizeof(float4);
+    } else {
+        n += mb * (size_t)B.n_hi * sizeof(float);
+        if (B.desc.has_conv) n += mb * (size_t)B.n_c * sizeof(float2);
+    }
+    return n;
+}
+
+static void drop_graphs(Batch& B)
+{
+    for (auto& kv : B.graphs) (void)hipGraphExecDestroy(kv.second);
+    B.graphs.clear();
+}
+
+static void alloc_work(Batch& B)
+{
+    if (B.work) return;
+    const size_t mb = (size_t)B.desc.max_batch;
+    B.partials.alloc(mb * B.nparts * sizeof(double));
+    if (B.fused.on) {
+        B.fused.S.alloc(mb * B.fused.sstride * sizeof(float2));
+        B.fused.lines.alloc(mb * B.fused.lstride * sizeof(float4));
+    } else {
+        B.hi.alloc(mb * B.n_hi * sizeof(float));
+        if (B.desc.has_conv) B.spec.alloc(mb * B.n_c * sizeof(float2));
+    }
+    B.work = true;
+}
+
+static void release_work(Batch& B)
+{
+    drop_graphs(B);          // graphs hold the buffer addresses
+    B.partials.release();
+    B.fused.S.release();
+    B.fused.lines.release();
+    B.hi.release();
+    B.spec.release();
+    B.work = false;
 }
 
 // Timing marks per chunk: the padded path records 4 (start, after smdisk,
@@ -806,9 +858,7 @@ static void plan_fused(Context& c, Batch& B)
     upload_stage_twiddles(F.twsy, ny);
     upload_stage_twiddles(F.twsx, nx);
     F.sstride = (long long)nzh * ny * F.px;
-    F.S.alloc((size_t)D.max_batch * F.sstride * sizeof(float2));
     F.lstride = (long long)ny * nx;
-    F.lines.alloc((size_t)D.max_batch * F.lstride * sizeof(float4));
     static std::once_flag once;
     std::call_once(once, [] {
         const void* ks[] = {GBK_FAST_KERNELS(fz_deposit_r2c),
@@ -1014,14 +1064,11 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
             plan_fused(*c, B);
             if (B.fused.on) {
                 B.nparts = B.fused.tiles_x * B.fused.tiles_y;
-                B.partials.alloc((size_t)D.max_batch * B.nparts *
-                                 sizeof(double));
+                alloc_work(B);
                 GBK_HIP(hipStreamSynchronize(c->stream));
                 *out = b;
                 return;
             }
-            B.hi.alloc((size_t)D.max_batch * B.n_hi * sizeof(float));
-            B.partials.alloc((size_t)D.max_batch * B.nparts * sizeof(double));
             if (D.has_conv) {
                 GBK_CHECK(D.pcube != nullptr, GBKHIP_ERR_ARG,
                           "pcube required with convolution");
@@ -1031,8 +1078,8 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
                 B.spectrum.alloc(B.n_c * sizeof(float2));
                 fft_forward(*c, *B.plan, D.pcube, B.spectrum.as<float2>(), 1,
                             0, 0);
-                B.spec.alloc((size_t)D.max_batch * B.n_c * sizeof(float2));
             }
+            alloc_work(B);
             fft_prepare_attributes();
             if (!B.thin) {
                 (void)hipFuncSetAttribute(
@@ -1071,6 +1118,20 @@ int gbkhip_batch_evaluate(gbkhip_batch* b, const float* params, int nsets,
         GBK_CHECK(nsets >= 0, GBKHIP_ERR_ARG, "negative set count");
         if (nsets == 0) return;
         GBK_CHECK(params && chi2, GBKHIP_ERR_ARG, "null params or chi2");
+        // the launches go to the context's stream member: point it at this
+        // batch's stream for the duration of the call (under the context
+        // lock, so batches of one context can be driven from several
+        // threads)
+        std::lock_guard<std::mutex> lk(c.mu);
+        const hipStream_t ctx_stream = c.stream;
+        const hipStream_t target = B.stream ? B.stream : ctx_stream;
+        struct Restore {
+            Context& c;
+            hipStream_t s;
+            ~Restore() { c.stream = s; }
+        } restore{c, ctx_stream};
+        alloc_work(B);
+        c.stream = target;
         if (!use_graph || B.timing) {   // stage events need direct launches
             run_all(B, params, nsets, chi2, cubes);
             return;
@@ -1082,9 +1143,8 @@ int gbkhip_batch_evaluate(gbkhip_batch* b, const float* params, int nsets,
             if (!B.cap_stream)
                 GBK_HIP(hipStreamCreateWithFlags(&B.cap_stream,
                                                  hipStreamNonBlocking));
-            // capture on a private stream, ordered after the context stream
-            GBK_HIP(hipStreamSynchronize(c.stream));
-            hipStream_t saved = c.stream;
+            // capture on a private stream, ordered after the target stream
+            GBK_HIP(hipStreamSynchronize(target));
             c.stream = B.cap_stream;
             hipGraph_t graph = nullptr;
             GBK_HIP(hipStreamBeginCapture(B.cap_stream,
@@ -1094,17 +1154,53 @@ int gbkhip_batch_evaluate(gbkhip_batch* b, const float* params, int nsets,
             } catch (...) {
                 (void)hipStreamEndCapture(B.cap_stream, &graph);
                 if (graph) (void)hipGraphDestroy(graph);
-                c.stream = saved;
                 throw;
             }
-            c.stream = saved;
             GBK_HIP(hipStreamEndCapture(B.cap_stream, &graph));
+            c.stream = target;
             hipGraphExec_t exec = nullptr;
             GBK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
             GBK_HIP(hipGraphDestroy(graph));
             it = B.graphs.emplace(key, exec).first;
         }
-        GBK_HIP(hipGraphLaunch(it->second, c.stream));
+        GBK_HIP(hipGraphLaunch(it->second, target));
+    });
+}
+
+int gbkhip_batch_set_stream(gbkhip_batch* b, void* stream)
+{
+    return guarded([&] {
+        GBK_CHECK(b != nullptr, GBKHIP_ERR_ARG, "null batch");
+        b->impl.stream = static_cast<hipStream_t>(stream);
+    });
+}
+
+int gbkhip_batch_release(gbkhip_batch* b)
+{
+    return guarded([&] {
+        GBK_CHECK(b != nullptr, GBKHIP_ERR_ARG, "null batch");
+        Batch& B = b->impl;
+        GBK_HIP(hipSetDevice(B.device));
+        // hipFree waits for work still using the buffers
+        release_work(B);
+    });
+}
+
+int gbkhip_batch_memory(gbkhip_batch* b, unsigned long long* resident,
+                        unsigned long long* work, int* allocated)
+{
+    return guarded([&] {
+        GBK_CHECK(b && resident && work && allocated, GBKHIP_ERR_ARG,
+                  "null argument");
+        const Batch& B = b->impl;
+        const Fused& F = B.fused;
+        size_t r = B.ints.bytes + B.rnodes.bytes + B.spectrum.bytes +
+                   F.phat.bytes + F.lhat.bytes + F.twz.bytes + F.twy.bytes +
+                   F.twx.bytes + F.tpz.bytes + F.twsz.bytes + F.twsy.bytes +
+                   F.twsx.bytes;
+        *resident = r;
+        *work = work_bytes(B);
+        *allocated = B.work ? 1 : 0;
     });
 }
 

@@ -91,6 +91,7 @@ class DriverHip(Driver):
             raise RuntimeError("the hip driver needs a GPU (torch.cuda)")
         self._device = t.device('cuda', device)
         self._ctx = None
+        self._ctx_stream = None
         self._backends = None
 
     @property
@@ -100,11 +101,15 @@ class DriverHip(Driver):
     def context(self):
         """The libgbkhip context of this driver, bound to torch's current
         stream so library kernels and tensor fills stay ordered."""
+        t = torch()
         if self._ctx is None:
-            t = torch()
             with t.cuda.device(self._device):
                 self._ctx = N.Context(self._device.index or 0)
-                self._ctx.set_stream(t.cuda.current_stream().cuda_stream)
+                self._ctx_stream = None
+        s = t.cuda.current_stream(self._device).cuda_stream
+        if s != self._ctx_stream:          # rebind when the caller switched
+            self._ctx.set_stream(s)
+            self._ctx_stream = s
         return self._ctx
 
     # ---- memory ----------------------------------------------------------

@@ -242,6 +242,16 @@ int gbkhip_batch_evaluate(gbkhip_batch* batch, const float* params,
  * of every chunk; stage_times() waits for them and returns the summed stage
  * durations (ms[3], milliseconds) of the last evaluate() and its chunk
  * count. */
+/* Stream of later evaluate() calls (NULL: the context's stream). */
+int gbkhip_batch_set_stream(gbkhip_batch* batch, void* stream);
+/* Frees the work buffers (everything sized by max_batch) and the cached
+ * graphs; the next evaluate() allocates them again.  Resident state (layout,
+ * kernel spectra, plans) stays. */
+int gbkhip_batch_release(gbkhip_batch* batch);
+/* Bytes of resident state and of work buffers (needed whether allocated or
+ * not), and whether the work buffers are allocated now. */
+int gbkhip_batch_memory(gbkhip_batch* batch, unsigned long long* resident,
+                        unsigned long long* work, int* allocated);
 /* info[0] = 1 when the spectral-first fused pipeline runs, info[1..3] = the
  * FFT sizes (x, y, z) in use, info[4] = chi^2 partial sums per set. */
 int gbkhip_batch_info(gbkhip_batch* batch, int* info);
