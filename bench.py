@@ -25,6 +25,13 @@ The plain path is the drop-in call sequence of the reference
 (Objective.log_likelihood per set: pack + upload trait arrays, smdisk,
 FFT convolution, downscale, residual, residual sum, scalar download).
 
+A serving leg (skip with --no-serve) replays a Poisson stream of
+single-set requests (default 1000/s for 8 s) over 16 resident galaxies with
+cube sizes log-uniform from 64x64x32 to 256x256x128; each request returns
+chi^2 and moment maps 0/1/2 through gbkfit_amd.serve.Server (continuous
+batching over 4 streams, HBM-budget admission).  Its throughput and p50/p99
+latencies go into the `serving` object.
+
 Prints one JSON line.
 """
 import argparse
@@ -85,6 +92,102 @@ def build_model(driver, thick=False):
                      psf=psflsf.PSFGauss(3.0 * SIG),
                      lsf=psflsf.LSFGauss(20.0 * SIG))
     return Model(driver, dm, gm)
+
+
+def build_galaxy_model(driver, size, scale=(1, 1, 1), rmax=None):
+    """Thin smdisk + PSF (FWHM 2.5 px) + LSF (FWHM 2 ch) model of a
+    `size` cube, ring nodes out to ~40 % of the field."""
+    from gbkfit_amd import gmodels, psflsf
+    from gbkfit_amd.dmodels import DModelSCube
+    from gbkfit_amd.objective import Model
+    from gbkfit_amd.traits import make
+    rmax = rmax or 0.4 * min(size[0], size[1])
+    rn = np.linspace(0, rmax, 16)
+    cmp = gmodels.SpectralSMDisk2D(
+        False, False, make('bpt', 'exponential'), make('vpt', 'tan_arctan'),
+        make('dpt', 'uniform'), rnodes=rn)
+    gm = gmodels.GModelKinematics2D([cmp])
+    dm = DModelSCube(tuple(size), (1, 1, 10), scale=scale,
+                     psf=psflsf.PSFGauss(2.5 * SIG),
+                     lsf=psflsf.LSFGauss(20.0 * SIG))
+    return Model(driver, dm, gm)
+
+
+def serve_benchmark(drv, ngal, seconds, rate, slots=4, max_batch=16,
+                    seed=77):
+    """Poisson stream of single-set requests (chi^2 + moment maps 0/1/2)
+    over `ngal` resident galaxies with cube sizes log-uniform from
+    64x64x32 to 256x256x128."""
+    import threading
+    import torch
+    from gbkfit_amd.serve import GpuGalaxy, Server, make_streams
+    rng = np.random.default_rng(seed)
+    streams = make_streams(slots)
+    gals, sizes = [], []
+    t0 = time.perf_counter()
+    for g in range(ngal):
+        f = np.exp(rng.uniform(np.log(1.0), np.log(4.0)))
+        size = (int(64 * f) // 2 * 2, int(64 * f) // 2 * 2, int(32 * f))
+        model = build_galaxy_model(drv, size)
+        truth = dict(xpos=0.0, ypos=0.0, vsys=0.0, posa=30.0, incl=50.0,
+                     bpt_a=1.0, bpt_s=0.15 * size[0], vpt_rt=0.05 * size[0],
+                     vpt_vt=200.0, dpt_a=20.0)
+        cube = model.model_h(truth)['scube']['d'].astype(np.float32)
+        mock = (cube + rng.normal(0, 0.01, cube.shape)).astype(np.float32)
+        gals.append(GpuGalaxy(drv, model.dmodel(), model.gmodel(), mock,
+                              error=0.01, max_batch=max_batch,
+                              template_params=truth, orders=(0, 1, 2),
+                              streams=streams))
+        sizes.append(size)
+    t_setup = time.perf_counter() - t0
+    works = [g.work_bytes() for g in gals]
+    free, total = torch.cuda.mem_get_info()
+    budget = int(min(sum(works), 0.5 * free))
+    srv = Server(hbm_budget=budget, slots=slots, max_batch=max_batch)
+    for i, g in enumerate(gals):
+        srv.add(i, g)
+    # arrival schedule and request parameters, fixed by the seed
+    n = int(rate * seconds)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    targets = rng.integers(0, ngal, n)
+    pars = [dict(xpos=0.0, ypos=0.0, vsys=0.0,
+                 posa=float(rng.uniform(0, 360)),
+                 incl=float(rng.uniform(20, 80)), bpt_a=1.0,
+                 bpt_s=0.15 * sizes[t][0], vpt_rt=0.05 * sizes[t][0],
+                 vpt_vt=float(rng.uniform(100, 300)),
+                 dpt_a=float(rng.uniform(5, 50))) for t in targets]
+    # warm the graphs of the first batch sizes of every galaxy
+    for i, g in enumerate(gals):
+        for k in (1, 2):
+            g.results(g.launch([pars[0]] * k, 0), k)
+        g.release()
+    torch.cuda.synchronize()
+    futs = []
+    srv.start()
+    start = time.perf_counter()
+    for i in range(n):
+        delay = start + arrivals[i] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        futs.append(srv.submit(int(targets[i]), pars[i]))
+    for f in futs:
+        f.result(300)
+    end = time.perf_counter()
+    srv.stop()
+    lat = srv.latency_percentiles()
+    out = dict(requests=n, offered_rate=rate, seconds=seconds,
+               galaxies=ngal, slots=slots, max_batch=max_batch,
+               throughput=n / (end - start), **lat,
+               launches=srv.stats['launches'],
+               mean_batch=float(np.mean(srv.stats['batch_sizes'])),
+               max_batch_seen=srv.stats['max_batch_seen'],
+               hbm_budget_bytes=budget, work_bytes_total=int(sum(works)),
+               releases=srv.stats['releases'],
+               cube_sizes=dict(min=list(min(sizes)), max=list(max(sizes))),
+               setup_s=t_setup, returns='chi2 + moment maps 0/1/2')
+    for g in gals:
+        g.close()
+    return out
 
 
 def percentiles(xs):
@@ -196,6 +299,10 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-graph', action='store_true')
     ap.add_argument('--thick', action='store_true')
+    ap.add_argument('--serve-galaxies', type=int, default=16)
+    ap.add_argument('--serve-seconds', type=float, default=8.0)
+    ap.add_argument('--serve-rate', type=float, default=1000.0)
+    ap.add_argument('--no-serve', action='store_true')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only copy a 1 GiB device buffer 3 times (known '
                          'bytes for scaling FETCH_SIZE / WRITE_SIZE)')
@@ -312,6 +419,12 @@ def main():
     achieved = kernels[dom].get('achieved_gbs')
     traffic = measured_traffic(dom)
 
+    serving = None
+    if not args.no_serve:
+        ev.close()
+        serving = serve_benchmark(drv, args.serve_galaxies,
+                                  args.serve_seconds, args.serve_rate)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -348,6 +461,7 @@ def main():
                       frac=(achieved / HBM_PEAK_GBS) if achieved else None,
                       traffic=traffic, kernel=dom,
                       sets_per_launch=per_chunk),
+        serving=serving,
         cpu_baseline=cpu,
         host=dict(cpu=platform.processor() or platform.machine(),
                   nproc=os.cpu_count()),

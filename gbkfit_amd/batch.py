@@ -277,7 +277,10 @@ class BatchEvaluator:
                                       dtype=t.float32,
                                       device=self.driver.device)
         # evaluate on torch's current stream, where the uploads and the
-        # downloads of evaluate() are ordered
+        # downloads of evaluate() are ordered: rebind the context to it (its
+        # handle is 0 for the default stream, which set_stream reads as "the
+        # context's stream")
+        self.driver.context()
         self.set_stream(t.cuda.current_stream())
         N.call('gbkhip_batch_evaluate', self.handle,
                self._params_dev.data_ptr(), int(nsets),
