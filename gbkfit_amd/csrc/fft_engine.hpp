@@ -170,6 +170,146 @@ template <> struct Bfly<16> {
 // threads of a block with n L <= kFastE T.
 constexpr int kFastE = 16;
 
+// ---- composite radices for compile-time plans --------------------------
+// cos and sin of 2 pi m / R at compile time: reduce to the nearest quarter
+// turn, Taylor series on |theta| <= pi / 4 (terms to x^25: error far below
+// FP64 rounding), then rotate by the quarter turns.
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+constexpr double cx_sin_small(double x)
+{
+    double term = x, sum = x;
+    for (int i = 1; i <= 12; ++i) {
+        term *= -x * x / ((2.0 * i) * (2.0 * i + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+
+constexpr double cx_cos_small(double x)
+{
+    double term = 1.0, sum = 1.0;
+    for (int i = 1; i <= 12; ++i) {
+        term *= -x * x / ((2.0 * i - 1.0) * (2.0 * i));
+        sum += term;
+    }
+    return sum;
+}
+
+// (cos, sin)(2 pi m / R); m taken mod R
+constexpr double cx_cos_turn(int m, int R)
+{
+    m = ((m % R) + R) % R;
+    const int q = (4 * m + R / 2) / R;              // nearest quarter turn
+    const double theta = kTwoPi * ((double)m / R - 0.25 * q);
+    const double c = cx_cos_small(theta), s = cx_sin_small(theta);
+    switch (q & 3) {
+    case 0: return c;
+    case 1: return -s;
+    case 2: return -c;
+    default: return s;
+    }
+}
+
+constexpr double cx_sin_turn(int m, int R)
+{
+    m = ((m % R) + R) % R;
+    const int q = (4 * m + R / 2) / R;
+    const double theta = kTwoPi * ((double)m / R - 0.25 * q);
+    const double c = cx_cos_small(theta), s = cx_sin_small(theta);
+    switch (q & 3) {
+    case 0: return s;
+    case 1: return c;
+    case 2: return -s;
+    default: return -c;
+    }
+}
+
+// v * exp(sign 2 pi i m / R) for constant m, R: exact quarter turns are
+// swaps and negations, other angles one complex multiply.
+template <int M, int R>
+__device__ __forceinline__ float2 rot_const(float2 v, float sign)
+{
+    constexpr int m = ((M % R) + R) % R;
+    if constexpr (m == 0) {
+        return v;
+    } else if constexpr (4 * m == R) {
+        return mul_si(v, sign);
+    } else if constexpr (2 * m == R) {
+        return make_float2(-v.x, -v.y);
+    } else if constexpr (4 * m == 3 * R) {
+        return mul_si(v, -sign);
+    } else {
+        constexpr float c = (float)cx_cos_turn(m, R);
+        constexpr float s = (float)cx_sin_turn(m, R);
+        return cmul(v, make_float2(c, sign * s));
+    }
+}
+
+// R = A B point DFT in registers: with n = n1 + A n2 and k = B k1 + k2,
+// X[k] = sum_n1 W_A^(n1 k1) W_R^(n1 k2) sum_n2 v[n1 + A n2] W_B^(n2 k2).
+// B-point DFTs over n2, twiddles, A-point DFTs over n1; X[B k1 + k2] ends
+// in v[k1 + A k2].
+template <int A, int B> struct Bfly2 {
+    static constexpr int R = A * B;
+    __device__ static constexpr int perm(int q) { return q / B + A * (q % B); }
+    template <int N1, int K2>
+    __device__ static __forceinline__ void twiddle(float2* v, float sign)
+    {
+        if constexpr (K2 < B) {
+            v[N1 + A * K2] = rot_const<N1 * K2, R>(v[N1 + A * K2], sign);
+            twiddle<N1, K2 + 1>(v, sign);
+        }
+    }
+    template <int N1>
+    __device__ static __forceinline__ void twiddles(float2* v, float sign)
+    {
+        if constexpr (N1 < A) {
+            twiddle<N1, 1>(v, sign);
+            twiddles<N1 + 1>(v, sign);
+        }
+    }
+    __device__ static void run(float2* v, float sign);
+};
+
+template <int A, int B>
+__device__ void Bfly2<A, B>::run(float2* v, float sign)
+{
+    float2 t[B > A ? B : A];
+#pragma unroll
+    for (int n1 = 0; n1 < A; ++n1) {
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) t[n2] = v[n1 + A * n2];
+        Bfly<B>::run(t, sign);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2)
+            v[n1 + A * k2] = t[Bfly<B>::perm(k2)];
+    }
+    twiddles<1>(v, sign);
+#pragma unroll
+    for (int k2 = 0; k2 < B; ++k2) {
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) t[n1] = v[n1 + A * k2];
+        Bfly<A>::run(t, sign);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1)
+            v[k1 + A * k2] = t[Bfly<A>::perm(k1)];
+    }
+}
+
+#define GBK_COMPOSITE(R_, A_, B_)                                            \
+    template <> struct Bfly<R_> : Bfly2<A_, B_> {};
+GBK_COMPOSITE(6, 2, 3)
+GBK_COMPOSITE(9, 3, 3)
+GBK_COMPOSITE(10, 2, 5)
+GBK_COMPOSITE(12, 4, 3)
+GBK_COMPOSITE(15, 3, 5)
+GBK_COMPOSITE(18, 2, 9)
+GBK_COMPOSITE(20, 4, 5)
+GBK_COMPOSITE(24, 8, 3)
+GBK_COMPOSITE(25, 5, 5)
+#undef GBK_COMPOSITE
+
 // One Stockham stage of radix R.  Thread (l, j0) = (tid % L, tid / L)
 // handles butterflies j = j0 + c JS (JS = T / L) of line l, so its LDS
 // addresses differ from slot to slot by compile-time offsets and from
@@ -265,22 +405,41 @@ struct CPlan {
     int tw_total = 0;
 };
 
-// Same radix choice as the runtime planner (fast_radices in fft.hip).
+// Radices of compile-time plans: the fewest stages with radices from
+// kCRadices (composites run as two-level butterflies in registers), ties
+// broken by the smallest largest radix; stages in descending radix order.
+constexpr int kCRadices[] = {25, 24, 20, 18, 16, 15, 12, 10, 9, 8, 6, 5, 4,
+                             3, 2};
+constexpr int kCPlanMax = 4096;
+
 __host__ __device__ constexpr CPlan cplan(int n)
 {
     CPlan P;
-    int m = n, a2 = 0, a3 = 0, a5 = 0;
-    while (m % 2 == 0) { m /= 2; ++a2; }
-    while (m % 3 == 0) { m /= 3; ++a3; }
-    while (m % 5 == 0) { m /= 5; ++a5; }
-    if (m != 1) return P;
-    const int s2 = (a2 + 3) / 4;
-    for (int s = 0; s < s2; ++s) {
-        const int bits = a2 / s2 + (s < a2 % s2 ? 1 : 0);
-        P.R[P.nst++] = 1 << bits;
+    if (n < 2 || n > kCPlanMax) return P;
+    int st[kCPlanMax + 1] = {}, mx[kCPlanMax + 1] = {}, ch[kCPlanMax + 1] = {};
+    for (int m = 2; m <= n; ++m) st[m] = 1 << 20;
+    for (int m = 2; m <= n; ++m) {
+        if (n % m) continue;
+        for (int r : kCRadices) {
+            if (m % r || st[m / r] >= (1 << 20)) continue;
+            const int sc = st[m / r] + 1;
+            const int xc = mx[m / r] > r ? mx[m / r] : r;
+            if (sc < st[m] || (sc == st[m] && xc < mx[m])) {
+                st[m] = sc;
+                mx[m] = xc;
+                ch[m] = r;
+            }
+        }
     }
-    for (int i = 0; i < a3; ++i) P.R[P.nst++] = 3;
-    for (int i = 0; i < a5; ++i) P.R[P.nst++] = 5;
+    if (st[n] >= (1 << 20) || st[n] > kFMax) return P;
+    for (int m = n; m > 1; m /= ch[m]) P.R[P.nst++] = ch[m];
+    for (int i = 0; i < P.nst; ++i)          // descending radix order
+        for (int j = i + 1; j < P.nst; ++j)
+            if (P.R[j] > P.R[i]) {
+                const int t = P.R[i];
+                P.R[i] = P.R[j];
+                P.R[j] = t;
+            }
     int ns = 1, off = 0;
     for (int s = 0; s < P.nst; ++s) {
         P.NS[s] = ns;
