@@ -52,6 +52,7 @@ sys.path.insert(0, os.path.join(ROOT, 'tests'))
 METRIC = ("model-cube evals/sec (128²×96 smdisk+PSF+LSF), "
           "batched vs plain; p99 latency")
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 SIG = 1 / math.sqrt(8 * math.log(2))
 
 
@@ -255,7 +256,17 @@ def algorithmic_bytes(ev, info):
     sx, sy, _ = ev.desc.scale
     X, Y = nxl * sx, nyl * sy
     n_lo = nxl * nyl * nzl
-    if info['fused']:
+    if info['pipeline'] == 'direct':
+        nx, ny, nz = info['fft_size']        # inputs that reach the outputs
+        p1 = 4 * nz * ny * nxl               # x-filtered rows
+        p2 = 4 * nz * nyl * nxl              # x- and y-filtered columns
+        return {
+            'batch_lines_thin': 16 * nx * ny,
+            'sep_xpass': 16 * nx * ny + p1,
+            'sep_ypass': p1 + p2,
+            'sep_zpass': p2 + 4 * n_lo,
+        }
+    if info['pipeline'] == 'fused':
         nx, ny, nz = info['fft_size']
         nzh = nz // 2 + 1
         plane = 8 * nzh * ny * nx            # one complex half spectrum
@@ -270,6 +281,21 @@ def algorithmic_bytes(ev, info):
     n_pad = int(np.prod(ev.size_hi))
     return {'smdisk': 4 * n_pad, 'fft_convolution': 12 * n_pad,
             'downscale_chi2': 4 * X * Y * nzl + 4 * n_lo}
+
+
+def algorithmic_flops(ev, info):
+    """FP32 flops per parameter set of the direct pipeline's filter passes:
+    one multiply-add per tap of the block-mean-folded kernel (s + 2 edge
+    taps along each axis), zero-padded taps not counted."""
+    if info['pipeline'] != 'direct':
+        return {}
+    nxl, nyl, nzl = ev.size_lo
+    nx, ny, nz = info['fft_size']
+    s, e = ev.desc.scale, ev.desc.edge_hi
+    tx, ty, tz = (s[i] + 2 * e[i] for i in range(3))
+    return {'sep_xpass': 2 * nz * ny * nxl * tx,
+            'sep_ypass': 2 * nz * nyl * nxl * ty,
+            'sep_zpass': 2 * nzl * nyl * nxl * tz}
 
 
 def measured_traffic(kernel):
@@ -406,6 +432,7 @@ def main():
     launches = args.steps * max(chunks, 1)
     per_chunk = args.nsets / max(chunks, 1)
     algo = algorithmic_bytes(ev, info)
+    aflops = algorithmic_flops(ev, info)
     kernels = {}
     for k, t in kern_ms.items():
         avg_ms = t / launches
@@ -415,9 +442,30 @@ def main():
             gbs = b * per_chunk / (avg_ms * 1e-3) / 1e9
             kernels[k].update(bytes_per_set=b, achieved_gbs=gbs,
                               frac=gbs / HBM_PEAK_GBS)
+        f = aflops.get(k)
+        if f is not None and avg_ms > 0:
+            tfs = f * per_chunk / (avg_ms * 1e-3) / 1e12
+            kernels[k].update(flops_per_set=f, achieved_tflops=tfs,
+                              frac_fp32=tfs / FP32_PEAK_TFLOPS)
     dom = max(kern_ms, key=kern_ms.get)
-    achieved = kernels[dom].get('achieved_gbs')
+    dk = kernels[dom]
     traffic = measured_traffic(dom)
+    # the binding roof of the dominant kernel: FP32 compute when its flop
+    # rate is the larger fraction of peak, else HBM bandwidth
+    if dk.get('frac_fp32', 0) > dk.get('frac', 0):
+        roofline = dict(bound='mfma', achieved=dk['achieved_tflops'],
+                        peak=FP32_PEAK_TFLOPS, unit='TFLOP/s',
+                        frac=dk['frac_fp32'], traffic=traffic, kernel=dom,
+                        note='FP32 multiply-adds on the vector ALUs '
+                             '(packed FP32 disabled: at most half of the '
+                             'FP32 peak is reachable); HBM: %.0f GB/s'
+                             % dk['achieved_gbs'])
+    else:
+        achieved = dk.get('achieved_gbs')
+        roofline = dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS,
+                        unit='GB/s',
+                        frac=(achieved / HBM_PEAK_GBS) if achieved else None,
+                        traffic=traffic, kernel=dom)
 
     serving = None
     if not args.no_serve:
@@ -443,8 +491,9 @@ def main():
             f"seeded mock; {args.nsets} parameter sets per step"),
             nsets=args.nsets, chunk=args.chunk, graph=use_graph,
             padding='reference', dcube_size_hi=list(ev.size_hi),
-            pipeline='spectral-first fused' if info['fused']
-            else 'padded-cube FFT',
+            pipeline=dict(direct='direct separable convolution',
+                          fused='spectral-first fused FFT',
+                          padded='padded-cube FFT')[info['pipeline']],
             fft_size=list(info['fft_size'])),
         plain=dict(value=plain_value, unit='evals/s',
                    sets=len(plain_sets), **percentiles(plat)),
@@ -452,15 +501,11 @@ def main():
         batch_latency=percentiles(lat),
         per_eval_latency_batched_ms=1e3 * t_batched / evals,
         direct_launch_evals_per_s=evals / t_timed,
-        stages_ms_per_step=dict(smdisk=stage_ms[0] / args.steps,
-                                fft_convolution=stage_ms[1] / args.steps,
+        stages_ms_per_step=dict(model=stage_ms[0] / args.steps,
+                                convolution=stage_ms[1] / args.steps,
                                 downscale_chi2=stage_ms[2] / args.steps),
         kernels=kernels,
-        roofline=dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS,
-                      unit='GB/s',
-                      frac=(achieved / HBM_PEAK_GBS) if achieved else None,
-                      traffic=traffic, kernel=dom,
-                      sets_per_launch=per_chunk),
+        roofline=dict(roofline, sets_per_launch=per_chunk),
         serving=serving,
         cpu_baseline=cpu,
         host=dict(cpu=platform.processor() or platform.machine(),

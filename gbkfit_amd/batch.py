@@ -234,9 +234,10 @@ class BatchEvaluator:
             d.mask = self._mask.data_ptr()
         d.weight = float(weight)
         d.max_batch = int(max_batch)
-        if path not in ('auto', 'padded'):
-            raise ValueError("path must be 'auto' or 'padded'")
-        d.path = 0 if path == 'auto' else 1
+        paths = dict(auto=0, padded=1, fused=2, direct=3)
+        if path not in paths:
+            raise ValueError(f"path must be one of {sorted(paths)}")
+        d.path = paths[path]
         self.desc = d
         self.size_lo = tuple(cube.size())
         self.size_hi = tuple(size_hi)
@@ -298,12 +299,13 @@ class BatchEvaluator:
         return chi2
 
     def info(self):
-        """dict(fused=bool, fft_size=(x, y, z), nparts=int): whether the
-        spectral-first fused pipeline runs, and the FFT sizes in use."""
+        """dict(pipeline='padded'|'fused'|'direct', fft_size=(x, y, z),
+        nparts=int): the batched pipeline in use and its transform sizes."""
         v = (_c.c_int * 5)()
         N.call('gbkhip_batch_info', self.handle, v)
-        return dict(fused=bool(v[0]), fft_size=(v[1], v[2], v[3]),
-                    nparts=v[4])
+        pipe = ('padded', 'fused', 'direct')[v[0]]
+        return dict(pipeline=pipe, fused=pipe == 'fused',
+                    fft_size=(v[1], v[2], v[3]), nparts=v[4])
 
     def set_timing(self, enable):
         N.call('gbkhip_batch_set_timing', self.handle, int(bool(enable)))
@@ -338,6 +340,8 @@ class BatchEvaluator:
                      'fx_rows_conv', 'fx_lines (inverse y)',
                      'fz_c2r_down_chi2', 'batch_chi2_final')
     PADDED_STAGES = ('smdisk', 'fft_convolution', 'downscale_chi2')
+    DIRECT_KERNELS = ('batch_lines_thin', 'sep_xpass', 'sep_ypass',
+                      'sep_zpass', 'batch_chi2_final')
 
     def kernel_times(self):
         """{kernel: milliseconds} of the last timed evaluation, summed over
@@ -345,7 +349,8 @@ class BatchEvaluator:
         ms = (_c.c_double * 8)()
         n = _c.c_int()
         N.call('gbkhip_batch_kernel_times', self.handle, ms, 8, _c.byref(n))
-        names = self.FUSED_KERNELS if n.value == 7 else self.PADDED_STAGES
+        names = {7: self.FUSED_KERNELS, 5: self.DIRECT_KERNELS}.get(
+            n.value, self.PADDED_STAGES)
         return {k: ms[i] for i, k in enumerate(names[:n.value])}
 
     def close(self):

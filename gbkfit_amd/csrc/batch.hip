@@ -508,6 +508,205 @@ __global__ __launch_bounds__(T) void fz_c2r_down_chi2(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Direct separable pipeline (thin disk, separable PSF x LSF kernel)
+//
+// With K = Wz(z) Wy(y) Wx(x) of half-widths e, and the block mean of the
+// downscale folded into the taps, the low-resolution cube is
+//   lo(X, Y, Z) = sum_u,v,w Wx[u] Wy[v] Wz[w] m(sx X + u, sy Y + v, sz Z + w)
+// with W_a[u] = (1/s_a) sum_{d < s_a} K_a[d - u + e_a], u in [0, s_a + 2 e_a)
+// (indices relative to the first input of the block).  Three passes:
+//   S1 sep_xpass   lines of one hi-res row deposited in LDS for a chunk of
+//                  channels, x taps -> P1 (z, y, X)
+//   S2 sep_ypass   y taps over LDS column tiles -> P2 (z, Y, X)
+//   S3 sep_zpass   z taps, residual, FP64 partial sums of r^2 per tile
+// Every output is a fixed-order FP32 dot product of at most TX (TY, TZ)
+// taps; templates fix the tap buckets and downscale factors so the inner
+// loops are register-blocked with constant offsets.
+// ---------------------------------------------------------------------------
+constexpr int kSepThreads = 256;
+constexpr int kSepZC = 32;          // most channels per S1 block
+constexpr int kSepJX = 4;           // outputs per thread, S1
+constexpr int kSepJY = 4;           // outputs per thread, S2
+constexpr int kSepJZ = 16;          // outputs per thread, S3
+
+template <int SX, int TX>
+__global__ __launch_bounds__(256) void sep_xpass(
+    const float4* __restrict__ lines, long long lstride, int lpitch,
+    gbkhip_grid g, const float* __restrict__ wx, int need_x, int need_z,
+    int nxl, int mp, int zc, float* __restrict__ P1, long long p1stride,
+    int ny1)
+{
+    extern __shared__ float M[];          // [zc][mp]
+    __shared__ float s_w[TX];
+    constexpr int T = kSepThreads, J = kSepJX;
+    const int y = blockIdx.x, z0 = blockIdx.y * zc, set = blockIdx.z;
+    for (int i = threadIdx.x; i < TX; i += T) s_w[i] = wx[i];
+    // hi-res model values of row y, channels [z0, z0 + ZC): the spectral
+    // line of each spaxel inside its rint window (gmodels.hpp:91-116)
+    const float4* lrow = lines + set * lstride + (size_t)y * lpitch;
+    for (int x = threadIdx.x; x < mp; x += T) {
+        float4 V = make_float4(0, 0, 1, 0);
+        if (x < need_x) V = lrow[x];
+        int k0 = 1, k1 = 0;
+        LineEval e;
+        if (V.x != 0.0f) {
+            line_window(g, V.y, V.z, k0, k1);
+            k1 = min(k1, need_z - 1);
+            e = line_eval(g, V.x, V.y, V.z);
+        }
+#pragma unroll 4
+        for (int zz = 0; zz < zc; ++zz) {
+            const int k = z0 + zz;
+            M[zz * mp + x] = (k >= k0 && k <= k1) ? e.at(k) : 0.0f;
+        }
+    }
+    __syncthreads();
+    // outputs (z, X .. X + J - 1): inputs sx X + [0, sx (J - 1) + TX)
+    constexpr int NIN = SX * (J - 1) + TX;
+    const int nxb = (nxl + J - 1) / J;
+    float* out = P1 + set * p1stride;
+    for (int task = threadIdx.x; task < zc * nxb; task += T) {
+        const int zz = task / nxb, xb = task - zz * nxb;
+        const int z = z0 + zz;
+        if (z >= need_z) continue;
+        const int X0 = xb * J;
+        const float* in = M + zz * mp + SX * X0;
+        float v[NIN];
+#pragma unroll
+        for (int i = 0; i < NIN; ++i) v[i] = in[i];
+        float o[J] = {};
+#pragma unroll
+        for (int u = 0; u < TX; ++u) {
+            const float w = s_w[u];
+#pragma unroll
+            for (int j = 0; j < J; ++j) o[j] = fmaf(w, v[SX * j + u], o[j]);
+        }
+        float* dst = out + ((size_t)z * ny1 + y) * nxl + X0;
+#pragma unroll
+        for (int j = 0; j < J; ++j)
+            if (X0 + j < nxl) dst[j] = o[j];
+    }
+}
+
+// P2[z][Y][X] = sum_v Wy[v] P1[z][sy Y + v][X]: a block covers 64 X and
+// 4 J Y outputs of one channel, its input rows staged in LDS.
+template <int SY, int TY>
+__global__ __launch_bounds__(256) void sep_ypass(
+    const float* __restrict__ P1, long long p1stride, const float* wy,
+    int nxl, int nyl, int ny1, int need_z, float* __restrict__ P2,
+    long long p2stride)
+{
+    constexpr int T = kSepThreads, J = kSepJY, TXB = 64;
+    constexpr int YB = (T / TXB) * J;                 // Y outputs per block
+    constexpr int ROWS = SY * (YB - 1) + TY;
+    __shared__ float tile[ROWS * TXB];
+    __shared__ float s_w[TY];
+    const int X0 = blockIdx.x * TXB, Y0 = blockIdx.y * YB;
+    const int z = blockIdx.z % need_z, set = blockIdx.z / need_z;
+    for (int i = threadIdx.x; i < TY; i += T) s_w[i] = wy[i];
+    const float* src = P1 + set * p1stride + (size_t)z * ny1 * nxl;
+    for (int i = threadIdx.x; i < ROWS * TXB; i += T) {
+        const int r = i / TXB, c = i - r * TXB;
+        const int y = SY * Y0 + r, X = X0 + c;
+        tile[i] = (y < ny1 && X < nxl) ? src[(size_t)y * nxl + X] : 0.0f;
+    }
+    __syncthreads();
+    const int c = threadIdx.x % TXB, yb = threadIdx.x / TXB;
+    constexpr int NIN = SY * (J - 1) + TY;
+    float v[NIN];
+#pragma unroll
+    for (int i = 0; i < NIN; ++i) v[i] = tile[(SY * J * yb + i) * TXB + c];
+    float o[J] = {};
+#pragma unroll
+    for (int u = 0; u < TY; ++u) {
+        const float w = s_w[u];
+#pragma unroll
+        for (int j = 0; j < J; ++j) o[j] = fmaf(w, v[SY * j + u], o[j]);
+    }
+    const int X = X0 + c;
+    float* dst = P2 + set * p2stride + (size_t)z * nyl * nxl;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int Y = Y0 + yb * J + j;
+        if (X < nxl && Y < nyl) dst[(size_t)Y * nxl + X] = o[j];
+    }
+}
+
+// lo[Z][Y][X] = sum_w Wz[w] P2[sz Z + w][Y][X], then the residual against
+// the data and one FP64 partial sum of r^2 per block (fixed order).  A block
+// covers 64 X x 4 Y columns and kSepJZ channels.
+template <int SZ, int TZ>
+__global__ __launch_bounds__(256) void sep_zpass(
+    const float* __restrict__ P2, long long p2stride, const float* wz,
+    int nxl, int nyl, int nzl, int need_z, int zblocks, DataArgs da,
+    float* __restrict__ lo_out, long long lstride,
+    double* __restrict__ partials, int nparts)
+{
+    constexpr int T = kSepThreads, J = kSepJZ, TXB = 64, TYB = T / TXB;
+    constexpr int NIN = SZ * (J - 1) + TZ;
+    __shared__ float s_w[TZ];
+    __shared__ double red[T / 64];
+    const int set = blockIdx.z / zblocks, zb = blockIdx.z % zblocks;
+    const int X = blockIdx.x * TXB + threadIdx.x % TXB;
+    const int Y = blockIdx.y * TYB + threadIdx.x / TXB;
+    const int Z0 = zb * J;
+    for (int i = threadIdx.x; i < TZ; i += T) s_w[i] = wz[i];
+    __syncthreads();
+    double acc = 0.0;
+    if (X < nxl && Y < nyl) {
+        const size_t plane = (size_t)nyl * nxl, col = (size_t)Y * nxl + X;
+        const float* src = P2 + set * p2stride + col;
+        float v[NIN];
+#pragma unroll
+        for (int i = 0; i < NIN; ++i) {
+            const int z = SZ * Z0 + i;
+            v[i] = z < need_z ? src[(size_t)z * plane] : 0.0f;
+        }
+        float o[J] = {};
+#pragma unroll
+        for (int w = 0; w < TZ; ++w) {
+            const float ww = s_w[w];
+#pragma unroll
+            for (int j = 0; j < J; ++j) o[j] = fmaf(ww, v[SZ * j + w], o[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const int Z = Z0 + j;
+            if (Z >= nzl) break;
+            const size_t gi = (size_t)Z * plane + col;
+            if (lo_out) lo_out[set * lstride + gi] = o[j];
+            // objective.hpp:7-33 with mdl_w = mdl_m = none
+            float r = o[j] - da.obs[gi];
+            r /= da.err ? da.err[gi] : da.err_scalar;
+            if (da.mask) r *= da.mask[gi];
+            r *= da.weight;
+            acc += (double)r * (double)r;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < T / 64; ++w) t += red[w];
+        const int part = (blockIdx.y * gridDim.x + blockIdx.x) * zblocks + zb;
+        partials[(size_t)set * nparts + part] = t;
+    }
+}
+
+struct Sep {
+    bool on = false;
+    int need[3] = {0, 0, 0};      // inputs that reach the outputs
+    int tx = 0, ty = 0, tz = 0;   // tap buckets (zero-padded)
+    int mp = 0;                   // S1 LDS row length
+    int zc = 0;                   // channels per S1 block (tile <= 64 KB)
+    DevBuf wx, wy, wz;            // taps
+    DevBuf P1, P2;                // work: per set (z, y, X) and (z, Y, X)
+    long long p1stride = 0, p2stride = 0;
+    int zblocks = 1;
+};
+
 struct Fused {
     bool on = false;
     int nx = 0, ny = 0, nz = 0, nzh = 0;   // FFT sizes (nz even)
@@ -540,6 +739,11 @@ struct Batch {
     DevBuf rnodes;
     std::shared_ptr<FFTPlan> plan;
     Fused fused;            // spectral-first pipeline, when it applies
+    Sep sep;                // direct separable pipeline, when it applies
+    // separable kernel found by the analysis of the PSF x LSF cube: sums of
+    // pcube over z (psf, ny x nx) and over y, x (lsf); empty when absent
+    std::vector<double> ka_psf, ka_lsf;
+    double ka_total = 0.0;
     DevBuf spectrum;        // kernel spectrum (nz, ny, nxh) complex
     DevBuf hi;              // chunk x padded cube
     DevBuf spec;            // chunk x complex work spectrum
@@ -629,7 +833,10 @@ static size_t work_bytes(const Batch& B)
 {
     const size_t mb = (size_t)B.desc.max_batch;
     size_t n = mb * (size_t)B.nparts * sizeof(double);
-    if (B.fused.on) {
+    if (B.sep.on) {
+        n += mb * (size_t)B.sep.need[1] * B.sep.need[0] * sizeof(float4);
+        n += mb * (size_t)(B.sep.p1stride + B.sep.p2stride) * sizeof(float);
+    } else if (B.fused.on) {
         n += mb * (size_t)B.fused.sstride * sizeof(float2);
         n += mb * (size_t)B.fused.lstride * sizeof(float4);
     } else {
@@ -650,7 +857,12 @@ static void alloc_work(Batch& B)
     if (B.work) return;
     const size_t mb = (size_t)B.desc.max_batch;
     B.partials.alloc(mb * B.nparts * sizeof(double));
-    if (B.fused.on) {
+    if (B.sep.on) {
+        B.fused.lines.alloc(mb * (size_t)B.sep.need[1] * B.sep.need[0] *
+                            sizeof(float4));
+        B.sep.P1.alloc(mb * (size_t)B.sep.p1stride * sizeof(float));
+        B.sep.P2.alloc(mb * (size_t)B.sep.p2stride * sizeof(float));
+    } else if (B.fused.on) {
         B.fused.S.alloc(mb * B.fused.sstride * sizeof(float2));
         B.fused.lines.alloc(mb * B.fused.lstride * sizeof(float4));
     } else {
@@ -666,6 +878,8 @@ static void release_work(Batch& B)
     B.partials.release();
     B.fused.S.release();
     B.fused.lines.release();
+    B.sep.P1.release();
+    B.sep.P2.release();
     B.hi.release();
     B.spec.release();
     B.work = false;
@@ -675,7 +889,10 @@ static void release_work(Batch& B)
 // after convolution, after chi^2), the fused one 8 (one after each kernel).
 constexpr int kMarks = 8;
 
-static int n_marks(const Batch& b) { return b.fused.on ? 8 : 4; }
+static int n_marks(const Batch& b)
+{
+    return b.sep.on ? 6 : b.fused.on ? 8 : 4;
+}
 
 static void ensure_events(Batch& b, int chunks)
 {
@@ -753,6 +970,9 @@ static void plan_fused(Context& c, Batch& B)
     for (int z = 0; z < Nz; ++z)
         (dist(z, Nz) <= ez ? in_l : out_l) += std::fabs(lsf[z]);
     if (out_p > 1e-9 * in_p || out_l > 1e-9 * in_l) return;
+    B.ka_psf = psf;
+    B.ka_lsf = lsf;
+    B.ka_total = total;
     for (int i = 0; i < 3; ++i)
         F.need[i] = D.size_lo[i] * D.scale[i] + 2 * D.edge_hi[i];
     // FFT sizes with engine shapes: x rows, y lines, z columns of 64 or 32
@@ -950,12 +1170,219 @@ static void run_chunk_fused(Batch& b, const float* params, int set0,
     mark(b, ci, 7);
 }
 
+static int tap_bucket(int n)
+{
+    for (int b : {8, 16, 24, 32, 48, 64, 96, 128})
+        if (n <= b) return b;
+    return 0;
+}
+
+// Output d of a block sits at padded index s X + d + e and sees input
+// s X + u through the kernel offset t = d + e - u (c(q) = sum_t k(t)
+// m(q - t), k wrapped so k(t) = k[t mod N]):
+//   W[u] = (1/s) sum_{d < s, |d + e - u| <= e} k[(d + e - u) mod N] / norm,
+// u in [0, s + 2e), zero-padded to the bucket.
+static std::vector<float> block_taps(const std::vector<double>& k, int N,
+                                     int e, int s, double norm, int bucket)
+{
+    std::vector<float> w(bucket, 0.0f);
+    for (int u = 0; u < s + 2 * e; ++u) {
+        double acc = 0.0;
+        for (int d = 0; d < s; ++d) {
+            const int t = d + e - u;
+            if (t < -e || t > e) continue;
+            acc += k[(t % N + N) % N] / norm;
+        }
+        w[u] = (float)(acc / s);
+    }
+    return w;
+}
+
+// Direct separable pipeline: needs the kernel analysis of plan_fused and a
+// PSF that factors as px(x) py(y) (a Gaussian cut at 8 sigma does, to
+// ~1e-14).  Returns false when it does not apply.
+static bool plan_sep(Batch& B)
+{
+    const gbkhip_batch_desc& D = B.desc;
+    Sep& P = B.sep;
+    P.on = false;
+    if (B.ka_psf.empty()) return false;
+    const int Nx = D.size_hi[0], Ny = D.size_hi[1], Nz = D.size_hi[2];
+    const std::vector<double>& psf = B.ka_psf;
+    std::vector<double> px(Nx, 0.0), py(Ny, 0.0);
+    double ptot = 0.0, pmax = 0.0;
+    for (int y = 0; y < Ny; ++y)
+        for (int x = 0; x < Nx; ++x) {
+            const double v = psf[(size_t)y * Nx + x];
+            px[x] += v;
+            py[y] += v;
+            ptot += v;
+            pmax = std::max(pmax, std::fabs(v));
+        }
+    if (!(ptot > 0.0)) return false;
+    double worst = 0.0;
+    for (int y = 0; y < Ny; ++y)
+        for (int x = 0; x < Nx; ++x)
+            worst = std::max(worst, std::fabs(psf[(size_t)y * Nx + x] -
+                                              px[x] * py[y] / ptot));
+    if (worst > 1e-6 * pmax) return false;
+    const int ex = D.edge_hi[0], ey = D.edge_hi[1], ez = D.edge_hi[2];
+    const int sx = D.scale[0], sy = D.scale[1], sz = D.scale[2];
+    if (sx < 1 || sx > 3 || sy < 1 || sy > 3 || sz < 1 || sz > 3) return false;
+    P.tx = tap_bucket(sx + 2 * ex);
+    P.ty = tap_bucket(sy + 2 * ey);
+    P.tz = tap_bucket(sz + 2 * ez);
+    if (!P.tx || !P.ty || !P.tz) return false;
+    for (int i = 0; i < 3; ++i)
+        P.need[i] = D.size_lo[i] * D.scale[i] + 2 * D.edge_hi[i];
+    const std::vector<float> wx = block_taps(px, Nx, ex, sx, ptot, P.tx);
+    const std::vector<float> wy = block_taps(py, Ny, ey, sy, ptot, P.ty);
+    const std::vector<float> wz = block_taps(B.ka_lsf, Nz, ez, sz,
+                                             B.ka_total, P.tz);
+    auto up = [](DevBuf& b, const std::vector<float>& h) {
+        b.alloc(h.size() * sizeof(float));
+        GBK_HIP(hipMemcpy(b.ptr, h.data(), h.size() * sizeof(float),
+                          hipMemcpyHostToDevice));
+    };
+    up(P.wx, wx);
+    up(P.wy, wy);
+    up(P.wz, wz);
+    const int nxl = D.size_lo[0], nyl = D.size_lo[1], nzl = D.size_lo[2];
+    const int nxb = (nxl + kSepJX - 1) / kSepJX;
+    P.mp = sx * nxb * kSepJX + P.tx;
+    P.zc = std::min(kSepZC, (int)((64 * 1024) / (P.mp * sizeof(float))));
+    if (P.zc < 1) return false;
+    P.p1stride = (long long)P.need[2] * P.need[1] * nxl;
+    P.p2stride = (long long)P.need[2] * nyl * nxl;
+    P.zblocks = (nzl + kSepJZ - 1) / kSepJZ;
+    return true;
+}
+
+// Multiply-adds per set of the direct pipeline and (complex butterfly
+// work, 6 flops per element per radix-2 level) of the fused FFT one.
+static bool sep_cheaper(const Batch& B)
+{
+    const gbkhip_batch_desc& D = B.desc;
+    const Sep& P = B.sep;
+    const double nxl = D.size_lo[0], nyl = D.size_lo[1], nzl = D.size_lo[2];
+    const double direct = (double)P.need[2] * P.need[1] * nxl * P.tx +
+                          (double)P.need[2] * nyl * nxl * P.ty +
+                          nzl * nyl * nxl * P.tz;
+    if (!B.fused.on) return true;
+    const Fused& F = B.fused;
+    const double nc = (double)F.nzh * F.ny * F.nx;
+    const double fft = 6.0 * nc * (2 * std::log2((double)F.nx) +
+                                   2 * std::log2((double)F.ny) +
+                                   std::log2((double)F.nz));
+    return direct < fft;
+}
+
+#define GBK_SEP_CASE(KERNEL, S_, T_, SEL, ...)                               \
+    case (S_) * 256 + (T_):                                                  \
+        hipLaunchKernelGGL((KERNEL<S_, T_>), __VA_ARGS__);                   \
+        break;
+#define GBK_SEP_DISPATCH(KERNEL, S, TAPS, ...)                               \
+    switch ((S) * 256 + (TAPS)) {                                            \
+    GBK_SEP_CASE(KERNEL, 1, 8, 0, __VA_ARGS__)                               \
+    GBK_SEP_CASE(KERNEL, 1, 16, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 1, 24, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 1, 32, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 1, 48, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 1, 64, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 1, 96, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 1, 128, 0, __VA_ARGS__)                             \
+    GBK_SEP_CASE(KERNEL, 2, 8, 0, __VA_ARGS__)                               \
+    GBK_SEP_CASE(KERNEL, 2, 16, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 2, 24, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 2, 32, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 2, 48, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 2, 64, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 2, 96, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 2, 128, 0, __VA_ARGS__)                             \
+    GBK_SEP_CASE(KERNEL, 3, 8, 0, __VA_ARGS__)                               \
+    GBK_SEP_CASE(KERNEL, 3, 16, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 3, 24, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 3, 32, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 3, 48, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 3, 64, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 3, 96, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 3, 128, 0, __VA_ARGS__)                             \
+    default:                                                                 \
+        throw Error{GBKHIP_ERR_INTERNAL, "no separable kernel for shape"};   \
+    }
+
+static void run_chunk_sep(Batch& b, const float* params, int set0,
+                          int nsets, double* chi2, float* cubes, int ci)
+{
+    Context& c = *b.ctx;
+    const gbkhip_batch_desc& D = b.desc;
+    const Sep& P = b.sep;
+    const float* par = params + (size_t)set0 * D.pstride;
+    const int nxl = D.size_lo[0], nyl = D.size_lo[1], nzl = D.size_lo[2];
+    const int nx = P.need[0], ny = P.need[1], nz = P.need[2];
+    float4* lines = b.fused.lines.as<float4>();
+    const long long lstride = (long long)ny * nx;
+    {
+        dim3 g0(ceil_div(nx, 64), ceil_div(ny, 4), nsets);
+        hipLaunchKernelGGL(batch_lines_thin, g0, dim3(64, 4), 0, c.stream,
+                           b.tmpl, b.lay, par, D.pstride, D.grid, nx, ny, nx,
+                           ny, lines, lstride);
+        GBK_HIP(hipGetLastError());
+    }
+    mark(b, ci, 1);
+    {
+        dim3 grid(ny, ceil_div(nz, P.zc), nsets);
+        const size_t lds = (size_t)P.zc * P.mp * sizeof(float);
+        GBK_SEP_DISPATCH(sep_xpass, D.scale[0], P.tx, grid, dim3(kSepThreads),
+                         lds, c.stream, lines, lstride, nx, D.grid,
+                         P.wx.as<float>(), nx, nz, nxl, P.mp, P.zc,
+                         P.P1.as<float>(), P.p1stride, ny);
+        GBK_HIP(hipGetLastError());
+    }
+    mark(b, ci, 2);
+    {
+        constexpr int YB = (kSepThreads / 64) * kSepJY;
+        dim3 grid(ceil_div(nxl, 64), ceil_div(nyl, YB), nz * nsets);
+        GBK_SEP_DISPATCH(sep_ypass, D.scale[1], P.ty, grid, dim3(kSepThreads),
+                         0, c.stream, P.P1.as<float>(), P.p1stride,
+                         P.wy.as<float>(), nxl, nyl, ny, nz,
+                         P.P2.as<float>(), P.p2stride);
+        GBK_HIP(hipGetLastError());
+    }
+    mark(b, ci, 3);
+    DataArgs da;
+    da.obs = D.obs;
+    da.err = D.err;
+    da.mask = D.mask;
+    da.err_scalar = D.err_scalar;
+    da.weight = D.weight;
+    {
+        dim3 grid(ceil_div(nxl, 64), ceil_div(nyl, kSepThreads / 64),
+                  P.zblocks * nsets);
+        GBK_SEP_DISPATCH(sep_zpass, D.scale[2], P.tz, grid, dim3(kSepThreads),
+                         0, c.stream, P.P2.as<float>(), P.p2stride,
+                         P.wz.as<float>(), nxl, nyl, nzl, nz, P.zblocks, da,
+                         cubes ? cubes + (size_t)set0 * b.n_lo : nullptr,
+                         b.n_lo, b.partials.as<double>(), b.nparts);
+        GBK_HIP(hipGetLastError());
+    }
+    mark(b, ci, 4);
+    hipLaunchKernelGGL(batch_chi2_final, dim3(nsets), dim3(256), 0, c.stream,
+                       b.partials.as<double>(), b.nparts, chi2, set0);
+    GBK_HIP(hipGetLastError());
+    mark(b, ci, 5);
+}
+
 static void run_chunk(Batch& b, const float* params, int set0, int nsets,
                       double* chi2, float* cubes)
 {
     Context& c = *b.ctx;
     const int ci = set0 / b.desc.max_batch;
     mark(b, ci, 0);
+    if (b.sep.on) {
+        run_chunk_sep(b, params, set0, nsets, chi2, cubes, ci);
+        return;
+    }
     if (b.fused.on) {
         run_chunk_fused(b, params, set0, nsets, chi2, cubes, ci);
         return;
@@ -1059,9 +1486,32 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
             B.n_lo = (long long)D.size_lo[0] * D.size_lo[1] * D.size_lo[2];
             B.nparts = (int)std::max<long long>(
                 1, std::min<long long>(512, (B.n_lo + 4095) / 4096));
-            GBK_CHECK(D.path == 0 || D.path == 1, GBKHIP_ERR_ARG,
-                      "path must be 0 (automatic) or 1 (padded-cube FFT)");
+            GBK_CHECK(D.path >= 0 && D.path <= 3, GBKHIP_ERR_ARG,
+                      "path must be 0 (automatic), 1 (padded-cube FFT), "
+                      "2 (fused FFT) or 3 (direct separable)");
             plan_fused(*c, B);
+            if ((D.path == 0 || D.path == 3) && plan_sep(B) &&
+                (D.path == 3 || sep_cheaper(B))) {
+                B.fused.on = false;
+                B.nparts = (int)(ceil_div(D.size_lo[0], 64) *
+                                 ceil_div(D.size_lo[1], kSepThreads / 64) *
+                                 B.sep.zblocks);
+                B.sep.on = true;
+                B.ka_psf.clear();
+                B.ka_lsf.clear();
+                alloc_work(B);
+                GBK_HIP(hipStreamSynchronize(c->stream));
+                *out = b;
+                return;
+            }
+            GBK_CHECK(D.path != 3, GBKHIP_ERR_UNSUPPORTED,
+                      "direct separable path unavailable for this model "
+                      "(thin disk and separable PSF x LSF within the edge "
+                      "needed)");
+            GBK_CHECK(D.path != 2 || B.fused.on, GBKHIP_ERR_UNSUPPORTED,
+                      "fused FFT path unavailable for this model");
+            B.ka_psf.clear();
+            B.ka_lsf.clear();
             if (B.fused.on) {
                 B.nparts = B.fused.tiles_x * B.fused.tiles_y;
                 alloc_work(B);
@@ -1210,10 +1660,11 @@ int gbkhip_batch_info(gbkhip_batch* b, int* info)
         GBK_CHECK(b && info, GBKHIP_ERR_ARG, "null argument");
         const Batch& B = b->impl;
         const Fused& F = B.fused;
-        info[0] = F.on ? 1 : 0;
-        info[1] = F.on ? F.nx : B.desc.size_hi[0];
-        info[2] = F.on ? F.ny : B.desc.size_hi[1];
-        info[3] = F.on ? F.nz : B.desc.size_hi[2];
+        const Sep& P = B.sep;
+        info[0] = P.on ? 2 : F.on ? 1 : 0;
+        info[1] = P.on ? P.need[0] : F.on ? F.nx : B.desc.size_hi[0];
+        info[2] = P.on ? P.need[1] : F.on ? F.ny : B.desc.size_hi[1];
+        info[3] = P.on ? P.need[2] : F.on ? F.nz : B.desc.size_hi[2];
         info[4] = B.nparts;
     });
 }
@@ -1235,8 +1686,10 @@ int gbkhip_batch_stage_times(gbkhip_batch* b, double* ms, int* chunks)
         *chunks = B.timed_chunks;
         // stage boundaries among the marks
         static const int kFusedBnd[4] = {0, 2, 5, 7};
+        static const int kSepBnd[4] = {0, 1, 3, 5};
         static const int kPaddedBnd[4] = {0, 1, 2, 3};
-        const int* bnd = B.fused.on ? kFusedBnd : kPaddedBnd;
+        const int* bnd = B.sep.on ? kSepBnd
+                         : B.fused.on ? kFusedBnd : kPaddedBnd;
         for (int c = 0; c < B.timed_chunks; ++c) {
             GBK_HIP(hipEventSynchronize(
                 B.events[kMarks * c + n_marks(B) - 1]));

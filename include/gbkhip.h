@@ -218,9 +218,12 @@ typedef struct {
     float err_scalar;        /* error used when err is NULL (1: none) */
     float weight;
     int max_batch;           /* sets evaluated per chunk (buffer sizing) */
-    int path;                /* 0: automatic (spectral-first fused pipeline
-                                when the disk is thin and the PSF x LSF cube
-                                is separable), 1: always FFT the padded cube */
+    int path;                /* 0: automatic, 1: FFT of the padded cube,
+                                2: spectral-first fused FFT pipeline,
+                                3: direct separable convolution; 2 and 3 need
+                                a thin disk and a PSF x LSF cube that
+                                factors within the DCube edge; automatic
+                                takes the cheaper of 2 and 3, else 1 */
 } gbkhip_batch_desc;
 
 typedef struct gbkhip_batch gbkhip_batch;
@@ -252,16 +255,19 @@ int gbkhip_batch_release(gbkhip_batch* batch);
  * not), and whether the work buffers are allocated now. */
 int gbkhip_batch_memory(gbkhip_batch* batch, unsigned long long* resident,
                         unsigned long long* work, int* allocated);
-/* info[0] = 1 when the spectral-first fused pipeline runs, info[1..3] = the
- * FFT sizes (x, y, z) in use, info[4] = chi^2 partial sums per set. */
+/* info[0] = pipeline (0 padded-cube FFT, 1 fused FFT, 2 direct separable),
+ * info[1..3] = its transform sizes (x, y, z) (for the direct pipeline: the
+ * extent of the inputs that reach the outputs), info[4] = chi^2 partial sums
+ * per set. */
 int gbkhip_batch_info(gbkhip_batch* batch, int* info);
 int gbkhip_batch_set_timing(gbkhip_batch* batch, int enable);
 int gbkhip_batch_stage_times(gbkhip_batch* batch, double* ms, int* chunks);
 /* Per-kernel times of the last timed evaluate(), summed over its chunks:
  * *n = 3 on the padded-cube path (smdisk, FFT convolution, downscale +
- * chi^2) and 7 on the fused path (lines, deposit + z FFT, y FFT, x FFT
+ * chi^2), 7 on the fused FFT path (lines, deposit + z FFT, y FFT, x FFT
  * convolution, inverse y FFT, inverse z FFT + downscale + chi^2, chi^2
- * final).  cap = length of ms. */
+ * final) and 5 on the direct path (lines, x pass, y pass, z pass + chi^2,
+ * chi^2 final).  cap = length of ms. */
 int gbkhip_batch_kernel_times(gbkhip_batch* batch, double* ms, int cap,
                               int* n);
 

@@ -79,29 +79,75 @@ def test_composition_independence(setup):
         assert alone[0] == full[i]
 
 
-def test_fused_pipeline_selected(setup):
-    """Thin disk + separable PSF x LSF: the spectral-first pipeline runs on
-    the smallest 5-smooth sizes >= S + 2 edge (256 + 40, 256 + 40, 96 + 14)."""
-    _, ev, _, _, _ = setup
+def test_pipeline_selection(setup):
+    """Thin disk + separable PSF x LSF: automatic selection takes the direct
+    separable pipeline (fewer multiply-adds than the FFTs at this kernel
+    width); the fused FFT one runs on the smallest 5-smooth sizes >= S +
+    2 edge (256 + 40, 256 + 40, 96 + 14)."""
+    from gbkfit_amd.batch import BatchEvaluator
+    drv, ev, _, mock, model = setup
     info = ev.info()
-    assert info['fused']
-    assert info['fft_size'] == (300, 300, 120)
+    assert info['pipeline'] == 'direct'
+    assert info['fft_size'] == (296, 296, 110)
+    fused = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock,
+                           error=0.01, max_batch=2, path='fused')
+    assert fused.info()['pipeline'] == 'fused'
+    assert fused.info()['fft_size'] == (300, 300, 120)
+    fused.close()
 
 
-def test_fused_matches_padded_path(setup):
-    """Spectral-first fused pipeline vs FFT of the whole padded cube."""
+@pytest.mark.parametrize('path', ['fused', 'padded'])
+def test_pipelines_agree(setup, path):
+    """Direct separable pipeline vs the fused FFT one and vs the FFT of the
+    whole padded cube: chi^2 within 1e-5 relative, cubes 1e-5 normwise."""
     from gbkfit_amd.batch import BatchEvaluator
     drv, ev, sets, mock, model = setup
     ref = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock,
-                         error=0.01, max_batch=4, path='padded')
-    assert not ref.info()['fused']
+                         error=0.01, max_batch=4, path=path)
+    assert ref.info()['pipeline'] == path
     params = [P.gparams(p) for p in sets]
-    c_f, cubes_f = ev.evaluate(params, want_cubes=True, use_graph=False)
+    c_d, cubes_d = ev.evaluate(params, want_cubes=True, use_graph=False)
     c_p, cubes_p = ref.evaluate(params, want_cubes=True, use_graph=False)
     for i in range(len(sets)):
-        assert abs(c_f[i] - c_p[i]) <= 1e-5 * c_p[i], (i, c_f[i], c_p[i])
-        assert H.normwise(cubes_f[i], cubes_p[i]) < 1e-5, i
+        assert abs(c_d[i] - c_p[i]) <= 1e-5 * c_p[i], (i, c_d[i], c_p[i])
+        assert H.normwise(cubes_d[i], cubes_p[i]) < 1e-5, i
     ref.close()
+
+
+@pytest.mark.parametrize('scale,size,psf,lsf', [
+    ((1, 1, 1), (48, 40, 32), 2.0, 25.0),
+    ((3, 2, 2), (24, 30, 20), 2.5, 30.0),
+    ((2, 3, 3), (20, 18, 16), 1.5, 60.0),
+])
+def test_direct_pipeline_other_shapes(scale, size, psf, lsf):
+    """Direct pipeline at other downscale factors and tap buckets: model
+    cubes against the FP64 oracle (1e-5 normwise + ambiguity bound) and
+    chi^2 against the fused FFT pipeline."""
+    from gbkfit_amd.batch import BatchEvaluator
+    drv = H.driver()
+    p = dict(P.c1_params(), size=size, scale=scale, psf_fwhm=psf,
+             lsf_fwhm=lsf)
+    model, _ = P.build_product(drv, p)
+    ref, amb, _ = P.oracle_cube(p)
+    rng = np.random.default_rng(11)
+    mock = (ref + rng.normal(0, 0.01, ref.shape)).astype(np.float32)
+    ev = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock,
+                        error=0.01, max_batch=3, path='direct')
+    assert ev.info()['pipeline'] == 'direct'
+    sets = [dict(p, incl=i, posa=a) for i, a in ((50, 30), (65, 140),
+                                                  (80, 280))]
+    chi2, cubes = ev.evaluate([P.gparams(s) for s in sets], want_cubes=True,
+                              use_graph=False)
+    for i, s in enumerate(sets):
+        r, a, _ = P.oracle_cube(s)
+        assert P.excess_error(cubes[i], r, a) < 1e-5, i
+    fu = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock,
+                        error=0.01, max_batch=3, path='fused')
+    c_f = fu.evaluate([P.gparams(s) for s in sets], use_graph=False)
+    for i in range(len(sets)):
+        assert abs(chi2[i] - c_f[i]) <= 1e-5 * c_f[i], (i, chi2[i], c_f[i])
+    fu.close()
+    ev.close()
 
 
 def test_thick_disk_uses_padded_path():
@@ -116,7 +162,7 @@ def test_thick_disk_uses_padded_path():
     mock = (ref + rng.normal(0, 0.01, ref.shape)).astype(np.float32)
     ev = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock,
                         error=0.01, max_batch=2)
-    assert not ev.info()['fused']
+    assert ev.info()['pipeline'] == 'padded'
     sets = [P.gparams(dict(p, incl=i, posa=a)) for i, a in
             ((40, 10), (60, 200), (75, 300))]
     chi2 = ev.evaluate(sets, use_graph=False)
