@@ -526,10 +526,19 @@ __global__ __launch_bounds__(T) void fz_c2r_down_chi2(
 // ---------------------------------------------------------------------------
 constexpr int kSepThreads = 256;
 constexpr int kSepZC = 32;          // most channels per S1 block
-constexpr int kSepJX = 4;           // outputs per thread, S1
-constexpr int kSepJY = 4;           // outputs per thread, S2
 constexpr int kSepJZ = 16;          // outputs per thread, S3
+constexpr int kSepXW = 32;          // X columns per S2 block
+constexpr int kSepYPitch = kSepXW + 1;
 
+// outputs per thread of S1 and S2 (fewer for wide taps: registers)
+__host__ __device__ constexpr int sep_jx(int taps) { return taps <= 64 ? 8 : 4; }
+__host__ __device__ constexpr int sep_jy(int taps) { return taps <= 64 ? 16 : 4; }
+
+// S1: block = (row y, channels [z0, z0 + zn), set).  The hi-res values of
+// the row are deposited into an LDS tile [zc][mp] (zero outside each
+// spaxel's rint window, gmodels.hpp:91-116); a thread then filters J
+// consecutive X outputs of one channel.  Lanes run over channels and mp is
+// odd, so the lanes of a wave read distinct banks.
 template <int SX, int TX>
 __global__ __launch_bounds__(256) void sep_xpass(
     const float4* __restrict__ lines, long long lstride, int lpitch,
@@ -537,39 +546,37 @@ __global__ __launch_bounds__(256) void sep_xpass(
     int nxl, int mp, int zc, float* __restrict__ P1, long long p1stride,
     int ny1)
 {
-    extern __shared__ float M[];          // [zc][mp]
+    extern __shared__ float4 M4[];        // [zc][mp] floats
+    float* M = reinterpret_cast<float*>(M4);
     __shared__ float s_w[TX];
-    constexpr int T = kSepThreads, J = kSepJX;
+    constexpr int T = kSepThreads, J = sep_jx(TX);
     const int y = blockIdx.x, z0 = blockIdx.y * zc, set = blockIdx.z;
+    const int zn = min(zc, need_z - z0);
     for (int i = threadIdx.x; i < TX; i += T) s_w[i] = wx[i];
-    // hi-res model values of row y, channels [z0, z0 + ZC): the spectral
-    // line of each spaxel inside its rint window (gmodels.hpp:91-116)
+    const int n4 = (zc * mp) >> 2;
+    for (int i = threadIdx.x; i < n4; i += T)
+        M4[i] = make_float4(0, 0, 0, 0);
+    for (int i = (n4 << 2) + threadIdx.x; i < zc * mp; i += T) M[i] = 0.0f;
+    __syncthreads();
     const float4* lrow = lines + set * lstride + (size_t)y * lpitch;
-    for (int x = threadIdx.x; x < mp; x += T) {
-        float4 V = make_float4(0, 0, 1, 0);
-        if (x < need_x) V = lrow[x];
-        int k0 = 1, k1 = 0;
-        LineEval e;
+    for (int x = threadIdx.x; x < need_x; x += T) {
+        const float4 V = lrow[x];
         if (V.x != 0.0f) {
+            int k0, k1;
             line_window(g, V.y, V.z, k0, k1);
-            k1 = min(k1, need_z - 1);
-            e = line_eval(g, V.x, V.y, V.z);
-        }
-#pragma unroll 4
-        for (int zz = 0; zz < zc; ++zz) {
-            const int k = z0 + zz;
-            M[zz * mp + x] = (k >= k0 && k <= k1) ? e.at(k) : 0.0f;
+            k0 = max(k0, z0);
+            k1 = min(k1, z0 + zn - 1);
+            const LineEval e = line_eval(g, V.x, V.y, V.z);
+            for (int k = k0; k <= k1; ++k) M[(k - z0) * mp + x] = e.at(k);
         }
     }
     __syncthreads();
     // outputs (z, X .. X + J - 1): inputs sx X + [0, sx (J - 1) + TX)
     constexpr int NIN = SX * (J - 1) + TX;
     const int nxb = (nxl + J - 1) / J;
-    float* out = P1 + set * p1stride;
-    for (int task = threadIdx.x; task < zc * nxb; task += T) {
-        const int zz = task / nxb, xb = task - zz * nxb;
-        const int z = z0 + zz;
-        if (z >= need_z) continue;
+    float* out = P1 + set * p1stride + (size_t)y * nxl;
+    for (int task = threadIdx.x; task < zn * nxb; task += T) {
+        const int xb = task / zn, zz = task - xb * zn;
         const int X0 = xb * J;
         const float* in = M + zz * mp + SX * X0;
         float v[NIN];
@@ -582,54 +589,108 @@ __global__ __launch_bounds__(256) void sep_xpass(
 #pragma unroll
             for (int j = 0; j < J; ++j) o[j] = fmaf(w, v[SX * j + u], o[j]);
         }
-        float* dst = out + ((size_t)z * ny1 + y) * nxl + X0;
+        float* dst = out + (size_t)(z0 + zz) * ny1 * nxl + X0;
+        if (X0 + J <= nxl) {
 #pragma unroll
-        for (int j = 0; j < J; ++j)
-            if (X0 + j < nxl) dst[j] = o[j];
+            for (int j = 0; j < J; j += 4)
+                *reinterpret_cast<float4*>(dst + j) =
+                    make_float4(o[j], o[j + 1], o[j + 2], o[j + 3]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < J; ++j)
+                if (X0 + j < nxl) dst[j] = o[j];
+        }
     }
 }
 
-// P2[z][Y][X] = sum_v Wy[v] P1[z][sy Y + v][X]: a block covers 64 X and
-// 4 J Y outputs of one channel, its input rows staged in LDS.
+// S2: P2[z][Y][X] = sum_v Wy[v] P1[z][sy Y + v][X].  Block = (32 X
+// columns, Y outputs [Y0, Y0 + yblk), channel, set): the rows they read are
+// staged once in LDS (pitch 33); a thread filters J consecutive Y of one
+// column.
 template <int SY, int TY>
 __global__ __launch_bounds__(256) void sep_ypass(
     const float* __restrict__ P1, long long p1stride, const float* wy,
-    int nxl, int nyl, int ny1, int need_z, float* __restrict__ P2,
-    long long p2stride)
+    int nxl, int nyl, int ny1, int need_z, int yblk,
+    float* __restrict__ P2, long long p2stride)
 {
-    constexpr int T = kSepThreads, J = kSepJY, TXB = 64;
-    constexpr int YB = (T / TXB) * J;                 // Y outputs per block
-    constexpr int ROWS = SY * (YB - 1) + TY;
-    __shared__ float tile[ROWS * TXB];
+    constexpr int T = kSepThreads, J = sep_jy(TY), XW = kSepXW;
+    constexpr int PITCH = kSepYPitch, G = T / XW;
+    constexpr int NIN = SY * (J - 1) + TY;
+    extern __shared__ float tile[];       // [rows][PITCH]
     __shared__ float s_w[TY];
-    const int X0 = blockIdx.x * TXB, Y0 = blockIdx.y * YB;
+    const int X0 = blockIdx.x * XW, Y0 = blockIdx.y * yblk;
     const int z = blockIdx.z % need_z, set = blockIdx.z / need_z;
+    const int ngrp = (min(yblk, nyl - Y0) + J - 1) / J;
+    const int nrows = SY * (ngrp * J - 1) + TY;
     for (int i = threadIdx.x; i < TY; i += T) s_w[i] = wy[i];
     const float* src = P1 + set * p1stride + (size_t)z * ny1 * nxl;
-    for (int i = threadIdx.x; i < ROWS * TXB; i += T) {
-        const int r = i / TXB, c = i - r * TXB;
-        const int y = SY * Y0 + r, X = X0 + c;
-        tile[i] = (y < ny1 && X < nxl) ? src[(size_t)y * nxl + X] : 0.0f;
+    // staging: U independent loads in flight per thread before the stores
+    constexpr int U = 8;
+    if ((nxl & 3) == 0 && X0 + XW <= nxl) {
+        // whole strip, rows 16-byte aligned: float4 loads
+        constexpr int Q = XW / 4;
+        const int n4 = nrows * Q;
+        for (int i0 = threadIdx.x; i0 < n4; i0 += T * U) {
+            float4 t[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + u * T, r = i / Q, q = i - r * Q;
+                const int y = SY * Y0 + r;
+                t[u] = (i < n4 && y < ny1)
+                           ? *reinterpret_cast<const float4*>(
+                                 src + (size_t)y * nxl + X0 + 4 * q)
+                           : make_float4(0, 0, 0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + u * T, r = i / Q, q = i - r * Q;
+                if (i < n4) {
+                    float* d = tile + r * PITCH + 4 * q;
+                    d[0] = t[u].x;
+                    d[1] = t[u].y;
+                    d[2] = t[u].z;
+                    d[3] = t[u].w;
+                }
+            }
+        }
+    } else {
+        const int n1 = nrows * XW;
+        for (int i0 = threadIdx.x; i0 < n1; i0 += T * U) {
+            float t[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + u * T, r = i / XW, c = i - r * XW;
+                const int y = SY * Y0 + r, X = X0 + c;
+                t[u] = (i < n1 && y < ny1 && X < nxl)
+                           ? src[(size_t)y * nxl + X] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + u * T, r = i / XW, c = i - r * XW;
+                if (i < n1) tile[r * PITCH + c] = t[u];
+            }
+        }
     }
     __syncthreads();
-    const int c = threadIdx.x % TXB, yb = threadIdx.x / TXB;
-    constexpr int NIN = SY * (J - 1) + TY;
-    float v[NIN];
+    const int c = threadIdx.x % XW, X = X0 + c;
+    float* dst = P2 + set * p2stride + (size_t)z * nyl * nxl + X;
+    for (int gi = threadIdx.x / XW; gi < ngrp; gi += G) {
+        const float* in = tile + SY * J * gi * PITCH + c;
+        float v[NIN];
 #pragma unroll
-    for (int i = 0; i < NIN; ++i) v[i] = tile[(SY * J * yb + i) * TXB + c];
-    float o[J] = {};
+        for (int i = 0; i < NIN; ++i) v[i] = in[i * PITCH];
+        float o[J] = {};
 #pragma unroll
-    for (int u = 0; u < TY; ++u) {
-        const float w = s_w[u];
+        for (int u = 0; u < TY; ++u) {
+            const float w = s_w[u];
 #pragma unroll
-        for (int j = 0; j < J; ++j) o[j] = fmaf(w, v[SY * j + u], o[j]);
-    }
-    const int X = X0 + c;
-    float* dst = P2 + set * p2stride + (size_t)z * nyl * nxl;
+            for (int j = 0; j < J; ++j) o[j] = fmaf(w, v[SY * j + u], o[j]);
+        }
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-        const int Y = Y0 + yb * J + j;
-        if (X < nxl && Y < nyl) dst[(size_t)Y * nxl + X] = o[j];
+        for (int j = 0; j < J; ++j) {
+            const int Y = Y0 + gi * J + j;
+            if (X < nxl && Y < nyl) dst[(size_t)Y * nxl] = o[j];
+        }
     }
 }
 
@@ -699,8 +760,9 @@ struct Sep {
     bool on = false;
     int need[3] = {0, 0, 0};      // inputs that reach the outputs
     int tx = 0, ty = 0, tz = 0;   // tap buckets (zero-padded)
-    int mp = 0;                   // S1 LDS row length
+    int mp = 0;                   // S1 LDS row length (odd)
     int zc = 0;                   // channels per S1 block (tile <= 64 KB)
+    int yblk = 0;                 // Y outputs per S2 block
     DevBuf wx, wy, wz;            // taps
     DevBuf P1, P2;                // work: per set (z, y, X) and (z, Y, X)
     long long p1stride = 0, p2stride = 0;
@@ -1248,10 +1310,20 @@ static bool plan_sep(Batch& B)
     up(P.wy, wy);
     up(P.wz, wz);
     const int nxl = D.size_lo[0], nyl = D.size_lo[1], nzl = D.size_lo[2];
-    const int nxb = (nxl + kSepJX - 1) / kSepJX;
-    P.mp = sx * nxb * kSepJX + P.tx;
-    P.zc = std::min(kSepZC, (int)((64 * 1024) / (P.mp * sizeof(float))));
-    if (P.zc < 1) return false;
+    const int jx = sep_jx(P.tx), jy = sep_jy(P.ty);
+    const int nxb = (nxl + jx - 1) / jx;
+    P.mp = (sx * nxb * jx + P.tx) | 1;
+    // channels per S1 block: at most 32 in a tile of at most 64 KB, spread
+    // evenly over the chunks
+    const int zmax = std::min(kSepZC, (int)((64 * 1024) / (P.mp * 4)));
+    if (zmax < 1) return false;
+    const int nchunks = (P.need[2] + zmax - 1) / zmax;
+    P.zc = (P.need[2] + nchunks - 1) / nchunks;
+    // Y outputs per S2 block: all of them when the rows fit 48 KB
+    const int max_rows = (48 * 1024) / (kSepYPitch * 4);
+    const int gmax = ((max_rows - P.ty) / sy + 1) / jy;
+    if (gmax < 1) return false;
+    P.yblk = std::min((nyl + jy - 1) / jy, gmax) * jy;
     P.p1stride = (long long)P.need[2] * P.need[1] * nxl;
     P.p2stride = (long long)P.need[2] * nyl * nxl;
     P.zblocks = (nzl + kSepJZ - 1) / kSepJZ;
@@ -1341,11 +1413,12 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
     }
     mark(b, ci, 2);
     {
-        constexpr int YB = (kSepThreads / 64) * kSepJY;
-        dim3 grid(ceil_div(nxl, 64), ceil_div(nyl, YB), nz * nsets);
+        dim3 grid(ceil_div(nxl, kSepXW), ceil_div(nyl, P.yblk), nz * nsets);
+        const int rows = D.scale[1] * (P.yblk - 1) + P.ty;
+        const size_t lds = (size_t)rows * kSepYPitch * sizeof(float);
         GBK_SEP_DISPATCH(sep_ypass, D.scale[1], P.ty, grid, dim3(kSepThreads),
-                         0, c.stream, P.P1.as<float>(), P.p1stride,
-                         P.wy.as<float>(), nxl, nyl, ny, nz,
+                         lds, c.stream, P.P1.as<float>(), P.p1stride,
+                         P.wy.as<float>(), nxl, nyl, ny, nz, P.yblk,
                          P.P2.as<float>(), P.p2stride);
         GBK_HIP(hipGetLastError());
     }
