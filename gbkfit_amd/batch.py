@@ -2,11 +2,13 @@
 
 The reference evaluates one parameter set per likelihood call (its fitters
 call Objective.log_likelihood per walker, src/gbkfit/fitting/fitters/emcee/
-sampler.py:34-57).  ``BatchEvaluator`` evaluates N sets of one smdisk model
-(DModelSCube + Kinematics2D/3D with one SpectralSMDisk component) against
-one observed cube in a single native call (gbkhip_batch_evaluate): batched
-smdisk, batched FFT convolution against the shared PSF x LSF spectrum, and a
-fused downscale + residual + FP64 chi^2 pass, captured in a HIP graph.
+sampler.py:34-57).  ``BatchEvaluator`` evaluates N sets of one disk model
+(DModelSCube + Kinematics2D/3D with one SpectralSMDisk or SpectralMCDisk
+component) against one observed cube in a single native call
+(gbkhip_batch_evaluate): the batched model, the convolution with the shared
+PSF x LSF kernel (direct separable filters, a spectral-first fused FFT, or an
+FFT of the padded cube) and a fused downscale + residual + FP64 chi^2 pass,
+captured in a HIP graph.
 """
 import ctypes
 
@@ -38,7 +40,10 @@ class BDesc(_c.Structure):
                 ('pcube', _c.c_void_p), ('obs', _c.c_void_p),
                 ('err', _c.c_void_p), ('mask', _c.c_void_p),
                 ('err_scalar', _c.c_float), ('weight', _c.c_float),
-                ('max_batch', _c.c_int), ('path', _c.c_int)]
+                ('max_batch', _c.c_int), ('path', _c.c_int),
+                ('mcdisk', _c.c_int), ('cflux', _c.c_float),
+                ('csum_len', _c.c_int), ('hoi', _c.c_void_p),
+                ('seed', _c.c_ulonglong)]
 
 
 N.register_signatures({
@@ -47,6 +52,11 @@ N.register_signatures({
     'gbkhip_batch_destroy': ([_c.c_void_p], _c.c_int),
     'gbkhip_batch_evaluate': ([_c.c_void_p, _c.c_void_p, _c.c_int,
                                _c.c_void_p, _c.c_void_p, _c.c_int], _c.c_int),
+    'gbkhip_batch_evaluate_clouds': ([_c.c_void_p, _c.c_void_p, _c.c_void_p,
+                                      _c.c_void_p, _c.c_int, _c.c_void_p,
+                                      _c.c_void_p, _c.c_int], _c.c_int),
+    'gbkhip_mcdisk_fixed_scale': ([_c.c_float, _c.c_int, _c.c_void_p,
+                                   _c.POINTER(_c.c_double)], _c.c_int),
     'gbkhip_batch_info': ([_c.c_void_p, _c.POINTER(_c.c_int)], _c.c_int),
     'gbkhip_batch_kernel_times': ([_c.c_void_p, _c.POINTER(_c.c_double),
                                    _c.c_int, _c.POINTER(_c.c_int)], _c.c_int),
@@ -153,10 +163,13 @@ class SetPacker:
 
 
 class BatchEvaluator:
-    """chi^2 of many parameter sets of one smdisk model against one cube.
+    """chi^2 of many parameter sets of one disk model against one cube.
 
     dmodel: DModelSCube (its DCube geometry, PSF and LSF are used);
-    gmodel: GModelKinematics2D/3D with exactly one SpectralSMDisk component;
+    gmodel: GModelKinematics2D/3D with exactly one SpectralSMDisk or
+            SpectralMCDisk component (mcdisk: padded-cube pipeline, each
+            set's cloud budget computed on the host as the reference does,
+            src/gbkfit/model/gmodels/_mcdisk.py:16-189);
     obs:    observed low-res cube (numpy, z-y-x); error: scalar or cube.
     """
 
@@ -166,9 +179,11 @@ class BatchEvaluator:
         self.driver = driver
         comps = gmodel.components
         if len(comps) != 1 or not hasattr(comps[0].disk, 'traits') or \
-                type(comps[0].disk).__name__ != 'SMDisk':
-            raise RuntimeError("BatchEvaluator needs one smdisk component")
+                type(comps[0].disk).__name__ not in ('SMDisk', 'MCDisk'):
+            raise RuntimeError(
+                "BatchEvaluator needs one smdisk or mcdisk component")
         disk = comps[0].disk
+        self.mc = type(disk).__name__ == 'MCDisk'
         cube = dmodel.dcube()
         be_fft = driver.backends().fft(cube.dtype(), cube._padding)
         size_hi, step_hi, edge_hi, zero_hi = cube.geometry(be_fft)
@@ -238,6 +253,17 @@ class BatchEvaluator:
         if path not in paths:
             raise ValueError(f"path must be one of {sorted(paths)}")
         d.path = paths[path]
+        self._csum_dev = self._qscale_dev = None
+        if self.mc:
+            csum0, hoi = disk.cloud_budget(template_params)
+            self._hoi = driver.mem_copy_h2d(np.ascontiguousarray(hoi,
+                                                                 np.uint8))
+            d.mcdisk = 1
+            d.cflux = disk.cflux
+            d.csum_len = int(csum0.size)
+            d.hoi = self._hoi.data_ptr()
+            d.seed = int(driver.backends().gmodel(np.float32).seed)
+            self.disk = disk
         self.desc = d
         self.size_lo = tuple(cube.size())
         self.size_hi = tuple(size_hi)
@@ -253,9 +279,31 @@ class BatchEvaluator:
         del t, dev
 
     def upload(self, params_list):
-        """Packs parameter dicts into the persistent device buffer."""
+        """Packs parameter dicts into the persistent device buffer (mcdisk:
+        with each set's cloud budget and fixed-point scale)."""
         host = self.packer.pack(params_list)
+        if self.mc:
+            self._upload_clouds(params_list)
         return self.upload_packed(host)
+
+    def _upload_clouds(self, params_list):
+        t = driver_torch()
+        n = len(params_list)
+        csum = np.zeros((n, self.desc.csum_len), np.int32)
+        qs = np.zeros(n, np.float64)
+        out = _c.c_double()
+        for i, p in enumerate(params_list):
+            c, _ = self.disk.cloud_budget(p)
+            csum[i] = c
+            N.call('gbkhip_mcdisk_fixed_scale', _c.c_float(self.desc.cflux),
+                   int(c[-1]), _c.byref(self.grid), _c.byref(out))
+            qs[i] = out.value
+        dev = self.driver.device
+        if self._csum_dev is None or self._csum_dev.shape != csum.shape:
+            self._csum_dev = t.empty(csum.shape, dtype=t.int32, device=dev)
+            self._qscale_dev = t.empty(n, dtype=t.float64, device=dev)
+        self._csum_dev.copy_(t.from_numpy(csum))
+        self._qscale_dev.copy_(t.from_numpy(qs))
 
     def upload_packed(self, host):
         t = driver_torch()
@@ -283,11 +331,16 @@ class BatchEvaluator:
         # context's stream")
         self.driver.context()
         self.set_stream(t.cuda.current_stream())
+        cubes = self._cubes_dev.data_ptr() if want_cubes else None
+        if self.mc:
+            N.call('gbkhip_batch_evaluate_clouds', self.handle,
+                   self._params_dev.data_ptr(), self._csum_dev.data_ptr(),
+                   self._qscale_dev.data_ptr(), int(nsets),
+                   self._chi2_dev.data_ptr(), cubes, int(bool(use_graph)))
+            return
         N.call('gbkhip_batch_evaluate', self.handle,
                self._params_dev.data_ptr(), int(nsets),
-               self._chi2_dev.data_ptr(),
-               self._cubes_dev.data_ptr() if want_cubes else None,
-               int(bool(use_graph)))
+               self._chi2_dev.data_ptr(), cubes, int(bool(use_graph)))
 
     def evaluate(self, params_list, want_cubes=False, use_graph=True):
         """chi^2 per set (numpy FP64), and the model cubes if asked."""

@@ -17,6 +17,7 @@
 #include "fft_engine.hpp"
 #include "launch.hpp"
 #include "disk.hpp"
+#include "mcdisk.hpp"
 
 #include <climits>
 #include <cmath>
@@ -228,12 +229,15 @@ __global__ void batch_chi2_final(const double* __restrict__ partials,
 // size) reach no stored output through a kernel of half-width edge_i, so
 // they are zero and the FFT sizes are the smallest 5-smooth ones above it.
 // ---------------------------------------------------------------------------
-// Line of sight of each spaxel (thin disk): (flux, v, sigma) of the line it
-// deposits, flux 0 when it deposits none or lies beyond the needed region.
+// Line of sight of each spaxel (thin disk): the line it deposits, flux 0
+// when it deposits none or lies beyond the needed region.  Form 0: (flux, v,
+// sigma, 0).  Form 1, ready for evaluation: (amp, v, 1 / (2 sigma^2),
+// k0 | k1 << 16) with the LineEval amplitude and the rint channel window
+// (amp 0 and an empty window when there is no line).
 __global__ __launch_bounds__(256) void batch_lines_thin(
     gbkhip_disk tmpl, SetLayout lay, const float* __restrict__ params,
     int pstride, gbkhip_grid g, int need_x, int need_y, int nx, int ny,
-    float4* __restrict__ lines, long long lstride)
+    float4* __restrict__ lines, long long lstride, int form)
 {
     const int x = blockIdx.x * 64 + threadIdx.x;
     const int y = blockIdx.y * 4 + threadIdx.y;
@@ -247,6 +251,23 @@ __global__ __launch_bounds__(256) void batch_lines_thin(
         const NodeGeom ng = node_geom(d, 0);
         const VoxelLine V = smdisk_voxel(d, g, ng, x, y, 0, need_theta, false);
         if (V.on) out = make_float4(V.orvalue, V.v, V.s, 0);
+    }
+    if (form == 1) {
+        int k0 = 1, k1 = 0;
+        float amp = 0.0f, inv2s2 = 0.0f;
+        if (out.x != 0.0f) {
+            line_window(g, out.y, out.z, k0, k1);
+            const LineEval e = line_eval(g, out.x, out.y, out.z);
+            amp = e.amp;
+            inv2s2 = e.inv2s2;
+            if (k0 > k1) {          // empty window
+                k0 = 1;
+                k1 = 0;
+            }
+        }
+        out = make_float4(amp, out.y, inv2s2,
+                          __uint_as_float((unsigned)k0 |
+                                          ((unsigned)k1 << 16)));
     }
     lines[set * lstride + (size_t)y * nx + x] = out;
 }
@@ -508,6 +529,63 @@ __global__ __launch_bounds__(T) void fz_c2r_down_chi2(
     }
 }
 
+// Batched mcdisk: blockIdx.y = set.  Each set builds its disk from its
+// parameter vector and runs the clouds of its own budget into its own int64
+// cube, with its own fixed-point scale; the per-cloud code is the plain
+// kernel's (mcdisk.hpp), so a set's cube equals a one-model evaluation.
+__global__ __launch_bounds__(256) void batch_mcdisk(
+    gbkhip_disk tmpl, SetLayout lay, const float* __restrict__ params,
+    int pstride, gbkhip_grid g, float cflux, const int* __restrict__ csum,
+    int csum_len, const unsigned char* hoi, uint32_t key0, uint32_t key1,
+    const double* __restrict__ qscale, unsigned long long* icube,
+    long long n_hi)
+{
+    const int set = blockIdx.y;
+    const gbkhip_disk d = set_disk(tmpl, lay, params + (size_t)set * pstride);
+    McParams mp;
+    mp.cflux = cflux;
+    mp.csum = csum + (size_t)set * csum_len;
+    mp.csum_len = csum_len;
+    mp.nclouds = mp.csum[csum_len - 1];
+    mp.hoi = hoi;
+    mp.key0 = key0;
+    mp.key1 = key1;
+    mp.qscale = qscale[set];
+    gbkhip_outputs o;
+    memset(&o, 0, sizeof(o));
+    unsigned long long* ic = icube + (size_t)set * n_hi;
+    for (long long ci = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+         ci < mp.nclouds; ci += (long long)gridDim.x * blockDim.x)
+        mc_cloud(ci, mp, d, g, o, ic);
+}
+
+// hi = icube / 2^S per set (0 where nothing landed), icube <- 0.
+__global__ void batch_fixed_to_hi(unsigned long long* icube, float* hi,
+                                  long long n, const double* qscale)
+{
+    const int set = blockIdx.y;
+    const double inv = 1.0 / qscale[set];
+    unsigned long long* ic = icube + (size_t)set * n;
+    float* h = hi + (size_t)set * n;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long q = (long long)ic[i];
+        float v = 0.0f;
+        if (q) {
+            v = (float)((double)q * inv);
+            ic[i] = 0ull;
+        }
+        h[i] = v;
+    }
+}
+
+__global__ void batch_zero_u64(unsigned long long* p, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
 // ---------------------------------------------------------------------------
 // Direct separable pipeline (thin disk, separable PSF x LSF kernel)
 //
@@ -525,7 +603,7 @@ __global__ __launch_bounds__(T) void fz_c2r_down_chi2(
 // loops are register-blocked with constant offsets.
 // ---------------------------------------------------------------------------
 constexpr int kSepThreads = 256;
-constexpr int kSepZC = 32;          // most channels per S1 block
+constexpr int kSepZC = 16;          // most channels per S1 block
 constexpr int kSepJZ = 16;          // outputs per thread, S3
 constexpr int kSepXW = 32;          // X columns per S2 block
 constexpr int kSepYPitch = kSepXW + 1;
@@ -533,6 +611,13 @@ constexpr int kSepYPitch = kSepXW + 1;
 // outputs per thread of S1 and S2 (fewer for wide taps: registers)
 __host__ __device__ constexpr int sep_jx(int taps) { return taps <= 64 ? 8 : 4; }
 __host__ __device__ constexpr int sep_jy(int taps) { return taps <= 64 ? 16 : 4; }
+// smallest d with stride d = 16 (mod 64) floats (stride a multiple of 4)
+__host__ __device__ constexpr int sep_dxb(int stride)
+{
+    int d = 1;
+    while ((stride * d) % 64 != 16) ++d;
+    return d;
+}
 
 // S1: block = (row y, channels [z0, z0 + zn), set).  The hi-res values of
 // the row are deposited into an LDS tile [zc][mp] (zero outside each
@@ -558,25 +643,38 @@ __global__ __launch_bounds__(256) void sep_xpass(
         M4[i] = make_float4(0, 0, 0, 0);
     for (int i = (n4 << 2) + threadIdx.x; i < zc * mp; i += T) M[i] = 0.0f;
     __syncthreads();
+    // lines in form 1 (batch_lines_thin): amplitude and window precomputed
     const float4* lrow = lines + set * lstride + (size_t)y * lpitch;
     for (int x = threadIdx.x; x < need_x; x += T) {
         const float4 V = lrow[x];
         if (V.x != 0.0f) {
-            int k0, k1;
-            line_window(g, V.y, V.z, k0, k1);
-            k0 = max(k0, z0);
-            k1 = min(k1, z0 + zn - 1);
-            const LineEval e = line_eval(g, V.x, V.y, V.z);
+            const unsigned kw = __float_as_uint(V.w);
+            const int k0 = max((int)(kw & 0xffffu), z0);
+            const int k1 = min((int)(kw >> 16), z0 + zn - 1);
+            LineEval e;
+            e.amp = V.x;
+            e.v = V.y;
+            e.inv2s2 = V.z;
+            e.zero = g.spec_zero;
+            e.step = g.spec_step;
             for (int k = k0; k <= k1; ++k) M[(k - z0) * mp + x] = e.at(k);
         }
     }
     __syncthreads();
-    // outputs (z, X .. X + J - 1): inputs sx X + [0, sx (J - 1) + TX)
+    // outputs (z, X .. X + J - 1): inputs sx X + [0, sx (J - 1) + TX).
+    // Lane l of a wave takes channel l % 16; its four 16-lane groups take
+    // output blocks DXB apart, whose inputs start 16 banks apart (SX J DXB
+    // = 16 mod 64), and the 16 channels of a group, mp (odd) apart, hit 16
+    // distinct banks: no bank conflicts with 32 or 64 banks.
     constexpr int NIN = SX * (J - 1) + TX;
+    constexpr int DXB = sep_dxb(SX * J);
     const int nxb = (nxl + J - 1) / J;
+    const int lane = threadIdx.x & 63, zz = lane & 15, grp = lane >> 4;
     float* out = P1 + set * p1stride + (size_t)y * nxl;
-    for (int task = threadIdx.x; task < zn * nxb; task += T) {
-        const int xb = task / zn, zz = task - xb * zn;
+    for (int w = threadIdx.x >> 6;; w += T / 64) {
+        const int g = w / DXB, xb = g * 4 * DXB + w % DXB + grp * DXB;
+        if (g * 4 * DXB >= nxb) break;           // uniform over the wave
+        if (zz >= zn || xb >= nxb) continue;
         const int X0 = xb * J;
         const float* in = M + zz * mp + SX * X0;
         float v[NIN];
@@ -810,15 +908,25 @@ struct Batch {
     DevBuf hi;              // chunk x padded cube
     DevBuf spec;            // chunk x complex work spectrum
     DevBuf partials;
+    DevBuf icube;           // mcdisk: chunk x int64 fixed-point cube
+    const int* csum = nullptr;       // mcdisk inputs of the current call
+    const double* qscale = nullptr;
     int nparts = 0;
     long long n_hi = 0, n_c = 0, n_lo = 0;
     bool thin = true;
     bool work = false;      // work buffers allocated
     hipStream_t stream = nullptr;   // evaluate() stream; null: the context's
     // graph cache: (params, nsets, chi2, cubes) -> executable graph
-    std::map<std::tuple<uintptr_t, int, uintptr_t, uintptr_t>,
+    std::map<std::tuple<uintptr_t, int, uintptr_t, uintptr_t, uintptr_t,
+                        uintptr_t>,
              hipGraphExec_t> graphs;
     hipStream_t cap_stream = nullptr;
+    // chunks in flight: with 2 lanes, odd chunks run on the aux stream with
+    // their own work buffers, so one chunk's compute-bound x pass overlaps
+    // the memory-bound y and z passes of the previous one
+    int lanes = 1;
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // stage timing: events around smdisk / convolution / chi^2 of every
     // chunk (recorded into graphs too), read back after the work completes
     bool timing = false;
@@ -829,6 +937,9 @@ struct Batch {
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
         for (auto e : events) (void)hipEventDestroy(e);
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
+        if (aux) (void)hipStreamDestroy(aux);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
     }
 };
 
@@ -893,7 +1004,7 @@ static void build_layout(Batch& b)
 // (layout, kernel spectra, plans, twiddles) stays.
 static size_t work_bytes(const Batch& B)
 {
-    const size_t mb = (size_t)B.desc.max_batch;
+    const size_t mb = (size_t)B.desc.max_batch * B.lanes;
     size_t n = mb * (size_t)B.nparts * sizeof(double);
     if (B.sep.on) {
         n += mb * (size_t)B.sep.need[1] * B.sep.need[0] * sizeof(float4);
@@ -904,6 +1015,7 @@ static size_t work_bytes(const Batch& B)
     } else {
         n += mb * (size_t)B.n_hi * sizeof(float);
         if (B.desc.has_conv) n += mb * (size_t)B.n_c * sizeof(float2);
+        if (B.desc.mcdisk) n += mb * (size_t)B.n_hi * sizeof(uint64_t);
     }
     return n;
 }
@@ -917,7 +1029,7 @@ static void drop_graphs(Batch& B)
 static void alloc_work(Batch& B)
 {
     if (B.work) return;
-    const size_t mb = (size_t)B.desc.max_batch;
+    const size_t mb = (size_t)B.desc.max_batch * B.lanes;
     B.partials.alloc(mb * B.nparts * sizeof(double));
     if (B.sep.on) {
         B.fused.lines.alloc(mb * (size_t)B.sep.need[1] * B.sep.need[0] *
@@ -930,6 +1042,15 @@ static void alloc_work(Batch& B)
     } else {
         B.hi.alloc(mb * B.n_hi * sizeof(float));
         if (B.desc.has_conv) B.spec.alloc(mb * B.n_c * sizeof(float2));
+        if (B.desc.mcdisk) {
+            // zeroed once; the conversion re-zeroes what it reads
+            B.icube.alloc(mb * B.n_hi * sizeof(uint64_t));
+            hipLaunchKernelGGL(batch_zero_u64, dim3(4096), dim3(256), 0,
+                               B.ctx->stream,
+                               B.icube.as<unsigned long long>(),
+                               (long long)(mb * B.n_hi));
+            GBK_HIP(hipGetLastError());
+        }
     }
     B.work = true;
 }
@@ -944,6 +1065,7 @@ static void release_work(Batch& B)
     B.sep.P2.release();
     B.hi.release();
     B.spec.release();
+    B.icube.release();
     B.work = false;
 }
 
@@ -1169,7 +1291,7 @@ static void run_chunk_fused(Batch& b, const float* params, int set0,
         dim3 g0(ceil_div(F.nx, 64), ceil_div(F.ny, 4), nsets);
         hipLaunchKernelGGL(batch_lines_thin, g0, dim3(64, 4), 0, c.stream,
                            b.tmpl, b.lay, P, D.pstride, D.grid, F.need[0],
-                           F.need[1], F.nx, F.ny, lines, F.lstride);
+                           F.need[1], F.nx, F.ny, lines, F.lstride, 0);
         GBK_HIP(hipGetLastError());
         mark(b, ci, 1);
         dim3 grid(ceil_div(F.nx, 1 << F.fz.lshift), F.ny, nsets);
@@ -1234,7 +1356,7 @@ static void run_chunk_fused(Batch& b, const float* params, int set0,
 
 static int tap_bucket(int n)
 {
-    for (int b : {8, 16, 24, 32, 48, 64, 96, 128})
+    for (int b : {8, 16, 24, 32, 44, 48, 64, 96, 128})
         if (n <= b) return b;
     return 0;
 }
@@ -1291,6 +1413,7 @@ static bool plan_sep(Batch& B)
     const int ex = D.edge_hi[0], ey = D.edge_hi[1], ez = D.edge_hi[2];
     const int sx = D.scale[0], sy = D.scale[1], sz = D.scale[2];
     if (sx < 1 || sx > 3 || sy < 1 || sy > 3 || sz < 1 || sz > 3) return false;
+    if (D.grid.spec_size > 65535) return false;    // 16-bit channel windows
     P.tx = tap_bucket(sx + 2 * ex);
     P.ty = tap_bucket(sy + 2 * ey);
     P.tz = tap_bucket(sz + 2 * ez);
@@ -1313,8 +1436,8 @@ static bool plan_sep(Batch& B)
     const int jx = sep_jx(P.tx), jy = sep_jy(P.ty);
     const int nxb = (nxl + jx - 1) / jx;
     P.mp = (sx * nxb * jx + P.tx) | 1;
-    // channels per S1 block: at most 32 in a tile of at most 64 KB, spread
-    // evenly over the chunks
+    // channels per S1 block: at most 16 (the lanes of a group) in a tile of
+    // at most 64 KB, spread evenly over the chunks
     const int zmax = std::min(kSepZC, (int)((64 * 1024) / (P.mp * 4)));
     if (zmax < 1) return false;
     const int nchunks = (P.need[2] + zmax - 1) / zmax;
@@ -1359,6 +1482,7 @@ static bool sep_cheaper(const Batch& B)
     GBK_SEP_CASE(KERNEL, 1, 16, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 1, 24, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 1, 32, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 1, 44, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 1, 48, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 1, 64, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 1, 96, 0, __VA_ARGS__)                              \
@@ -1367,6 +1491,7 @@ static bool sep_cheaper(const Batch& B)
     GBK_SEP_CASE(KERNEL, 2, 16, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 2, 24, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 2, 32, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 2, 44, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 2, 48, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 2, 64, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 2, 96, 0, __VA_ARGS__)                              \
@@ -1375,6 +1500,7 @@ static bool sep_cheaper(const Batch& B)
     GBK_SEP_CASE(KERNEL, 3, 16, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 3, 24, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 3, 32, 0, __VA_ARGS__)                              \
+    GBK_SEP_CASE(KERNEL, 3, 44, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 3, 48, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 3, 64, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 3, 96, 0, __VA_ARGS__)                              \
@@ -1384,7 +1510,8 @@ static bool sep_cheaper(const Batch& B)
     }
 
 static void run_chunk_sep(Batch& b, const float* params, int set0,
-                          int nsets, double* chi2, float* cubes, int ci)
+                          int nsets, double* chi2, float* cubes, int ci,
+                          int lane)
 {
     Context& c = *b.ctx;
     const gbkhip_batch_desc& D = b.desc;
@@ -1392,13 +1519,17 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
     const float* par = params + (size_t)set0 * D.pstride;
     const int nxl = D.size_lo[0], nyl = D.size_lo[1], nzl = D.size_lo[2];
     const int nx = P.need[0], ny = P.need[1], nz = P.need[2];
-    float4* lines = b.fused.lines.as<float4>();
+    const size_t mb = (size_t)D.max_batch * lane;    // sets before this lane
     const long long lstride = (long long)ny * nx;
+    float4* lines = b.fused.lines.as<float4>() + mb * lstride;
+    float* P1 = P.P1.as<float>() + mb * P.p1stride;
+    float* P2 = P.P2.as<float>() + mb * P.p2stride;
+    double* partials = b.partials.as<double>() + mb * b.nparts;
     {
         dim3 g0(ceil_div(nx, 64), ceil_div(ny, 4), nsets);
         hipLaunchKernelGGL(batch_lines_thin, g0, dim3(64, 4), 0, c.stream,
                            b.tmpl, b.lay, par, D.pstride, D.grid, nx, ny, nx,
-                           ny, lines, lstride);
+                           ny, lines, lstride, 1);
         GBK_HIP(hipGetLastError());
     }
     mark(b, ci, 1);
@@ -1408,7 +1539,7 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
         GBK_SEP_DISPATCH(sep_xpass, D.scale[0], P.tx, grid, dim3(kSepThreads),
                          lds, c.stream, lines, lstride, nx, D.grid,
                          P.wx.as<float>(), nx, nz, nxl, P.mp, P.zc,
-                         P.P1.as<float>(), P.p1stride, ny);
+                         P1, P.p1stride, ny);
         GBK_HIP(hipGetLastError());
     }
     mark(b, ci, 2);
@@ -1417,9 +1548,9 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
         const int rows = D.scale[1] * (P.yblk - 1) + P.ty;
         const size_t lds = (size_t)rows * kSepYPitch * sizeof(float);
         GBK_SEP_DISPATCH(sep_ypass, D.scale[1], P.ty, grid, dim3(kSepThreads),
-                         lds, c.stream, P.P1.as<float>(), P.p1stride,
+                         lds, c.stream, P1, P.p1stride,
                          P.wy.as<float>(), nxl, nyl, ny, nz, P.yblk,
-                         P.P2.as<float>(), P.p2stride);
+                         P2, P.p2stride);
         GBK_HIP(hipGetLastError());
     }
     mark(b, ci, 3);
@@ -1433,27 +1564,27 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
         dim3 grid(ceil_div(nxl, 64), ceil_div(nyl, kSepThreads / 64),
                   P.zblocks * nsets);
         GBK_SEP_DISPATCH(sep_zpass, D.scale[2], P.tz, grid, dim3(kSepThreads),
-                         0, c.stream, P.P2.as<float>(), P.p2stride,
+                         0, c.stream, P2, P.p2stride,
                          P.wz.as<float>(), nxl, nyl, nzl, nz, P.zblocks, da,
                          cubes ? cubes + (size_t)set0 * b.n_lo : nullptr,
-                         b.n_lo, b.partials.as<double>(), b.nparts);
+                         b.n_lo, partials, b.nparts);
         GBK_HIP(hipGetLastError());
     }
     mark(b, ci, 4);
     hipLaunchKernelGGL(batch_chi2_final, dim3(nsets), dim3(256), 0, c.stream,
-                       b.partials.as<double>(), b.nparts, chi2, set0);
+                       partials, b.nparts, chi2, set0);
     GBK_HIP(hipGetLastError());
     mark(b, ci, 5);
 }
 
 static void run_chunk(Batch& b, const float* params, int set0, int nsets,
-                      double* chi2, float* cubes)
+                      double* chi2, float* cubes, int lane)
 {
     Context& c = *b.ctx;
     const int ci = set0 / b.desc.max_batch;
     mark(b, ci, 0);
     if (b.sep.on) {
-        run_chunk_sep(b, params, set0, nsets, chi2, cubes, ci);
+        run_chunk_sep(b, params, set0, nsets, chi2, cubes, ci, lane);
         return;
     }
     if (b.fused.on) {
@@ -1464,7 +1595,19 @@ static void run_chunk(Batch& b, const float* params, int set0, int nsets,
     const gbkhip_grid& g = D.grid;
     const float* P = params + (size_t)set0 * D.pstride;
     float* hi = b.hi.as<float>();
-    if (b.thin) {
+    if (D.mcdisk) {
+        // the same 256-block x 256-thread cloud loop for every set
+        hipLaunchKernelGGL(batch_mcdisk, dim3(256, nsets), dim3(256), 0,
+                           c.stream, b.tmpl, b.lay, P, D.pstride, g, D.cflux,
+                           b.csum + (size_t)set0 * D.csum_len, D.csum_len,
+                           D.hoi, (uint32_t)D.seed, (uint32_t)(D.seed >> 32),
+                           b.qscale + set0,
+                           b.icube.as<unsigned long long>(), b.n_hi);
+        GBK_HIP(hipGetLastError());
+        hipLaunchKernelGGL(batch_fixed_to_hi, dim3(1024, nsets), dim3(256), 0,
+                           c.stream, b.icube.as<unsigned long long>(), hi,
+                           b.n_hi, b.qscale + set0);
+    } else if (b.thin) {
         dim3 grid(ceil_div(g.spat_size[0], 64), ceil_div(g.spat_size[1], 4),
                   nsets);
         hipLaunchKernelGGL(batch_smdisk_thin, grid, dim3(64, 4), 0, c.stream,
@@ -1506,12 +1649,33 @@ static void run_chunk(Batch& b, const float* params, int set0, int nsets,
 static void run_all(Batch& b, const float* params, int nsets, double* chi2,
                     float* cubes)
 {
+    Context& c = *b.ctx;
     const int chunks = (nsets + b.desc.max_batch - 1) / b.desc.max_batch;
     if (b.timing) ensure_events(b, chunks);
     b.timed_chunks = b.timing ? chunks : 0;
-    for (int s0 = 0; s0 < nsets; s0 += b.desc.max_batch)
+    // per-kernel timing measures each kernel alone: one lane then
+    const bool two = b.lanes > 1 && chunks > 1 && !b.timing;
+    const hipStream_t main = c.stream;
+    if (two) {
+        GBK_HIP(hipEventRecord(b.ev_fork, main));
+        GBK_HIP(hipStreamWaitEvent(b.aux, b.ev_fork, 0));
+    }
+    struct Restore {
+        Context& c;
+        hipStream_t s;
+        ~Restore() { c.stream = s; }
+    } restore{c, main};
+    for (int s0 = 0, ci = 0; s0 < nsets; s0 += b.desc.max_batch, ++ci) {
+        const int lane = two ? ci & 1 : 0;
+        c.stream = lane ? b.aux : main;
         run_chunk(b, params, s0, std::min(b.desc.max_batch, nsets - s0), chi2,
-                  cubes);
+                  cubes, lane);
+    }
+    c.stream = main;
+    if (two) {
+        GBK_HIP(hipEventRecord(b.ev_join, b.aux));
+        GBK_HIP(hipStreamWaitEvent(main, b.ev_join, 0));
+    }
 }
 
 }  // namespace gbkhip
@@ -1562,6 +1726,19 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
             GBK_CHECK(D.path >= 0 && D.path <= 3, GBKHIP_ERR_ARG,
                       "path must be 0 (automatic), 1 (padded-cube FFT), "
                       "2 (fused FFT) or 3 (direct separable)");
+            if (D.mcdisk) {
+                GBK_CHECK(D.path <= 1, GBKHIP_ERR_UNSUPPORTED,
+                          "mcdisk batches run the padded-cube pipeline");
+                GBK_CHECK(D.hoi != nullptr && D.csum_len > 0, GBKHIP_ERR_ARG,
+                          "mcdisk batch needs hoi and csum_len");
+                // cloud i of trait t samples height trait t
+                // (gmodels.hpp:406-416)
+                GBK_CHECK(D.traits[0].n > 0 && D.traits[1].n >= D.traits[0].n,
+                          GBKHIP_ERR_ARG,
+                          "mcdisk needs one density height trait per polar "
+                          "trait");
+                B.thin = false;
+            }
             plan_fused(*c, B);
             if ((D.path == 0 || D.path == 3) && plan_sep(B) &&
                 (D.path == 3 || sep_cheaper(B))) {
@@ -1572,6 +1749,13 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
                 B.sep.on = true;
                 B.ka_psf.clear();
                 B.ka_lsf.clear();
+                B.lanes = 2;
+                GBK_HIP(hipStreamCreateWithFlags(&B.aux,
+                                                 hipStreamNonBlocking));
+                GBK_HIP(hipEventCreateWithFlags(&B.ev_fork,
+                                                hipEventDisableTiming));
+                GBK_HIP(hipEventCreateWithFlags(&B.ev_join,
+                                                hipEventDisableTiming));
                 alloc_work(B);
                 GBK_HIP(hipStreamSynchronize(c->stream));
                 *out = b;
@@ -1604,7 +1788,7 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
             }
             alloc_work(B);
             fft_prepare_attributes();
-            if (!B.thin) {
+            if (!B.thin && !D.mcdisk) {
                 (void)hipFuncSetAttribute(
                     (const void*)batch_smdisk_thick,
                     hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
@@ -1630,63 +1814,99 @@ int gbkhip_batch_destroy(gbkhip_batch* b)
     });
 }
 
+static void evaluate_impl(gbkhip_batch* b, const float* params,
+                          const int* csum, const double* qscale, int nsets,
+                          double* chi2, float* cubes, int use_graph)
+{
+    GBK_CHECK(b != nullptr, GBKHIP_ERR_ARG, "null batch");
+    Batch& B = b->impl;
+    Context& c = *B.ctx;
+    GBK_HIP(hipSetDevice(c.device));
+    GBK_CHECK(nsets >= 0, GBKHIP_ERR_ARG, "negative set count");
+    if (nsets == 0) return;
+    GBK_CHECK(params && chi2, GBKHIP_ERR_ARG, "null params or chi2");
+    // the launches go to the context's stream member: point it at this
+    // batch's stream for the duration of the call (under the context lock,
+    // so batches of one context can be driven from several threads)
+    std::lock_guard<std::mutex> lk(c.mu);
+    const hipStream_t ctx_stream = c.stream;
+    const hipStream_t target = B.stream ? B.stream : ctx_stream;
+    struct Restore {
+        Context& c;
+        hipStream_t s;
+        ~Restore() { c.stream = s; }
+    } restore{c, ctx_stream};
+    alloc_work(B);
+    c.stream = target;
+    B.csum = csum;
+    B.qscale = qscale;
+    if (!use_graph || B.timing) {   // stage events need direct launches
+        run_all(B, params, nsets, chi2, cubes);
+        return;
+    }
+    const auto key = std::make_tuple((uintptr_t)params, nsets,
+                                     (uintptr_t)chi2, (uintptr_t)cubes,
+                                     (uintptr_t)csum, (uintptr_t)qscale);
+    auto it = B.graphs.find(key);
+    if (it == B.graphs.end()) {
+        if (!B.cap_stream)
+            GBK_HIP(hipStreamCreateWithFlags(&B.cap_stream,
+                                             hipStreamNonBlocking));
+        // capture on a private stream, ordered after the target stream
+        GBK_HIP(hipStreamSynchronize(target));
+        c.stream = B.cap_stream;
+        hipGraph_t graph = nullptr;
+        GBK_HIP(hipStreamBeginCapture(B.cap_stream,
+                                      hipStreamCaptureModeThreadLocal));
+        try {
+            run_all(B, params, nsets, chi2, cubes);
+        } catch (...) {
+            (void)hipStreamEndCapture(B.cap_stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        GBK_HIP(hipStreamEndCapture(B.cap_stream, &graph));
+        c.stream = target;
+        hipGraphExec_t exec = nullptr;
+        GBK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        GBK_HIP(hipGraphDestroy(graph));
+        it = B.graphs.emplace(key, exec).first;
+    }
+    GBK_HIP(hipGraphLaunch(it->second, target));
+}
+
 int gbkhip_batch_evaluate(gbkhip_batch* b, const float* params, int nsets,
                           double* chi2, float* cubes, int use_graph)
 {
     return guarded([&] {
-        GBK_CHECK(b != nullptr, GBKHIP_ERR_ARG, "null batch");
-        Batch& B = b->impl;
-        Context& c = *B.ctx;
-        GBK_HIP(hipSetDevice(c.device));
-        GBK_CHECK(nsets >= 0, GBKHIP_ERR_ARG, "negative set count");
-        if (nsets == 0) return;
-        GBK_CHECK(params && chi2, GBKHIP_ERR_ARG, "null params or chi2");
-        // the launches go to the context's stream member: point it at this
-        // batch's stream for the duration of the call (under the context
-        // lock, so batches of one context can be driven from several
-        // threads)
-        std::lock_guard<std::mutex> lk(c.mu);
-        const hipStream_t ctx_stream = c.stream;
-        const hipStream_t target = B.stream ? B.stream : ctx_stream;
-        struct Restore {
-            Context& c;
-            hipStream_t s;
-            ~Restore() { c.stream = s; }
-        } restore{c, ctx_stream};
-        alloc_work(B);
-        c.stream = target;
-        if (!use_graph || B.timing) {   // stage events need direct launches
-            run_all(B, params, nsets, chi2, cubes);
-            return;
-        }
-        const auto key = std::make_tuple((uintptr_t)params, nsets,
-                                         (uintptr_t)chi2, (uintptr_t)cubes);
-        auto it = B.graphs.find(key);
-        if (it == B.graphs.end()) {
-            if (!B.cap_stream)
-                GBK_HIP(hipStreamCreateWithFlags(&B.cap_stream,
-                                                 hipStreamNonBlocking));
-            // capture on a private stream, ordered after the target stream
-            GBK_HIP(hipStreamSynchronize(target));
-            c.stream = B.cap_stream;
-            hipGraph_t graph = nullptr;
-            GBK_HIP(hipStreamBeginCapture(B.cap_stream,
-                                          hipStreamCaptureModeThreadLocal));
-            try {
-                run_all(B, params, nsets, chi2, cubes);
-            } catch (...) {
-                (void)hipStreamEndCapture(B.cap_stream, &graph);
-                if (graph) (void)hipGraphDestroy(graph);
-                throw;
-            }
-            GBK_HIP(hipStreamEndCapture(B.cap_stream, &graph));
-            c.stream = target;
-            hipGraphExec_t exec = nullptr;
-            GBK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-            GBK_HIP(hipGraphDestroy(graph));
-            it = B.graphs.emplace(key, exec).first;
-        }
-        GBK_HIP(hipGraphLaunch(it->second, target));
+        GBK_CHECK(b != nullptr && !b->impl.desc.mcdisk, GBKHIP_ERR_ARG,
+                  "mcdisk batches evaluate with gbkhip_batch_evaluate_clouds");
+        evaluate_impl(b, params, nullptr, nullptr, nsets, chi2, cubes,
+                      use_graph);
+    });
+}
+
+int gbkhip_batch_evaluate_clouds(gbkhip_batch* b, const float* params,
+                                 const int* csum, const double* qscale,
+                                 int nsets, double* chi2, float* cubes,
+                                 int use_graph)
+{
+    return guarded([&] {
+        GBK_CHECK(b != nullptr && b->impl.desc.mcdisk, GBKHIP_ERR_ARG,
+                  "not an mcdisk batch");
+        GBK_CHECK(nsets == 0 || (csum && qscale), GBKHIP_ERR_ARG,
+                  "null csum or qscale");
+        evaluate_impl(b, params, csum, qscale, nsets, chi2, cubes,
+                      use_graph);
+    });
+}
+
+int gbkhip_mcdisk_fixed_scale(float cflux, int nclouds,
+                              const gbkhip_grid* grid, double* out)
+{
+    return guarded([&] {
+        GBK_CHECK(grid && out, GBKHIP_ERR_ARG, "null grid or out");
+        *out = mcdisk_fixed_scale(cflux, nclouds, *grid);
     });
 }
 

@@ -24,6 +24,7 @@ SOURCES = ['api.hip', 'smdisk.hip', 'mcdisk.hip', 'fft.hip', 'dmodel.hip',
 FLAGS = ['--offload-arch=gfx950', '-O3', '-fPIC', '-std=c++17',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
          '-Wall', '-Wno-unused-result', '-Wno-unknown-pragmas',
+         '-Werror=uninitialized', '-Werror=sometimes-uninitialized',
          '-I' + os.path.join(ROOT, 'include')]
 
 

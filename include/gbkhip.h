@@ -224,6 +224,14 @@ typedef struct {
                                 a thin disk and a PSF x LSF cube that
                                 factors within the DCube edge; automatic
                                 takes the cheaper of 2 and 3, else 1 */
+    /* Monte Carlo cloud disk (gmodels.hpp:325-643) instead of smdisk: runs
+     * the padded-cube pipeline; evaluate with gbkhip_batch_evaluate_clouds */
+    int mcdisk;
+    float cflux;             /* flux of one cloud */
+    int csum_len;            /* length of each set's cumulative cloud counts */
+    const unsigned char* hoi;   /* device, per density trait: analytic
+                                   integral (one sub-ring) or not */
+    unsigned long long seed; /* Philox key of the cloud stream */
 } gbkhip_batch_desc;
 
 typedef struct gbkhip_batch gbkhip_batch;
@@ -240,6 +248,20 @@ int gbkhip_batch_destroy(gbkhip_batch* batch);
 int gbkhip_batch_evaluate(gbkhip_batch* batch, const float* params,
                           int nsets, double* chi2, float* cubes,
                           int use_graph);
+/* evaluate() of an mcdisk batch.  csum (device, nsets x csum_len int32):
+ * each set's cumulative cloud counts per sub-ring, the integral of its
+ * density traits over cflux (_mcdisk.py:16-189; the host computes it, as the
+ * reference does); qscale (device, nsets FP64): each set's fixed-point scale
+ * (gbkhip_mcdisk_fixed_scale of its total cloud count).  A set's result
+ * depends only on its own parameters and counts. */
+int gbkhip_batch_evaluate_clouds(gbkhip_batch* batch, const float* params,
+                                 const int* csum, const double* qscale,
+                                 int nsets, double* chi2, float* cubes,
+                                 int use_graph);
+/* Fixed-point scale 2^S of the int64 cube of an mcdisk evaluation of nclouds
+ * clouds of flux cflux on grid: the value gbkhip_mcdisk_evaluate uses. */
+int gbkhip_mcdisk_fixed_scale(float cflux, int nclouds,
+                              const gbkhip_grid* grid, double* out);
 /* Stage timing: with timing on, evaluate() launches directly (no graph) and
  * records events around the smdisk, convolution and downscale+chi^2 stages
  * of every chunk; stage_times() waits for them and returns the summed stage

@@ -74,6 +74,25 @@ def build_product(driver, p, orders=None, padding='reference'):
     return Model(driver, dm, gm), gparams(p)
 
 
+def build_mc_product(driver, p, cflux, padding='reference'):
+    """(Model, params) of a Monte Carlo cloud disk (SpectralMCDisk3D with
+    the same traits as the thick smdisk) through gbkfit_amd's model layer."""
+    from gbkfit_amd import gmodels, psflsf
+    from gbkfit_amd.dmodels import DModelSCube
+    from gbkfit_amd.objective import Model
+    from gbkfit_amd.traits import make
+    psf = psflsf.PSFGauss(p['psf_fwhm'] * SIG_FWHM)
+    lsf = psflsf.LSFGauss(p['lsf_fwhm'] * SIG_FWHM)
+    cmp = gmodels.SpectralMCDisk3D(
+        cflux, False, False, make('bpt', 'exponential'),
+        make('vpt', 'tan_arctan'), make('dpt', 'uniform'),
+        make('bht', 'sech2'), rnodes=p['rnodes'])
+    gm = gmodels.GModelKinematics3D([cmp], size_z=p['size_z'])
+    dm = DModelSCube(p['size'], p['step'], scale=p['scale'], psf=psf,
+                     lsf=lsf, padding=padding)
+    return Model(driver, dm, gm), gparams(dict(p, thin=False))
+
+
 def oracle_spec(p):
     T = oracle.Trait
     traits = dict(rp=[T(2, params=(p['bpt_a'], p['bpt_s']))],

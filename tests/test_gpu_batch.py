@@ -173,3 +173,33 @@ def test_thick_disk_uses_padded_path():
         plain = -2 * obj.log_likelihood(s)[0]
         assert abs(plain - chi2[i]) <= 1e-5 * chi2[i], (i, plain, chi2[i])
     ev.close()
+
+
+def test_mcdisk_batch_matches_plain_path():
+    """Batched mcdisk (padded-cube pipeline, per-set cloud budgets): model
+    cubes and chi^2 of every set against the plain one-model path (which the
+    kernel tests pin to the oracle's Philox stream), a set alone equal to
+    the same set in the batch, and graph replay equal to direct launches."""
+    from gbkfit_amd.batch import BatchEvaluator
+    drv = H.driver()
+    p = dict(P.c1_params(), size=(32, 32, 24), size_z=24)
+    model, base = P.build_mc_product(drv, p, cflux=0.01)
+    ref = model.model_h(base)['scube']['d']
+    rng = np.random.default_rng(5)
+    mock = (ref + rng.normal(0, 0.01, ref.shape)).astype(np.float32)
+    ev = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock,
+                        error=0.01, max_batch=2, template_params=base)
+    assert ev.info()['pipeline'] == 'padded'
+    sets = [dict(base, incl=i, posa=a, bpt_s=s) for i, a, s in
+            ((45, 30, 8.0), (60, 200, 6.0), (75, 300, 10.0))]
+    chi2, cubes = ev.evaluate(sets, want_cubes=True, use_graph=False)
+    for i, s in enumerate(sets):
+        plain = model.model_h(s)['scube']['d']
+        assert H.normwise(cubes[i], plain) < 1e-5, i
+        c_ref = float(np.sum(((plain.astype(np.float64) - mock) / 0.01) ** 2))
+        assert abs(chi2[i] - c_ref) <= 1e-5 * c_ref, (i, chi2[i], c_ref)
+    alone = ev.evaluate([sets[1]], use_graph=False)
+    assert alone[0] == chi2[1]
+    g = ev.evaluate(sets, use_graph=True)
+    assert np.array_equal(g, chi2)
+    ev.close()
