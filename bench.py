@@ -26,11 +26,18 @@ The plain path is the drop-in call sequence of the reference
 FFT convolution, downscale, residual, residual sum, scalar download).
 
 A serving leg (skip with --no-serve) replays a Poisson stream of
-single-set requests (default 1000/s for 8 s) over 16 resident galaxies with
-cube sizes log-uniform from 64x64x32 to 256x256x128; each request returns
-chi^2 and moment maps 0/1/2 through gbkfit_amd.serve.Server (continuous
-batching over 4 streams, HBM-budget admission).  Its throughput and p50/p99
-latencies go into the `serving` object.
+single-set requests (default 1000/s for 30 s) over 64 resident galaxies with
+cube sizes log-uniform from 64x64x32 to 256x256x128, half smdisk, half mcdisk
+with 2^20 clouds; each request returns chi^2 and moment maps 0/1/2 through
+gbkfit_amd.serve.Server (continuous batching over 4 streams, HBM-budget
+admission).  Its throughput and p50/p99 latencies go into the `serving`
+object.
+
+A ragged leg (skip with --no-ragged) autotunes 32 galaxies of log-uniform
+sizes from 64x64x32 to 512x512x256 (smdisk / mcdisk with 2^20..2^24 clouds,
+2x or 3x supersampling, random PSF / LSF widths, 2/3/5-smooth padding) and
+times one graph launch evaluating one set of every galaxy, plus the moment
+maps of all of them: the `ragged` object.
 
 Prints one JSON line.
 """
@@ -95,23 +102,45 @@ def build_model(driver, thick=False):
     return Model(driver, dm, gm)
 
 
-def build_galaxy_model(driver, size, scale=(1, 1, 1), rmax=None):
-    """Thin smdisk + PSF (FWHM 2.5 px) + LSF (FWHM 2 ch) model of a
-    `size` cube, ring nodes out to ~40 % of the field."""
+def build_galaxy_model(driver, size, scale=(1, 1, 1), rmax=None,
+                       kind='smdisk', psf_fwhm=2.5, lsf_ch=2.0,
+                       padding='reference', nclouds=None, size_z=None):
+    """(Model, truth params) of a galaxy in a `size` cube: a thin smdisk
+    (Kinematics2D) or a Monte Carlo cloud disk with a sech^2 height profile
+    (Kinematics3D, explicit spatial z size, about `nclouds` clouds), ring
+    nodes out to ~40 % of the field, Gaussian PSF / LSF (FWHM in pixels and
+    channels of 10 km/s)."""
     from gbkfit_amd import gmodels, psflsf
     from gbkfit_amd.dmodels import DModelSCube
     from gbkfit_amd.objective import Model
     from gbkfit_amd.traits import make
     rmax = rmax or 0.4 * min(size[0], size[1])
     rn = np.linspace(0, rmax, 16)
-    cmp = gmodels.SpectralSMDisk2D(
-        False, False, make('bpt', 'exponential'), make('vpt', 'tan_arctan'),
-        make('dpt', 'uniform'), rnodes=rn)
-    gm = gmodels.GModelKinematics2D([cmp])
+    truth = dict(xpos=0.0, ypos=0.0, vsys=0.0, posa=30.0, incl=50.0,
+                 bpt_a=1.0, bpt_s=0.15 * size[0], vpt_rt=0.05 * size[0],
+                 vpt_vt=200.0, dpt_a=20.0)
     dm = DModelSCube(tuple(size), (1, 1, 10), scale=scale,
-                     psf=psflsf.PSFGauss(2.5 * SIG),
-                     lsf=psflsf.LSFGauss(20.0 * SIG))
-    return Model(driver, dm, gm)
+                     psf=psflsf.PSFGauss(psf_fwhm * SIG),
+                     lsf=psflsf.LSFGauss(10.0 * lsf_ch * SIG),
+                     padding=padding)
+    if kind == 'smdisk':
+        cmp = gmodels.SpectralSMDisk2D(
+            False, False, make('bpt', 'exponential'),
+            make('vpt', 'tan_arctan'), make('dpt', 'uniform'), rnodes=rn)
+        return Model(driver, dm, gmodels.GModelKinematics2D([cmp])), truth
+    truth['bht_s'] = 1.0
+
+    def mc(cflux):
+        cmp = gmodels.SpectralMCDisk3D(
+            cflux, False, False, make('bpt', 'exponential'),
+            make('vpt', 'tan_arctan'), make('dpt', 'uniform'),
+            make('bht', 'sech2'), rnodes=rn)
+        return cmp, gmodels.GModelKinematics3D([cmp], size_z=size_z or 16)
+    # flux per cloud for about `nclouds` clouds over the disk's integral
+    cmp, _ = mc(1e-3)
+    total = float(cmp.disk.cloud_budget(truth)[0][-1]) * 1e-3
+    cmp, gm = mc(total / float(nclouds or (1 << 20)))
+    return Model(driver, dm, gm), truth
 
 
 def serve_benchmark(drv, ngal, seconds, rate, slots=4, max_batch=16,
@@ -126,13 +155,14 @@ def serve_benchmark(drv, ngal, seconds, rate, slots=4, max_batch=16,
     streams = make_streams(slots)
     gals, sizes = [], []
     t0 = time.perf_counter()
+    kinds = []
     for g in range(ngal):
         f = np.exp(rng.uniform(np.log(1.0), np.log(4.0)))
         size = (int(64 * f) // 2 * 2, int(64 * f) // 2 * 2, int(32 * f))
-        model = build_galaxy_model(drv, size)
-        truth = dict(xpos=0.0, ypos=0.0, vsys=0.0, posa=30.0, incl=50.0,
-                     bpt_a=1.0, bpt_s=0.15 * size[0], vpt_rt=0.05 * size[0],
-                     vpt_vt=200.0, dpt_a=20.0)
+        kind = 'mcdisk' if g % 2 else 'smdisk'
+        model, truth = build_galaxy_model(drv, size, kind=kind,
+                                          nclouds=1 << 20)
+        kinds.append(kind)
         cube = model.model_h(truth)['scube']['d'].astype(np.float32)
         mock = (cube + rng.normal(0, 0.01, cube.shape)).astype(np.float32)
         gals.append(GpuGalaxy(drv, model.dmodel(), model.gmodel(), mock,
@@ -156,11 +186,17 @@ def serve_benchmark(drv, ngal, seconds, rate, slots=4, max_batch=16,
                  incl=float(rng.uniform(20, 80)), bpt_a=1.0,
                  bpt_s=0.15 * sizes[t][0], vpt_rt=0.05 * sizes[t][0],
                  vpt_vt=float(rng.uniform(100, 300)),
-                 dpt_a=float(rng.uniform(5, 50))) for t in targets]
+                 dpt_a=float(rng.uniform(5, 50)),
+                 **({'bht_s': 1.0} if kinds[t] == 'mcdisk' else {}))
+            for t in targets]
     # warm the graphs of the first batch sizes of every galaxy
     for i, g in enumerate(gals):
+        p0 = next(pars[j] for j in range(n) if targets[j] == i) \
+            if i in set(targets.tolist()) else None
+        if p0 is None:
+            continue
         for k in (1, 2):
-            g.results(g.launch([pars[0]] * k, 0), k)
+            g.results(g.launch([p0] * k, 0), k)
         g.release()
     torch.cuda.synchronize()
     futs = []
@@ -185,10 +221,108 @@ def serve_benchmark(drv, ngal, seconds, rate, slots=4, max_batch=16,
                hbm_budget_bytes=budget, work_bytes_total=int(sum(works)),
                releases=srv.stats['releases'],
                cube_sizes=dict(min=list(min(sizes)), max=list(max(sizes))),
+               models=dict(smdisk=kinds.count('smdisk'),
+                           mcdisk=kinds.count('mcdisk'),
+                           mcdisk_clouds=1 << 20),
                setup_s=t_setup, returns='chi2 + moment maps 0/1/2')
     for g in gals:
         g.close()
     return out
+
+
+def ragged_benchmark(drv, ngal, reps, seed=2024, budget_frac=0.6):
+    """Ragged batch: `ngal` galaxies with cube sizes log-uniform from
+    64x64x32 to 512x512x256, smdisk or mcdisk 50/50 (2^20..2^24 clouds),
+    2x or 3x spatial supersampling, PSF FWHM U(1.5, 4) px, LSF FWHM U(1, 3)
+    channels, 2/3/5-smooth FFT padding, explicit spatial z size.  Each galaxy
+    is autotuned (pipeline, tiles) on its own parameter set; the group then
+    evaluates one set per galaxy in one graph launch, followed by the moment
+    maps (orders 0, 1, 2) of every galaxy's cube."""
+    import torch
+    from gbkfit_amd.batch import RaggedGroup, autotune
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    free, _ = torch.cuda.mem_get_info()
+    budget = budget_frac * free
+    evs, metas, lists = [], [], []
+    used = 0
+    for g in range(ngal):
+        f = np.exp(rng.uniform(0.0, np.log(8.0)))
+        size = (int(64 * f) // 2 * 2, int(64 * f) // 2 * 2, int(32 * f))
+        kind = 'mcdisk' if rng.uniform() < 0.5 else 'smdisk'
+        ncl = int(np.exp(rng.uniform(np.log(2.0 ** 20), np.log(2.0 ** 24))))
+        sc = int(rng.choice([2, 3]))
+        psf = float(rng.uniform(1.5, 4.0))
+        lsf = float(rng.uniform(1.0, 3.0))
+        model, truth = build_galaxy_model(
+            drv, size, scale=(sc, sc, 1), kind=kind, psf_fwhm=psf,
+            lsf_ch=lsf, padding='smooth235', nclouds=ncl, size_z=16 * sc)
+        cube = model.model_h(truth)['scube']['d'].astype(np.float32)
+        mock = (cube + rng.normal(0, 0.01, cube.shape)).astype(np.float32)
+        del cube
+        p = dict(truth, incl=float(rng.uniform(20, 80)),
+                 posa=float(rng.uniform(0, 360)))
+        ev, rep = autotune(drv, model.dmodel(), model.gmodel(), mock, [p],
+                           error=0.01, max_batches=(1,), reps=2,
+                           budget_bytes=max(budget - used, 0))
+        mem = ev.memory()
+        used += mem['resident'] + mem['work']
+        evs.append(ev)
+        lists.append([p])
+        dc = model.dmodel().dcube()
+        metas.append(dict(size=size, kind=kind, scale=sc,
+                          clouds=ncl if kind == 'mcdisk' else 0,
+                          pipeline=rep['choice'][0], tiles=rep['choice'][2],
+                          us_per_eval=1e6 * rep['s_per_set'],
+                          lo=tuple(dc.size()), step=tuple(dc.step()),
+                          zero=tuple(dc.zero()),
+                          hi_voxels=int(np.prod(ev.size_hi))))
+    t_setup = time.perf_counter() - t0
+    grp = RaggedGroup(evs)
+    be = drv.backends().dmodel(np.float32)
+    orders = torch.tensor([0, 1, 2], dtype=torch.int32, device=drv.device)
+    maps = [torch.empty((3, m['lo'][1], m['lo'][0]), dtype=torch.float32,
+                        device=drv.device) for m in metas]
+    mmask = [torch.empty((m['lo'][1], m['lo'][0]), dtype=torch.float32,
+                         device=drv.device) for m in metas]
+
+    def one():
+        chi2, _ = grp.evaluate(lists, want_cubes=True, use_graph=True,
+                               download_cubes=False)
+        for ev, m, mp, mk in zip(evs, metas, maps, mmask):
+            be.mmaps_moments(m['lo'], m['step'], m['zero'],
+                             ev._cubes_dev[0], None, 1e-6, orders, mp, None,
+                             mk)
+        torch.cuda.synchronize()
+        return chi2
+    one()
+    one()
+    ts = []
+    for _ in range(reps):
+        t1 = time.perf_counter()
+        chi2 = one()
+        ts.append(time.perf_counter() - t1)
+    grp.close()
+    for ev in evs:
+        ev.close()
+    ms = 1e3 * float(np.median(ts))
+    hv = sum(m['hi_voxels'] for m in metas)
+    pipes = {}
+    for m in metas:
+        pipes[m['pipeline']] = pipes.get(m['pipeline'], 0) + 1
+    return dict(galaxies=ngal, ms_per_batch=ms, galaxies_per_s=1e3 * ngal / ms,
+                padded_hi_voxels_per_s=hv / (ms * 1e-3), reps=reps,
+                chi2_finite=bool(all(np.isfinite(c).all() for c in chi2)),
+                models=dict(smdisk=sum(m['kind'] == 'smdisk' for m in metas),
+                            mcdisk=sum(m['kind'] == 'mcdisk' for m in metas)),
+                pipelines=pipes, setup_s=t_setup, device_bytes=int(used),
+                cube_sizes=dict(min=list(min(m['size'] for m in metas)),
+                                max=list(max(m['size'] for m in metas))),
+                returns='cubes + chi2 + moment maps 0/1/2 per galaxy',
+                per_galaxy=[{k: m[k] for k in ('size', 'kind', 'scale',
+                                               'clouds', 'pipeline',
+                                               'us_per_eval')}
+                            for m in metas])
 
 
 def percentiles(xs):
@@ -325,10 +459,13 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-graph', action='store_true')
     ap.add_argument('--thick', action='store_true')
-    ap.add_argument('--serve-galaxies', type=int, default=16)
-    ap.add_argument('--serve-seconds', type=float, default=8.0)
+    ap.add_argument('--serve-galaxies', type=int, default=64)
+    ap.add_argument('--serve-seconds', type=float, default=30.0)
     ap.add_argument('--serve-rate', type=float, default=1000.0)
     ap.add_argument('--no-serve', action='store_true')
+    ap.add_argument('--ragged-galaxies', type=int, default=32)
+    ap.add_argument('--ragged-reps', type=int, default=5)
+    ap.add_argument('--no-ragged', action='store_true')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only copy a 1 GiB device buffer 3 times (known '
                          'bytes for scaling FETCH_SIZE / WRITE_SIZE)')
@@ -468,10 +605,18 @@ def main():
                         traffic=traffic, kernel=dom)
 
     serving = None
+    ev.close()
     if not args.no_serve:
-        ev.close()
         serving = serve_benchmark(drv, args.serve_galaxies,
                                   args.serve_seconds, args.serve_rate)
+        print(f"serving leg done: {serving['throughput']:.0f} req/s",
+              file=sys.stderr, flush=True)
+    ragged = None
+    if not args.no_ragged:
+        ragged = ragged_benchmark(drv, args.ragged_galaxies,
+                                  args.ragged_reps)
+        print(f"ragged leg done: {ragged['ms_per_batch']:.1f} ms per batch",
+              file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -507,6 +652,7 @@ def main():
         kernels=kernels,
         roofline=dict(roofline, sets_per_launch=per_chunk),
         serving=serving,
+        ragged=ragged,
         cpu_baseline=cpu,
         host=dict(cpu=platform.processor() or platform.machine(),
                   nproc=os.cpu_count()),

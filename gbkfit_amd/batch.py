@@ -58,6 +58,16 @@ N.register_signatures({
     'gbkhip_mcdisk_fixed_scale': ([_c.c_float, _c.c_int, _c.c_void_p,
                                    _c.POINTER(_c.c_double)], _c.c_int),
     'gbkhip_batch_info': ([_c.c_void_p, _c.POINTER(_c.c_int)], _c.c_int),
+    'gbkhip_batch_set_tiles': ([_c.c_void_p, _c.c_int, _c.c_int], _c.c_int),
+    'gbkhip_group_create': ([_c.c_void_p, _c.POINTER(_c.c_void_p), _c.c_int,
+                             _c.POINTER(_c.c_void_p)], _c.c_int),
+    'gbkhip_group_destroy': ([_c.c_void_p], _c.c_int),
+    'gbkhip_group_evaluate': ([_c.c_void_p, _c.POINTER(_c.c_void_p),
+                               _c.POINTER(_c.c_void_p),
+                               _c.POINTER(_c.c_void_p),
+                               _c.POINTER(_c.c_int),
+                               _c.POINTER(_c.c_void_p),
+                               _c.POINTER(_c.c_void_p), _c.c_int], _c.c_int),
     'gbkhip_batch_kernel_times': ([_c.c_void_p, _c.POINTER(_c.c_double),
                                    _c.c_int, _c.POINTER(_c.c_int)], _c.c_int),
     'gbkhip_batch_set_timing': ([_c.c_void_p, _c.c_int], _c.c_int),
@@ -286,8 +296,10 @@ class BatchEvaluator:
             self._upload_clouds(params_list)
         return self.upload_packed(host)
 
-    def _upload_clouds(self, params_list):
-        t = driver_torch()
+    def cloud_budgets(self, params_list):
+        """mcdisk: each set's cumulative cloud counts (int32, nsets x
+        csum_len) and fixed-point scale (FP64), as evaluate_clouds takes
+        them."""
         n = len(params_list)
         csum = np.zeros((n, self.desc.csum_len), np.int32)
         qs = np.zeros(n, np.float64)
@@ -298,6 +310,12 @@ class BatchEvaluator:
             N.call('gbkhip_mcdisk_fixed_scale', _c.c_float(self.desc.cflux),
                    int(c[-1]), _c.byref(self.grid), _c.byref(out))
             qs[i] = out.value
+        return csum, qs
+
+    def _upload_clouds(self, params_list):
+        t = driver_torch()
+        n = len(params_list)
+        csum, qs = self.cloud_budgets(params_list)
         dev = self.driver.device
         if self._csum_dev is None or self._csum_dev.shape != csum.shape:
             self._csum_dev = t.empty(csum.shape, dtype=t.int32, device=dev)
@@ -359,6 +377,12 @@ class BatchEvaluator:
         pipe = ('padded', 'fused', 'direct')[v[0]]
         return dict(pipeline=pipe, fused=pipe == 'fused',
                     fft_size=(v[1], v[2], v[3]), nparts=v[4])
+
+    def set_tiles(self, lanes=2, ysplit=1):
+        """Direct pipeline tiles: chunks in flight (1 or 2) and the split of
+        a y-pass column strip's outputs over blocks (1, 2 or 4)."""
+        N.call('gbkhip_batch_set_tiles', self.handle, int(lanes), int(ysplit))
+        self.tiles = (int(lanes), int(ysplit))
 
     def set_timing(self, enable):
         N.call('gbkhip_batch_set_timing', self.handle, int(bool(enable)))
@@ -424,3 +448,174 @@ def driver_torch():
 
 
 __all__ = ['BatchEvaluator', 'SetPacker', 'NativeMemoryHip']
+
+
+# ---------------------------------------------------------------------------
+# Autotuner and ragged groups
+# ---------------------------------------------------------------------------
+_TUNE_CACHE = {}
+
+
+def _tune_key(dmodel, gmodel, mc):
+    cube = dmodel.dcube()
+    comp = gmodel.components[0]
+    return (tuple(cube.size()), tuple(cube.scale()), tuple(cube.step()),
+            repr(cube.psf()), repr(cube.lsf()), cube._padding, bool(mc),
+            bool(getattr(comp, 'thin', True)))
+
+
+def autotune(driver, dmodel, gmodel, obs, sample_params, error=None,
+             mask=None, weight=1.0, max_batches=(16, 32, 64), reps=3,
+             budget_bytes=None, template_params=None, cache=True):
+    """Chooses the batched pipeline (direct / fused / padded), the chunk size
+    and the direct pipeline's tiles (lanes, ysplit) for one galaxy model by
+    timing each candidate on `sample_params` (graph replays after a warm-up),
+    and returns (BatchEvaluator of the fastest, report).  Candidates whose
+    work buffers exceed budget_bytes are skipped.  Choices are cached per
+    geometry / kernel / model kind, so galaxies of one shape tune once."""
+    import time
+    t = driver_torch()
+    mc = type(gmodel.components[0].disk).__name__ == 'MCDisk'
+    key = _tune_key(dmodel, gmodel, mc)
+    n = len(sample_params)
+    tp = template_params if template_params is not None else sample_params[0]
+
+    def make(path, mb):
+        return BatchEvaluator(driver, dmodel, gmodel, obs, error=error,
+                              mask=mask, weight=weight, max_batch=mb,
+                              template_params=tp, path=path)
+
+    if cache and key in _TUNE_CACHE:
+        path, mb, tiles = _TUNE_CACHE[key]['choice']
+        ev = make(path, min(mb, max(1, n)))
+        if ev.info()['pipeline'] == 'direct':
+            ev.set_tiles(*tiles)
+        return ev, dict(_TUNE_CACHE[key], cached=True)
+
+    def timed(ev):
+        ev.evaluate(sample_params)               # warm-up + graph capture
+        t.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ev.evaluate(sample_params)
+        t.cuda.synchronize()
+        return (time.perf_counter() - t0) / (reps * n)
+
+    mbs = sorted({min(max(1, n), m) for m in max_batches})
+    trials, best = [], None            # best: (s per set, path, mb, tiles)
+    for path in (('padded',) if mc else ('direct', 'fused', 'padded')):
+        try:
+            probe = make(path, 1)
+        except N.GbkHipError:
+            continue                           # pipeline does not apply
+        per_set = probe.memory()['work']
+        probe.close()
+        for mb in mbs:
+            if budget_bytes is not None and per_set * mb > budget_bytes:
+                continue
+            ev = make(path, mb)
+            try:
+                direct = ev.info()['pipeline'] == 'direct'
+                for tiles in ([(2, 1), (1, 1), (2, 2), (2, 4)] if direct
+                              else [(1, 1)]):
+                    if direct:
+                        ev.set_tiles(*tiles)
+                    sec = timed(ev)
+                    trials.append(dict(path=path, max_batch=mb, tiles=tiles,
+                                       s_per_set=sec))
+                    if best is None or sec < best[0]:
+                        best = (sec, path, mb, tiles)
+            finally:
+                ev.close()
+    if best is None:
+        raise RuntimeError("no batched pipeline fits the memory budget")
+    ev = make(best[1], best[2])
+    if ev.info()['pipeline'] == 'direct':
+        ev.set_tiles(*best[3])
+    report = dict(choice=(best[1], best[2], best[3]), s_per_set=best[0],
+                  trials=trials)
+    if cache:
+        _TUNE_CACHE[key] = report
+    return ev, dict(report, cached=False)
+
+
+class RaggedGroup:
+    """Galaxies of different cube sizes, models (smdisk / mcdisk) and data,
+    each a BatchEvaluator (typically from autotune), evaluated together in
+    one native call (gbkhip_group_evaluate): one HIP graph launch per set of
+    buffers, members spread over parallel streams.  Each member's chi^2
+    equals its own evaluate()."""
+
+    def __init__(self, evaluators):
+        self.members = list(evaluators)
+        if not self.members:
+            raise ValueError("empty group")
+        drv = self.members[0].driver
+        self.driver = drv
+        self._ctx = drv.context()
+        arr = (_c.c_void_p * len(self.members))(
+            *[m.handle.value for m in self.members])
+        h = _c.c_void_p()
+        N.call('gbkhip_group_create', self._ctx.handle, arr,
+               len(self.members), _c.byref(h))
+        self.handle = h
+
+    def evaluate(self, params_lists, want_cubes=False, use_graph=True,
+                 download_cubes=True):
+        """params_lists[i]: the parameter dicts of member i.  Returns the
+        list of chi^2 arrays (and of model cube arrays if asked; with
+        download_cubes False the cubes stay on the device, in each member's
+        cube buffer, and None is returned for them)."""
+        t = driver_torch()
+        n = len(self.members)
+        if len(params_lists) != n:
+            raise ValueError("one parameter list per member")
+        P = (_c.c_void_p * n)()
+        C = (_c.c_void_p * n)()
+        Q = (_c.c_void_p * n)()
+        S = (_c.c_int * n)()
+        X = (_c.c_void_p * n)()
+        K = (_c.c_void_p * n)()
+        any_mc = False
+        for i, (ev, pl) in enumerate(zip(self.members, params_lists)):
+            k = ev.upload(pl) if len(pl) else 0
+            S[i] = k
+            if not k:
+                continue
+            P[i] = ev._params_dev.data_ptr()
+            X[i] = ev._chi2_dev.data_ptr()
+            if ev.mc:
+                any_mc = True
+                C[i] = ev._csum_dev.data_ptr()
+                Q[i] = ev._qscale_dev.data_ptr()
+            if want_cubes:
+                if ev._cubes_dev is None or ev._cubes_dev.shape[0] != k:
+                    ev._cubes_dev = t.empty((k,) + ev.size_lo[::-1],
+                                            dtype=t.float32,
+                                            device=ev.driver.device)
+                K[i] = ev._cubes_dev.data_ptr()
+        # the group launches on the context stream: bind it to torch's
+        # current stream, where the uploads above are ordered
+        self.driver.context()
+        N.call('gbkhip_group_evaluate', self.handle, P,
+               C if any_mc else None, Q if any_mc else None, S, X,
+               K if want_cubes else None, int(bool(use_graph)))
+        chi2 = [ev._chi2_dev.cpu().numpy().copy() if S[i] else
+                np.zeros(0) for i, ev in enumerate(self.members)]
+        if want_cubes:
+            cubes = [ev._cubes_dev.cpu().numpy().copy()
+                     if S[i] and download_cubes else None
+                     for i, ev in enumerate(self.members)]
+            return chi2, cubes
+        return chi2
+
+    def close(self):
+        if getattr(self, 'handle', None):
+            N.call('gbkhip_group_destroy', self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -602,6 +602,7 @@ __global__ void batch_zero_u64(unsigned long long* p, long long n)
 // taps; templates fix the tap buckets and downscale factors so the inner
 // loops are register-blocked with constant offsets.
 // ---------------------------------------------------------------------------
+constexpr int kGroupBranches = 4;   // parallel streams of a ragged group
 constexpr int kSepThreads = 256;
 constexpr int kSepZC = 16;          // most channels per S1 block
 constexpr int kSepJZ = 16;          // outputs per thread, S3
@@ -861,6 +862,7 @@ struct Sep {
     int mp = 0;                   // S1 LDS row length (odd)
     int zc = 0;                   // channels per S1 block (tile <= 64 KB)
     int yblk = 0;                 // Y outputs per S2 block
+    int yblk_full = 0;            // ... when one block covers every Y
     DevBuf wx, wy, wz;            // taps
     DevBuf P1, P2;                // work: per set (z, y, X) and (z, Y, X)
     long long p1stride = 0, p2stride = 0;
@@ -1447,6 +1449,7 @@ static bool plan_sep(Batch& B)
     const int gmax = ((max_rows - P.ty) / sy + 1) / jy;
     if (gmax < 1) return false;
     P.yblk = std::min((nyl + jy - 1) / jy, gmax) * jy;
+    P.yblk_full = P.yblk;
     P.p1stride = (long long)P.need[2] * P.need[1] * nxl;
     P.p2stride = (long long)P.need[2] * nyl * nxl;
     P.zblocks = (nzl + kSepJZ - 1) / kSepJZ;
@@ -1646,6 +1649,14 @@ static void run_chunk(Batch& b, const float* params, int set0, int nsets,
     mark(b, ci, 3);
 }
 
+static void ensure_lane_streams(Batch& B)
+{
+    if (B.aux) return;
+    GBK_HIP(hipStreamCreateWithFlags(&B.aux, hipStreamNonBlocking));
+    GBK_HIP(hipEventCreateWithFlags(&B.ev_fork, hipEventDisableTiming));
+    GBK_HIP(hipEventCreateWithFlags(&B.ev_join, hipEventDisableTiming));
+}
+
 static void run_all(Batch& b, const float* params, int nsets, double* chi2,
                     float* cubes)
 {
@@ -1682,6 +1693,27 @@ static void run_all(Batch& b, const float* params, int nsets, double* chi2,
 
 struct gbkhip_batch {
     gbkhip::Batch impl;
+};
+
+// A ragged group: member batches (galaxies of different geometry, model
+// and data) launched together, on up to kGroupBranches parallel streams,
+// captured in one HIP graph per set of call pointers.
+struct gbkhip_group {
+    gbkhip::Context* ctx = nullptr;
+    std::vector<gbkhip_batch*> members;
+    std::vector<hipStream_t> branch;
+    std::vector<hipEvent_t> ev_join;
+    hipEvent_t ev_fork = nullptr;
+    hipStream_t cap = nullptr;
+    std::map<std::vector<uintptr_t>, hipGraphExec_t> graphs;
+    ~gbkhip_group()
+    {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        for (auto s : branch) (void)hipStreamDestroy(s);
+        for (auto e : ev_join) (void)hipEventDestroy(e);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (cap) (void)hipStreamDestroy(cap);
+    }
 };
 
 using namespace gbkhip;
@@ -1750,12 +1782,7 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
                 B.ka_psf.clear();
                 B.ka_lsf.clear();
                 B.lanes = 2;
-                GBK_HIP(hipStreamCreateWithFlags(&B.aux,
-                                                 hipStreamNonBlocking));
-                GBK_HIP(hipEventCreateWithFlags(&B.ev_fork,
-                                                hipEventDisableTiming));
-                GBK_HIP(hipEventCreateWithFlags(&B.ev_join,
-                                                hipEventDisableTiming));
+                ensure_lane_streams(B);
                 alloc_work(B);
                 GBK_HIP(hipStreamSynchronize(c->stream));
                 *out = b;
@@ -1944,6 +1971,169 @@ int gbkhip_batch_memory(gbkhip_batch* b, unsigned long long* resident,
         *resident = r;
         *work = work_bytes(B);
         *allocated = B.work ? 1 : 0;
+    });
+}
+
+int gbkhip_batch_set_tiles(gbkhip_batch* b, int lanes, int ysplit)
+{
+    return guarded([&] {
+        GBK_CHECK(b != nullptr, GBKHIP_ERR_ARG, "null batch");
+        GBK_CHECK(lanes == 1 || lanes == 2, GBKHIP_ERR_ARG,
+                  "lanes must be 1 or 2");
+        GBK_CHECK(ysplit == 1 || ysplit == 2 || ysplit == 4, GBKHIP_ERR_ARG,
+                  "ysplit must be 1, 2 or 4");
+        Batch& B = b->impl;
+        GBK_HIP(hipSetDevice(B.device));
+        if (!B.sep.on) {
+            GBK_CHECK(lanes == 1 && ysplit == 1, GBKHIP_ERR_UNSUPPORTED,
+                      "tile settings apply to the direct pipeline");
+            return;
+        }
+        std::lock_guard<std::mutex> lk(B.ctx->mu);
+        const int jy = sep_jy(B.sep.ty);
+        const int groups = B.sep.yblk_full / jy;
+        B.sep.yblk = ceil_div(groups, ysplit) * jy;
+        if (lanes != B.lanes) {
+            release_work(B);          // re-allocated for the new lane count
+            B.lanes = lanes;
+            if (lanes == 2) ensure_lane_streams(B);
+        }
+        drop_graphs(B);
+    });
+}
+
+int gbkhip_group_create(gbkhip_ctx* ctx, gbkhip_batch* const* members, int n,
+                        gbkhip_group** out)
+{
+    return guarded([&] {
+        Context* c = ctx_of(ctx);
+        GBK_CHECK(out && members && n >= 1, GBKHIP_ERR_ARG,
+                  "need at least one member");
+        GBK_HIP(hipSetDevice(c->device));
+        auto* g = new gbkhip_group();
+        try {
+            g->ctx = c;
+            for (int i = 0; i < n; ++i) {
+                GBK_CHECK(members[i] != nullptr, GBKHIP_ERR_ARG,
+                          "null member");
+                GBK_CHECK(members[i]->impl.ctx == c, GBKHIP_ERR_ARG,
+                          "members must share the group's context");
+                g->members.push_back(members[i]);
+            }
+            const int k = std::min(n, kGroupBranches);
+            for (int i = 0; i < k; ++i) {
+                hipStream_t s;
+                GBK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+                g->branch.push_back(s);
+                hipEvent_t e;
+                GBK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                g->ev_join.push_back(e);
+            }
+            GBK_HIP(hipEventCreateWithFlags(&g->ev_fork,
+                                            hipEventDisableTiming));
+            GBK_HIP(hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking));
+        } catch (...) {
+            delete g;
+            throw;
+        }
+        *out = g;
+    });
+}
+
+int gbkhip_group_destroy(gbkhip_group* g)
+{
+    return guarded([&] {
+        if (!g) return;
+        GBK_HIP(hipSetDevice(g->ctx->device));
+        for (auto s : g->branch) GBK_HIP(hipStreamSynchronize(s));
+        delete g;
+    });
+}
+
+int gbkhip_group_evaluate(gbkhip_group* g, const float* const* params,
+                          const int* const* csum,
+                          const double* const* qscale, const int* nsets,
+                          double* const* chi2, float* const* cubes,
+                          int use_graph)
+{
+    return guarded([&] {
+        GBK_CHECK(g && params && nsets && chi2, GBKHIP_ERR_ARG,
+                  "null argument");
+        Context& c = *g->ctx;
+        GBK_HIP(hipSetDevice(c.device));
+        const int n = (int)g->members.size();
+        for (int i = 0; i < n; ++i) {
+            const Batch& B = g->members[i]->impl;
+            GBK_CHECK(nsets[i] >= 0, GBKHIP_ERR_ARG, "negative set count");
+            GBK_CHECK(nsets[i] == 0 || (params[i] && chi2[i]),
+                      GBKHIP_ERR_ARG, "null params or chi2");
+            GBK_CHECK(!B.desc.mcdisk || nsets[i] == 0 ||
+                          (csum && qscale && csum[i] && qscale[i]),
+                      GBKHIP_ERR_ARG, "mcdisk member without cloud budgets");
+            GBK_CHECK(!B.timing, GBKHIP_ERR_ARG,
+                      "member has stage timing on");
+        }
+        std::lock_guard<std::mutex> lk(c.mu);
+        const hipStream_t target = c.stream;
+        struct Restore {
+            Context& c;
+            hipStream_t s;
+            ~Restore() { c.stream = s; }
+        } restore{c, target};
+        for (auto* m : g->members) alloc_work(m->impl);
+        const int k = (int)g->branch.size();
+        auto launch_all = [&](hipStream_t main) {
+            GBK_HIP(hipEventRecord(g->ev_fork, main));
+            for (int j = 0; j < k; ++j)
+                GBK_HIP(hipStreamWaitEvent(g->branch[j], g->ev_fork, 0));
+            for (int i = 0; i < n; ++i) {
+                if (nsets[i] == 0) continue;
+                Batch& B = g->members[i]->impl;
+                c.stream = g->branch[i % k];
+                B.csum = csum ? csum[i] : nullptr;
+                B.qscale = qscale ? qscale[i] : nullptr;
+                run_all(B, params[i], nsets[i], chi2[i],
+                        cubes ? cubes[i] : nullptr);
+            }
+            c.stream = main;
+            for (int j = 0; j < k; ++j) {
+                GBK_HIP(hipEventRecord(g->ev_join[j], g->branch[j]));
+                GBK_HIP(hipStreamWaitEvent(main, g->ev_join[j], 0));
+            }
+        };
+        if (!use_graph) {
+            launch_all(target);
+            return;
+        }
+        std::vector<uintptr_t> key;
+        for (int i = 0; i < n; ++i) {
+            key.push_back((uintptr_t)params[i]);
+            key.push_back((uintptr_t)nsets[i]);
+            key.push_back((uintptr_t)chi2[i]);
+            key.push_back(cubes ? (uintptr_t)cubes[i] : 0);
+            key.push_back(csum ? (uintptr_t)csum[i] : 0);
+            key.push_back(qscale ? (uintptr_t)qscale[i] : 0);
+        }
+        auto it = g->graphs.find(key);
+        if (it == g->graphs.end()) {
+            GBK_HIP(hipStreamSynchronize(target));
+            hipGraph_t graph = nullptr;
+            GBK_HIP(hipStreamBeginCapture(g->cap,
+                                          hipStreamCaptureModeThreadLocal));
+            try {
+                launch_all(g->cap);
+            } catch (...) {
+                (void)hipStreamEndCapture(g->cap, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            GBK_HIP(hipStreamEndCapture(g->cap, &graph));
+            hipGraphExec_t exec = nullptr;
+            GBK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            GBK_HIP(hipGraphDestroy(graph));
+            it = g->graphs.emplace(key, exec).first;
+        }
+        GBK_HIP(hipGraphLaunch(it->second, target));
     });
 }
 

@@ -301,6 +301,16 @@ class GpuGalaxy:
                                         dtype=torch.float32, pin_memory=True)
             self._orders_dev = torch.tensor(self.orders, dtype=torch.int32,
                                             device=dev)
+        if self.ev.mc:               # per-set cloud budgets (mcdisk)
+            L = self.ev.desc.csum_len
+            self._csum = torch.empty((max_batch, L), dtype=torch.int32,
+                                     device=dev)
+            self._qscale = torch.empty(max_batch, dtype=torch.float64,
+                                       device=dev)
+            self._h_csum = torch.empty((max_batch, L), dtype=torch.int32,
+                                       pin_memory=True)
+            self._h_qscale = torch.empty(max_batch, dtype=torch.float64,
+                                         pin_memory=True)
         self._streams = streams
         self.ev.release()            # admission allocates the work buffers
         self._allocated = False
@@ -322,14 +332,26 @@ class GpuGalaxy:
         stream = self._streams[slot]
         host = self.ev.packer.pack(params_list)
         self._h_params[:n].copy_(t.from_numpy(host))
+        if self.ev.mc:
+            csum, qs = self.ev.cloud_budgets(params_list)
+            self._h_csum[:n].copy_(t.from_numpy(csum))
+            self._h_qscale[:n].copy_(t.from_numpy(qs))
+        cubes = self._cubes.data_ptr() if self._cubes is not None else None
         with t.cuda.stream(stream):
             self._params[:n].copy_(self._h_params[:n], non_blocking=True)
             self.ev.set_stream(stream)
             from . import _native as N
-            N.call('gbkhip_batch_evaluate', self.ev.handle,
-                   self._params.data_ptr(), n, self._chi2.data_ptr(),
-                   self._cubes.data_ptr() if self._cubes is not None else None,
-                   1)
+            if self.ev.mc:
+                self._csum[:n].copy_(self._h_csum[:n], non_blocking=True)
+                self._qscale[:n].copy_(self._h_qscale[:n], non_blocking=True)
+                N.call('gbkhip_batch_evaluate_clouds', self.ev.handle,
+                       self._params.data_ptr(), self._csum.data_ptr(),
+                       self._qscale.data_ptr(), n, self._chi2.data_ptr(),
+                       cubes, 1)
+            else:
+                N.call('gbkhip_batch_evaluate', self.ev.handle,
+                       self._params.data_ptr(), n, self._chi2.data_ptr(),
+                       cubes, 1)
             self._allocated = True
             if self.orders is not None:
                 be = self.driver.backends().dmodel(np.float32)

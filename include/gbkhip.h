@@ -262,6 +262,31 @@ int gbkhip_batch_evaluate_clouds(gbkhip_batch* batch, const float* params,
  * clouds of flux cflux on grid: the value gbkhip_mcdisk_evaluate uses. */
 int gbkhip_mcdisk_fixed_scale(float cflux, int nclouds,
                               const gbkhip_grid* grid, double* out);
+/* Tile settings of the direct separable pipeline (the autotuner's knobs;
+ * other pipelines accept only 1, 1): lanes = chunks in flight on parallel
+ * streams (1 or 2), ysplit = blocks the Y outputs of a y-pass column strip
+ * are split over (1, 2 or 4).  Takes effect on the next evaluate(). */
+int gbkhip_batch_set_tiles(gbkhip_batch* batch, int lanes, int ysplit);
+
+/* ---- Ragged groups: galaxies of different cube sizes, models and data
+ *      (each a batch of its own) evaluated in one launch -- one HIP graph
+ *      per set of call pointers, members spread over parallel streams.  No
+ *      counterpart in the reference, which evaluates one model per call. */
+typedef struct gbkhip_group gbkhip_group;
+int gbkhip_group_create(gbkhip_ctx* ctx, gbkhip_batch* const* members, int n,
+                        gbkhip_group** out);
+int gbkhip_group_destroy(gbkhip_group* group);
+/* Member i evaluates nsets[i] sets from params[i] into chi2[i] (and its model
+ * cubes into cubes[i] when cubes and cubes[i] are not NULL); csum/qscale
+ * hold the cloud budgets of mcdisk members (NULL entries for the others,
+ * NULL arrays when there are none).  Each member's results equal those of
+ * its own gbkhip_batch_evaluate(_clouds). */
+int gbkhip_group_evaluate(gbkhip_group* group, const float* const* params,
+                          const int* const* csum,
+                          const double* const* qscale, const int* nsets,
+                          double* const* chi2, float* const* cubes,
+                          int use_graph);
+
 /* Stage timing: with timing on, evaluate() launches directly (no graph) and
  * records events around the smdisk, convolution and downscale+chi^2 stages
  * of every chunk; stage_times() waits for them and returns the summed stage
