@@ -37,7 +37,9 @@ A ragged leg (skip with --no-ragged) autotunes 32 galaxies of log-uniform
 sizes from 64x64x32 to 512x512x256 (smdisk / mcdisk with 2^20..2^24 clouds,
 2x or 3x supersampling, random PSF / LSF widths, 2/3/5-smooth padding) and
 times one graph launch evaluating one set of every galaxy, plus the moment
-maps of all of them: the `ragged` object.
+maps of all of them: the `ragged` object.  A plain mcdisk leg (skip with
+--no-mcdisk) times the one-model path on 256x256x128 with 2^24 clouds: the
+`mcdisk_plain` object.
 
 Prints one JSON line.
 """
@@ -325,6 +327,52 @@ def ragged_benchmark(drv, ngal, reps, seed=2024, budget_frac=0.6):
                             for m in metas])
 
 
+def mcdisk_benchmark(drv, reps, nclouds=1 << 24):
+    """Plain mcdisk (one model per call): 256x256x128 cube, 2^24 clouds of
+    an exponential (h = 20 px) x sech^2 (z0 = 2 px) disk, lramp rotation
+    curve (vmax 220 km/s, rt 15 px), dispersion 15 km/s, PSF FWHM 3 px, LSF
+    FWHM 2 channels, moment maps 0/1/2; wall time per model evaluation and
+    cloud rate."""
+    import torch
+    from gbkfit_amd import gmodels, psflsf
+    from gbkfit_amd.dmodels import DModelMMaps
+    from gbkfit_amd.objective import Model
+    from gbkfit_amd.traits import make
+    rn = np.linspace(0, 120, 41)
+    truth = dict(xpos=0.0, ypos=0.0, vsys=0.0, posa=30.0, incl=50.0,
+                 bpt_a=1.0, bpt_s=20.0, bht_s=2.0, vpt_rt=15.0,
+                 vpt_vt=220.0, dpt_a=15.0)
+
+    def model(cflux):
+        cmp = gmodels.SpectralMCDisk3D(
+            cflux, False, False, make('bpt', 'exponential'),
+            make('vpt', 'tan_lramp'), make('dpt', 'uniform'),
+            make('bht', 'sech2'), rnodes=rn)
+        gm = gmodels.GModelKinematics3D([cmp], size_z=32)
+        dm = DModelMMaps((256, 256, 128), (1, 1, 10),
+                         psf=psflsf.PSFGauss(3.0 * SIG),
+                         lsf=psflsf.LSFGauss(20.0 * SIG), orders=(0, 1, 2))
+        return cmp, Model(drv, dm, gm)
+    cmp, _ = model(1e-3)
+    total = float(cmp.disk.cloud_budget(truth)[0][-1]) * 1e-3
+    cmp, m = model(total / nclouds)
+    ncl = int(cmp.disk.cloud_budget(truth)[0][-1])
+    for _ in range(2):
+        m.model_d(truth)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t1 = time.perf_counter()
+        m.model_d(truth)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t1)
+    ms = 1e3 * float(np.median(ts))
+    return dict(workload='mcdisk 256x256x128, 2^24 clouds, PSF + LSF, '
+                         'moment maps 0/1/2', clouds=ncl, reps=reps,
+                ms_per_eval=ms, clouds_per_s=ncl / (ms * 1e-3),
+                **percentiles(ts))
+
+
 def percentiles(xs):
     xs = np.asarray(xs, np.float64) * 1e3
     return dict(p50_ms=float(np.percentile(xs, 50)),
@@ -384,15 +432,19 @@ def cpu_baseline(seconds, thick=False):
                        f"is absent)")
 
 
-def algorithmic_bytes(ev, info):
-    """Compulsory HBM bytes per parameter set of each kernel (or stage)."""
+def algorithmic_bytes(ev, info, live=1.0):
+    """Compulsory HBM bytes per parameter set of each kernel (or stage).
+    Direct pipeline: `live` is the fraction of x-pass tiles (row x channel
+    chunk) that some spectral line reaches; the others are skipped, neither
+    written by the x pass nor read by the y pass (live = 1: the dense
+    figure)."""
     nxl, nyl, nzl = ev.size_lo
     sx, sy, _ = ev.desc.scale
     X, Y = nxl * sx, nyl * sy
     n_lo = nxl * nyl * nzl
     if info['pipeline'] == 'direct':
         nx, ny, nz = info['fft_size']        # inputs that reach the outputs
-        p1 = 4 * nz * ny * nxl               # x-filtered rows
+        p1 = 4 * nz * ny * nxl * live        # x-filtered rows (live tiles)
         p2 = 4 * nz * nyl * nxl              # x- and y-filtered columns
         return {
             'batch_lines_thin': 16 * nx * ny,
@@ -417,17 +469,18 @@ def algorithmic_bytes(ev, info):
             'downscale_chi2': 4 * X * Y * nzl + 4 * n_lo}
 
 
-def algorithmic_flops(ev, info):
+def algorithmic_flops(ev, info, live=1.0):
     """FP32 flops per parameter set of the direct pipeline's filter passes:
     one multiply-add per tap of the block-mean-folded kernel (s + 2 edge
-    taps along each axis), zero-padded taps not counted."""
+    taps along each axis), zero-padded taps not counted; the x pass only
+    over its live tiles (see algorithmic_bytes)."""
     if info['pipeline'] != 'direct':
         return {}
     nxl, nyl, nzl = ev.size_lo
     nx, ny, nz = info['fft_size']
     s, e = ev.desc.scale, ev.desc.edge_hi
     tx, ty, tz = (s[i] + 2 * e[i] for i in range(3))
-    return {'sep_xpass': 2 * nz * ny * nxl * tx,
+    return {'sep_xpass': 2 * nz * ny * nxl * tx * live,
             'sep_ypass': 2 * nz * nyl * nxl * ty,
             'sep_zpass': 2 * nzl * nyl * nxl * tz}
 
@@ -466,6 +519,7 @@ def main():
     ap.add_argument('--ragged-galaxies', type=int, default=32)
     ap.add_argument('--ragged-reps', type=int, default=5)
     ap.add_argument('--no-ragged', action='store_true')
+    ap.add_argument('--no-mcdisk', action='store_true')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only copy a 1 GiB device buffer 3 times (known '
                          'bytes for scaling FETCH_SIZE / WRITE_SIZE)')
@@ -568,8 +622,17 @@ def main():
     # one chunk) over its average launch time from the HIP events above
     launches = args.steps * max(chunks, 1)
     per_chunk = args.nsets / max(chunks, 1)
-    algo = algorithmic_bytes(ev, info)
-    aflops = algorithmic_flops(ev, info)
+    # the last timed step ran one lane: its tile statistics are the last
+    # chunk's (64 sets of the workload)
+    live_tiles = None
+    if info['pipeline'] == 'direct':
+        nzt, tot = ev.tile_stats()
+        live_tiles = nzt / tot if tot else None
+    live = live_tiles if live_tiles is not None else 1.0
+    algo = algorithmic_bytes(ev, info, live)
+    aflops = algorithmic_flops(ev, info, live)
+    algo_dense = algorithmic_bytes(ev, info)
+    flops_dense = algorithmic_flops(ev, info)
     kernels = {}
     for k, t in kern_ms.items():
         avg_ms = t / launches
@@ -579,6 +642,10 @@ def main():
             gbs = b * per_chunk / (avg_ms * 1e-3) / 1e9
             kernels[k].update(bytes_per_set=b, achieved_gbs=gbs,
                               frac=gbs / HBM_PEAK_GBS)
+        if k in algo_dense and algo_dense[k] != b:
+            kernels[k]['dense_bytes_per_set'] = algo_dense[k]
+        if k in flops_dense and flops_dense[k] != aflops.get(k):
+            kernels[k]['dense_flops_per_set'] = flops_dense[k]
         f = aflops.get(k)
         if f is not None and avg_ms > 0:
             tfs = f * per_chunk / (avg_ms * 1e-3) / 1e12
@@ -603,6 +670,12 @@ def main():
                         unit='GB/s',
                         frac=(achieved / HBM_PEAK_GBS) if achieved else None,
                         traffic=traffic, kernel=dom)
+    if live_tiles is not None:
+        roofline['live_tile_fraction'] = live_tiles
+        roofline['note'] = (roofline.get('note', '') + '; ' if
+                            'note' in roofline else '') + (
+            'achieved counts the work of the x-pass tiles some line reaches '
+            '(live_tile_fraction of all); the others are skipped')
 
     serving = None
     ev.close()
@@ -610,6 +683,11 @@ def main():
         serving = serve_benchmark(drv, args.serve_galaxies,
                                   args.serve_seconds, args.serve_rate)
         print(f"serving leg done: {serving['throughput']:.0f} req/s",
+              file=sys.stderr, flush=True)
+    mcdisk = None
+    if not args.no_mcdisk:
+        mcdisk = mcdisk_benchmark(drv, 10)
+        print(f"mcdisk leg done: {mcdisk['ms_per_eval']:.1f} ms per eval",
               file=sys.stderr, flush=True)
     ragged = None
     if not args.no_ragged:
@@ -653,6 +731,7 @@ def main():
         roofline=dict(roofline, sets_per_launch=per_chunk),
         serving=serving,
         ragged=ragged,
+        mcdisk_plain=mcdisk,
         cpu_baseline=cpu,
         host=dict(cpu=platform.processor() or platform.machine(),
                   nproc=os.cpu_count()),

@@ -59,6 +59,8 @@ N.register_signatures({
                                    _c.POINTER(_c.c_double)], _c.c_int),
     'gbkhip_batch_info': ([_c.c_void_p, _c.POINTER(_c.c_int)], _c.c_int),
     'gbkhip_batch_set_tiles': ([_c.c_void_p, _c.c_int, _c.c_int], _c.c_int),
+    'gbkhip_batch_tile_stats': ([_c.c_void_p, _c.POINTER(_c.c_longlong),
+                                 _c.POINTER(_c.c_longlong)], _c.c_int),
     'gbkhip_group_create': ([_c.c_void_p, _c.POINTER(_c.c_void_p), _c.c_int,
                              _c.POINTER(_c.c_void_p)], _c.c_int),
     'gbkhip_group_destroy': ([_c.c_void_p], _c.c_int),
@@ -383,6 +385,14 @@ class BatchEvaluator:
         a y-pass column strip's outputs over blocks (1, 2 or 4)."""
         N.call('gbkhip_batch_set_tiles', self.handle, int(lanes), int(ysplit))
         self.tiles = (int(lanes), int(ysplit))
+
+    def tile_stats(self):
+        """Direct pipeline: (nonzero, total) x-pass tiles of the last chunk
+        on the first lane; tiles no line reaches are skipped."""
+        a, b = _c.c_longlong(), _c.c_longlong()
+        N.call('gbkhip_batch_tile_stats', self.handle, _c.byref(a),
+               _c.byref(b))
+        return a.value, b.value
 
     def set_timing(self, enable):
         N.call('gbkhip_batch_set_timing', self.handle, int(bool(enable)))

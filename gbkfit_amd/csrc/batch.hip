@@ -540,11 +540,14 @@ __global__ __launch_bounds__(256) void batch_mcdisk(
     const double* __restrict__ qscale, unsigned long long* icube,
     long long n_hi)
 {
+    extern __shared__ int s_csum[];
+    __shared__ float s_nodes[kMcNodesLds];
     const int set = blockIdx.y;
-    const gbkhip_disk d = set_disk(tmpl, lay, params + (size_t)set * pstride);
+    gbkhip_disk d = set_disk(tmpl, lay, params + (size_t)set * pstride);
+    d.rnodes = mc_stage_nodes(d.rnodes, d.nrnodes, s_nodes);
     McParams mp;
     mp.cflux = cflux;
-    mp.csum = csum + (size_t)set * csum_len;
+    mp.csum = mc_stage_csum(csum + (size_t)set * csum_len, csum_len, s_csum);
     mp.csum_len = csum_len;
     mp.nclouds = mp.csum[csum_len - 1];
     mp.hoi = hoi;
@@ -608,6 +611,7 @@ constexpr int kSepZC = 16;          // most channels per S1 block
 constexpr int kSepJZ = 16;          // outputs per thread, S3
 constexpr int kSepXW = 32;          // X columns per S2 block
 constexpr int kSepYPitch = kSepXW + 1;
+constexpr int kSepMaxRows = (48 * 1024) / (kSepYPitch * 4);   // S2 LDS rows
 
 // outputs per thread of S1 and S2 (fewer for wide taps: registers)
 __host__ __device__ constexpr int sep_jx(int taps) { return taps <= 64 ? 8 : 4; }
@@ -630,7 +634,7 @@ __global__ __launch_bounds__(256) void sep_xpass(
     const float4* __restrict__ lines, long long lstride, int lpitch,
     gbkhip_grid g, const float* __restrict__ wx, int need_x, int need_z,
     int nxl, int mp, int zc, float* __restrict__ P1, long long p1stride,
-    int ny1)
+    int ny1, unsigned char* __restrict__ rowflag)
 {
     extern __shared__ float4 M4[];        // [zc][mp] floats
     float* M = reinterpret_cast<float*>(M4);
@@ -646,6 +650,7 @@ __global__ __launch_bounds__(256) void sep_xpass(
     __syncthreads();
     // lines in form 1 (batch_lines_thin): amplitude and window precomputed
     const float4* lrow = lines + set * lstride + (size_t)y * lpitch;
+    int any = 0;
     for (int x = threadIdx.x; x < need_x; x += T) {
         const float4 V = lrow[x];
         if (V.x != 0.0f) {
@@ -659,9 +664,16 @@ __global__ __launch_bounds__(256) void sep_xpass(
             e.zero = g.spec_zero;
             e.step = g.spec_step;
             for (int k = k0; k <= k1; ++k) M[(k - z0) * mp + x] = e.at(k);
+            any |= k0 <= k1;
         }
     }
-    __syncthreads();
+    // a tile no line reaches filters to zeros: flag it for the y pass
+    // (which then reads zeros instead of the row) and skip it
+    const int nonzero = __syncthreads_or(any);
+    if (threadIdx.x == 0)
+        rowflag[((size_t)set * gridDim.y + blockIdx.y) * ny1 + y] =
+            (unsigned char)(nonzero != 0);
+    if (!nonzero) return;
     // outputs (z, X .. X + J - 1): inputs sx X + [0, sx (J - 1) + TX).
     // Lane l of a wave takes channel l % 16; its four 16-lane groups take
     // output blocks DXB apart, whose inputs start 16 banks apart (SX J DXB
@@ -710,7 +722,8 @@ template <int SY, int TY>
 __global__ __launch_bounds__(256) void sep_ypass(
     const float* __restrict__ P1, long long p1stride, const float* wy,
     int nxl, int nyl, int ny1, int need_z, int yblk,
-    float* __restrict__ P2, long long p2stride)
+    float* __restrict__ P2, long long p2stride,
+    const unsigned char* __restrict__ rowflag, int zc, int nchunks)
 {
     constexpr int T = kSepThreads, J = sep_jy(TY), XW = kSepXW;
     constexpr int PITCH = kSepYPitch, G = T / XW;
@@ -721,8 +734,17 @@ __global__ __launch_bounds__(256) void sep_ypass(
     const int z = blockIdx.z % need_z, set = blockIdx.z / need_z;
     const int ngrp = (min(yblk, nyl - Y0) + J - 1) / J;
     const int nrows = SY * (ngrp * J - 1) + TY;
+    __shared__ unsigned char s_flag[kSepMaxRows];
     for (int i = threadIdx.x; i < TY; i += T) s_w[i] = wy[i];
     const float* src = P1 + set * p1stride + (size_t)z * ny1 * nxl;
+    // rows the x pass flagged as all-zero are not read (never written)
+    const unsigned char* fl =
+        rowflag + ((size_t)set * nchunks + z / zc) * ny1;
+    for (int r = threadIdx.x; r < nrows; r += T) {
+        const int y = SY * Y0 + r;
+        s_flag[r] = y < ny1 ? fl[y] : 0;
+    }
+    __syncthreads();
     // staging: U independent loads in flight per thread before the stores
     constexpr int U = 8;
     if ((nxl & 3) == 0 && X0 + XW <= nxl) {
@@ -735,7 +757,7 @@ __global__ __launch_bounds__(256) void sep_ypass(
             for (int u = 0; u < U; ++u) {
                 const int i = i0 + u * T, r = i / Q, q = i - r * Q;
                 const int y = SY * Y0 + r;
-                t[u] = (i < n4 && y < ny1)
+                t[u] = (i < n4 && s_flag[r])
                            ? *reinterpret_cast<const float4*>(
                                  src + (size_t)y * nxl + X0 + 4 * q)
                            : make_float4(0, 0, 0, 0);
@@ -760,7 +782,7 @@ __global__ __launch_bounds__(256) void sep_ypass(
             for (int u = 0; u < U; ++u) {
                 const int i = i0 + u * T, r = i / XW, c = i - r * XW;
                 const int y = SY * Y0 + r, X = X0 + c;
-                t[u] = (i < n1 && y < ny1 && X < nxl)
+                t[u] = (i < n1 && s_flag[r] && X < nxl)
                            ? src[(size_t)y * nxl + X] : 0.0f;
             }
 #pragma unroll
@@ -865,6 +887,9 @@ struct Sep {
     int yblk_full = 0;            // ... when one block covers every Y
     DevBuf wx, wy, wz;            // taps
     DevBuf P1, P2;                // work: per set (z, y, X) and (z, Y, X)
+    DevBuf flags;                 // work: per set, nonzero x-pass tiles
+    int nchunks = 0;              // S1 channel chunks
+    int last_sets = 0;            // sets of the last chunk run on lane 0
     long long p1stride = 0, p2stride = 0;
     int zblocks = 1;
 };
@@ -1011,6 +1036,7 @@ static size_t work_bytes(const Batch& B)
     if (B.sep.on) {
         n += mb * (size_t)B.sep.need[1] * B.sep.need[0] * sizeof(float4);
         n += mb * (size_t)(B.sep.p1stride + B.sep.p2stride) * sizeof(float);
+        n += mb * (size_t)B.sep.nchunks * B.sep.need[1];
     } else if (B.fused.on) {
         n += mb * (size_t)B.fused.sstride * sizeof(float2);
         n += mb * (size_t)B.fused.lstride * sizeof(float4);
@@ -1038,6 +1064,7 @@ static void alloc_work(Batch& B)
                             sizeof(float4));
         B.sep.P1.alloc(mb * (size_t)B.sep.p1stride * sizeof(float));
         B.sep.P2.alloc(mb * (size_t)B.sep.p2stride * sizeof(float));
+        B.sep.flags.alloc(mb * (size_t)B.sep.nchunks * B.sep.need[1]);
     } else if (B.fused.on) {
         B.fused.S.alloc(mb * B.fused.sstride * sizeof(float2));
         B.fused.lines.alloc(mb * B.fused.lstride * sizeof(float4));
@@ -1065,6 +1092,7 @@ static void release_work(Batch& B)
     B.fused.lines.release();
     B.sep.P1.release();
     B.sep.P2.release();
+    B.sep.flags.release();
     B.hi.release();
     B.spec.release();
     B.icube.release();
@@ -1444,8 +1472,9 @@ static bool plan_sep(Batch& B)
     if (zmax < 1) return false;
     const int nchunks = (P.need[2] + zmax - 1) / zmax;
     P.zc = (P.need[2] + nchunks - 1) / nchunks;
+    P.nchunks = (P.need[2] + P.zc - 1) / P.zc;
     // Y outputs per S2 block: all of them when the rows fit 48 KB
-    const int max_rows = (48 * 1024) / (kSepYPitch * 4);
+    const int max_rows = kSepMaxRows;
     const int gmax = ((max_rows - P.ty) / sy + 1) / jy;
     if (gmax < 1) return false;
     P.yblk = std::min((nyl + jy - 1) / jy, gmax) * jy;
@@ -1528,6 +1557,9 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
     float* P1 = P.P1.as<float>() + mb * P.p1stride;
     float* P2 = P.P2.as<float>() + mb * P.p2stride;
     double* partials = b.partials.as<double>() + mb * b.nparts;
+    unsigned char* flags = P.flags.as<unsigned char>() +
+                           mb * (size_t)P.nchunks * ny;
+    if (lane == 0) b.sep.last_sets = nsets;
     {
         dim3 g0(ceil_div(nx, 64), ceil_div(ny, 4), nsets);
         hipLaunchKernelGGL(batch_lines_thin, g0, dim3(64, 4), 0, c.stream,
@@ -1542,7 +1574,7 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
         GBK_SEP_DISPATCH(sep_xpass, D.scale[0], P.tx, grid, dim3(kSepThreads),
                          lds, c.stream, lines, lstride, nx, D.grid,
                          P.wx.as<float>(), nx, nz, nxl, P.mp, P.zc,
-                         P1, P.p1stride, ny);
+                         P1, P.p1stride, ny, flags);
         GBK_HIP(hipGetLastError());
     }
     mark(b, ci, 2);
@@ -1553,7 +1585,7 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
         GBK_SEP_DISPATCH(sep_ypass, D.scale[1], P.ty, grid, dim3(kSepThreads),
                          lds, c.stream, P1, P.p1stride,
                          P.wy.as<float>(), nxl, nyl, ny, nz, P.yblk,
-                         P2, P.p2stride);
+                         P2, P.p2stride, flags, P.zc, P.nchunks);
         GBK_HIP(hipGetLastError());
     }
     mark(b, ci, 3);
@@ -1600,7 +1632,9 @@ static void run_chunk(Batch& b, const float* params, int set0, int nsets,
     float* hi = b.hi.as<float>();
     if (D.mcdisk) {
         // the same 256-block x 256-thread cloud loop for every set
-        hipLaunchKernelGGL(batch_mcdisk, dim3(256, nsets), dim3(256), 0,
+        const size_t lds = D.csum_len <= kMcCsumLds
+                               ? D.csum_len * sizeof(int) : 0;
+        hipLaunchKernelGGL(batch_mcdisk, dim3(256, nsets), dim3(256), lds,
                            c.stream, b.tmpl, b.lay, P, D.pstride, g, D.cflux,
                            b.csum + (size_t)set0 * D.csum_len, D.csum_len,
                            D.hoi, (uint32_t)D.seed, (uint32_t)(D.seed >> 32),
@@ -2134,6 +2168,29 @@ int gbkhip_group_evaluate(gbkhip_group* g, const float* const* params,
             it = g->graphs.emplace(key, exec).first;
         }
         GBK_HIP(hipGraphLaunch(it->second, target));
+    });
+}
+
+int gbkhip_batch_tile_stats(gbkhip_batch* b, long long* nonzero,
+                            long long* total)
+{
+    return guarded([&] {
+        GBK_CHECK(b && nonzero && total, GBKHIP_ERR_ARG, "null argument");
+        Batch& B = b->impl;
+        *nonzero = *total = 0;
+        if (!B.sep.on || !B.work || B.sep.last_sets == 0) return;
+        GBK_HIP(hipSetDevice(B.device));
+        const hipStream_t s = B.stream ? B.stream : B.ctx->stream;
+        GBK_HIP(hipStreamSynchronize(s));
+        const size_t n = (size_t)B.sep.last_sets * B.sep.nchunks *
+                         B.sep.need[1];
+        std::vector<unsigned char> h(n);
+        GBK_HIP(hipMemcpy(h.data(), B.sep.flags.ptr, n,
+                          hipMemcpyDeviceToHost));
+        long long c = 0;
+        for (unsigned char v : h) c += v != 0;
+        *nonzero = c;
+        *total = (long long)n;
     });
 }
 

@@ -14,6 +14,10 @@ __global__ __launch_bounds__(256) void mcdisk_kernel(
     McParams mp, gbkhip_disk d, gbkhip_grid g, gbkhip_outputs o,
     unsigned long long* icube)
 {
+    extern __shared__ int s_csum[];
+    __shared__ float s_nodes[kMcNodesLds];
+    mp.csum = mc_stage_csum(mp.csum, mp.csum_len, s_csum);
+    d.rnodes = mc_stage_nodes(d.rnodes, d.nrnodes, s_nodes);
     for (long long ci = blockIdx.x * (long long)blockDim.x + threadIdx.x;
          ci < mp.nclouds; ci += (long long)gridDim.x * blockDim.x)
         mc_cloud(ci, mp, d, g, o, icube);
@@ -86,7 +90,8 @@ void launch_mcdisk(Context& c, float cflux, int nclouds, const int* csum,
     mp.key1 = (uint32_t)(seed >> 32);
     mp.qscale = mcdisk_fixed_scale(cflux, nclouds, g);
     const int blocks = std::max(1, std::min(ceil_div(nclouds, 256), 8192));
-    hipLaunchKernelGGL(mcdisk_kernel, dim3(blocks), dim3(256), 0, c.stream,
+    const size_t lds = csum_len <= kMcCsumLds ? csum_len * sizeof(int) : 0;
+    hipLaunchKernelGGL(mcdisk_kernel, dim3(blocks), dim3(256), lds, c.stream,
                        mp, d, g, o, icube);
     GBK_HIP(hipGetLastError());
     if (icube) {

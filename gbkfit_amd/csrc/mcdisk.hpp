@@ -18,7 +18,7 @@ struct CloudRng {
     uint32_t key0, key1, cid;
     uint32_t block;
     uint32_t next;        // index of the next draw
-    uint32_t w[4];
+    uint32_t w0, w1, w2, w3;   // words of the current block (registers)
 };
 
 __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2,
@@ -43,9 +43,18 @@ __device__ __forceinline__ float rng_next(CloudRng& s)
     const uint32_t i = s.next++;
     if ((i >> 2) != s.block) {
         s.block = i >> 2;
-        philox(s.cid, s.block, 0u, 0u, s.key0, s.key1, s.w);
+        uint32_t w[4];
+        philox(s.cid, s.block, 0u, 0u, s.key0, s.key1, w);
+        s.w0 = w[0];
+        s.w1 = w[1];
+        s.w2 = w[2];
+        s.w3 = w[3];
     }
-    return ((float)(s.w[i & 3u] >> 9) + 0.5f) * (1.0f / 8388608.0f);
+    // word i % 4 by selects, not an indexed array (which would live in
+    // scratch memory)
+    const uint32_t j = i & 3u;
+    const uint32_t w = j == 0 ? s.w0 : j == 1 ? s.w1 : j == 2 ? s.w2 : s.w3;
+    return ((float)(w >> 9) + 0.5f) * (1.0f / 8388608.0f);
 }
 
 // Samplers (math.hpp:152-498) in the reference's draw order.
@@ -107,6 +116,33 @@ struct McParams {
     uint32_t key0, key1;
     double qscale;          // fixed-point scale 2^S
 };
+
+// The cumulative cloud counts are read by every cloud's ring search (a
+// linear scan, as gmodels.hpp:401-416): a block stages them in LDS once
+// when they fit.  Every thread of the block calls this (it syncs).
+constexpr int kMcCsumLds = 8192;
+
+__device__ __forceinline__ const int* mc_stage_csum(const int* csum, int len,
+                                                    int* lds)
+{
+    if (len > kMcCsumLds) return csum;
+    for (int i = threadIdx.x; i < len; i += blockDim.x) lds[i] = csum[i];
+    __syncthreads();
+    return lds;
+}
+
+// Sub-ring nodes, read several times per cloud (ring bounds, disk_info,
+// interpolation): staged in LDS as well when they fit.
+constexpr int kMcNodesLds = 1024;
+
+__device__ __forceinline__ const float* mc_stage_nodes(const float* nodes,
+                                                      int n, float* lds)
+{
+    if (n > kMcNodesLds) return nodes;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = nodes[i];
+    __syncthreads();
+    return lds;
+}
 
 __device__ __forceinline__ void mc_cloud(long long ci, const McParams& mp,
                                          const gbkhip_disk& d,

@@ -287,6 +287,12 @@ int gbkhip_group_evaluate(gbkhip_group* group, const float* const* params,
                           double* const* chi2, float* const* cubes,
                           int use_graph);
 
+/* Direct pipeline: x-pass tiles (one hi-res row x one channel chunk of one
+ * set) of the last chunk evaluated on the first lane that some line reaches
+ * (*nonzero) out of all (*total); the others are skipped (their outputs are
+ * zero).  Waits for the batch's stream.  Zeros for other pipelines. */
+int gbkhip_batch_tile_stats(gbkhip_batch* batch, long long* nonzero,
+                            long long* total);
 /* Stage timing: with timing on, evaluate() launches directly (no graph) and
  * records events around the smdisk, convolution and downscale+chi^2 stages
  * of every chunk; stage_times() waits for them and returns the summed stage
