@@ -432,12 +432,15 @@ def cpu_baseline(seconds, thick=False):
                        f"is absent)")
 
 
-def algorithmic_bytes(ev, info, live=1.0):
+def algorithmic_bytes(ev, info, live=1.0, glive=1.0, blive=1.0):
     """Compulsory HBM bytes per parameter set of each kernel (or stage).
     Direct pipeline: `live` is the fraction of x-pass tiles (row x channel
     chunk) that some spectral line reaches; the others are skipped, neither
-    written by the x pass nor read by the y pass (live = 1: the dense
-    figure)."""
+    written by the x pass nor read by the y pass; `glive` the fraction of
+    y-pass output groups with a nonzero input, the only ones the y pass
+    writes and the z pass reads; `blive` the fraction of z-pass blocks with
+    a live input, the only ones that read the data (1, 1, 1: the dense
+    figures)."""
     nxl, nyl, nzl = ev.size_lo
     sx, sy, _ = ev.desc.scale
     X, Y = nxl * sx, nyl * sy
@@ -445,12 +448,12 @@ def algorithmic_bytes(ev, info, live=1.0):
     if info['pipeline'] == 'direct':
         nx, ny, nz = info['fft_size']        # inputs that reach the outputs
         p1 = 4 * nz * ny * nxl * live        # x-filtered rows (live tiles)
-        p2 = 4 * nz * nyl * nxl              # x- and y-filtered columns
+        p2 = 4 * nz * nyl * nxl * glive      # x- and y-filtered columns
         return {
             'batch_lines_thin': 16 * nx * ny,
             'sep_xpass': 16 * nx * ny + p1,
             'sep_ypass': p1 + p2,
-            'sep_zpass': p2 + 4 * n_lo,
+            'sep_zpass': p2 + 4 * n_lo * blive,
         }
     if info['pipeline'] == 'fused':
         nx, ny, nz = info['fft_size']
@@ -469,11 +472,11 @@ def algorithmic_bytes(ev, info, live=1.0):
             'downscale_chi2': 4 * X * Y * nzl + 4 * n_lo}
 
 
-def algorithmic_flops(ev, info, live=1.0):
+def algorithmic_flops(ev, info, live=1.0, glive=1.0):
     """FP32 flops per parameter set of the direct pipeline's filter passes:
     one multiply-add per tap of the block-mean-folded kernel (s + 2 edge
-    taps along each axis), zero-padded taps not counted; the x pass only
-    over its live tiles (see algorithmic_bytes)."""
+    taps along each axis), zero-padded taps not counted; the x and y passes
+    only over their live tiles / groups (see algorithmic_bytes)."""
     if info['pipeline'] != 'direct':
         return {}
     nxl, nyl, nzl = ev.size_lo
@@ -481,22 +484,26 @@ def algorithmic_flops(ev, info, live=1.0):
     s, e = ev.desc.scale, ev.desc.edge_hi
     tx, ty, tz = (s[i] + 2 * e[i] for i in range(3))
     return {'sep_xpass': 2 * nz * ny * nxl * tx * live,
-            'sep_ypass': 2 * nz * nyl * nxl * ty,
+            'sep_ypass': 2 * nz * nyl * nxl * ty * glive,
             'sep_zpass': 2 * nzl * nyl * nxl * tz}
 
 
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration),
-    or None when no summary for it is present."""
+    summaries (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel
+    calibration): a line 'traffic_bytes_per_launch <kernel>: <bytes>' in
+    profiles/*_traffic.txt, or None when none covers it."""
     name = kernel.split(' ')[0]
-    path = os.path.join(ROOT, 'profiles', f'{name}_c3_traffic.txt')
-    if not os.path.exists(path):
+    d = os.path.join(ROOT, 'profiles')
+    if not os.path.isdir(d):
         return None
-    with open(path) as f:
-        for line in f:
-            if line.startswith('traffic_bytes_per_launch:'):
-                return float(line.split(':', 1)[1])
+    for fn in sorted(os.listdir(d)):
+        if not fn.endswith('_traffic.txt'):
+            continue
+        with open(os.path.join(d, fn)) as f:
+            for line in f:
+                if line.startswith(f'traffic_bytes_per_launch {name}:'):
+                    return float(line.split(':', 1)[1])
     return None
 
 
@@ -523,6 +530,10 @@ def main():
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only copy a 1 GiB device buffer 3 times (known '
                          'bytes for scaling FETCH_SIZE / WRITE_SIZE)')
+    ap.add_argument('--atomic-calibration', action='store_true',
+                    help='only time 2^27 int64 atomic adds at random '
+                         'addresses of a 256x256x128 cube (the roof of the '
+                         'mcdisk deposits)')
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -534,6 +545,24 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps(dict(copy_bytes_read=4 * n, copy_bytes_written=4 * n,
                               launches=3)))
+        return
+    if args.atomic_calibration:
+        import torch
+        cells, n = 256 * 256 * 128, 1 << 27
+        g = torch.Generator(device='cuda').manual_seed(5)
+        idx = torch.randint(0, cells, (n,), device='cuda', generator=g)
+        val = torch.ones(n, dtype=torch.int64, device='cuda')
+        out = torch.zeros(cells, dtype=torch.int64, device='cuda')
+        for _ in range(2):
+            out.index_add_(0, idx, val)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            out.index_add_(0, idx, val)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / 5
+        print(json.dumps(dict(atomic_adds=n, cells=cells, ms=1e3 * dt,
+                              atomics_per_s=n / dt)))
         return
     if args.gpus != 1:
         print(json.dumps(dict(error="this benchmark runs on one GPU; "
@@ -624,13 +653,17 @@ def main():
     per_chunk = args.nsets / max(chunks, 1)
     # the last timed step ran one lane: its tile statistics are the last
     # chunk's (64 sets of the workload)
-    live_tiles = None
+    live_tiles = live_groups = live_blocks = None
     if info['pipeline'] == 'direct':
-        nzt, tot = ev.tile_stats()
+        (nzt, tot), (gz, gt), (bz, bt) = ev.tile_stats()
         live_tiles = nzt / tot if tot else None
+        live_groups = gz / gt if gt else None
+        live_blocks = bz / bt if bt else None
     live = live_tiles if live_tiles is not None else 1.0
-    algo = algorithmic_bytes(ev, info, live)
-    aflops = algorithmic_flops(ev, info, live)
+    glive = live_groups if live_groups is not None else 1.0
+    blive = live_blocks if live_blocks is not None else 1.0
+    algo = algorithmic_bytes(ev, info, live, glive, blive)
+    aflops = algorithmic_flops(ev, info, live, glive)
     algo_dense = algorithmic_bytes(ev, info)
     flops_dense = algorithmic_flops(ev, info)
     kernels = {}
@@ -672,10 +705,14 @@ def main():
                         traffic=traffic, kernel=dom)
     if live_tiles is not None:
         roofline['live_tile_fraction'] = live_tiles
+        roofline['live_group_fraction'] = live_groups
+        roofline['live_zblock_fraction'] = live_blocks
         roofline['note'] = (roofline.get('note', '') + '; ' if
                             'note' in roofline else '') + (
-            'achieved counts the work of the x-pass tiles some line reaches '
-            '(live_tile_fraction of all); the others are skipped')
+            'achieved counts the work actually done: the x-pass tiles some '
+            'line reaches (live_tile_fraction), the y-pass output groups '
+            'and z-pass blocks with a nonzero input (live_group_fraction, '
+            'live_zblock_fraction); the rest is zero and skipped')
 
     serving = None
     ev.close()

@@ -59,8 +59,8 @@ N.register_signatures({
                                    _c.POINTER(_c.c_double)], _c.c_int),
     'gbkhip_batch_info': ([_c.c_void_p, _c.POINTER(_c.c_int)], _c.c_int),
     'gbkhip_batch_set_tiles': ([_c.c_void_p, _c.c_int, _c.c_int], _c.c_int),
-    'gbkhip_batch_tile_stats': ([_c.c_void_p, _c.POINTER(_c.c_longlong),
-                                 _c.POINTER(_c.c_longlong)], _c.c_int),
+    'gbkhip_batch_tile_stats': ([_c.c_void_p, _c.POINTER(_c.c_longlong)],
+                                _c.c_int),
     'gbkhip_group_create': ([_c.c_void_p, _c.POINTER(_c.c_void_p), _c.c_int,
                              _c.POINTER(_c.c_void_p)], _c.c_int),
     'gbkhip_group_destroy': ([_c.c_void_p], _c.c_int),
@@ -387,12 +387,12 @@ class BatchEvaluator:
         self.tiles = (int(lanes), int(ysplit))
 
     def tile_stats(self):
-        """Direct pipeline: (nonzero, total) x-pass tiles of the last chunk
-        on the first lane; tiles no line reaches are skipped."""
-        a, b = _c.c_longlong(), _c.c_longlong()
-        N.call('gbkhip_batch_tile_stats', self.handle, _c.byref(a),
-               _c.byref(b))
-        return a.value, b.value
+        """Direct pipeline, last chunk on the first lane: (live, total)
+        x-pass tiles, y-pass output groups and z-pass blocks; the others
+        are all-zero and skipped."""
+        v = (_c.c_longlong * 6)()
+        N.call('gbkhip_batch_tile_stats', self.handle, v)
+        return (v[0], v[1]), (v[2], v[3]), (v[4], v[5])
 
     def set_timing(self, enable):
         N.call('gbkhip_batch_set_timing', self.handle, int(bool(enable)))

@@ -229,6 +229,13 @@ __global__ void batch_chi2_final(const double* __restrict__ partials,
 // size) reach no stored output through a kernel of half-width edge_i, so
 // they are zero and the FFT sizes are the smallest 5-smooth ones above it.
 // ---------------------------------------------------------------------------
+__global__ void sep_zero_masks(unsigned long long* m, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+         i < n; i += (long long)gridDim.x * blockDim.x)
+        m[i] = 0ull;
+}
+
 // Line of sight of each spaxel (thin disk): the line it deposits, flux 0
 // when it deposits none or lies beyond the needed region.  Form 0: (flux, v,
 // sigma, 0).  Form 1, ready for evaluation: (amp, v, 1 / (2 sigma^2),
@@ -237,14 +244,15 @@ __global__ void batch_chi2_final(const double* __restrict__ partials,
 __global__ __launch_bounds__(256) void batch_lines_thin(
     gbkhip_disk tmpl, SetLayout lay, const float* __restrict__ params,
     int pstride, gbkhip_grid g, int need_x, int need_y, int nx, int ny,
-    float4* __restrict__ lines, long long lstride, int form)
+    float4* __restrict__ lines, long long lstride, int form,
+    unsigned long long* __restrict__ rowmask, int zc, int need_z)
 {
     const int x = blockIdx.x * 64 + threadIdx.x;
     const int y = blockIdx.y * 4 + threadIdx.y;
     const int set = blockIdx.z;
-    if (x >= nx || y >= ny) return;
+    if (y >= ny) return;               // whole waves: a wave is one row
     float4 out = make_float4(0, 0, 1, 0);
-    if (x < need_x && y < need_y) {
+    if (x < nx && x < need_x && y < need_y) {
         const gbkhip_disk d = set_disk(tmpl, lay,
                                        params + (size_t)set * pstride);
         const bool need_theta = disk_needs_theta(d);
@@ -268,8 +276,21 @@ __global__ __launch_bounds__(256) void batch_lines_thin(
         out = make_float4(amp, out.y, inv2s2,
                           __uint_as_float((unsigned)k0 |
                                           ((unsigned)k1 << 16)));
+        // channel chunks of the x pass this spaxel's line reaches, OR-ed
+        // over the row (the wave) into the row's mask
+        unsigned long long bits = 0;
+        const int k1c = min(k1, need_z - 1);
+        if (amp != 0.0f && k0 <= k1c) {
+            const int c0 = k0 / zc, c1 = k1c / zc;
+            bits = (c1 - c0 + 1 >= 64 ? ~0ull
+                                      : ((1ull << (c1 - c0 + 1)) - 1ull))
+                   << c0;
+        }
+        for (int o = 32; o > 0; o >>= 1) bits |= __shfl_xor(bits, o, 64);
+        if (threadIdx.x == 0 && bits)
+            atomicOr(&rowmask[(size_t)set * ny + y], bits);
     }
-    lines[set * lstride + (size_t)y * nx + x] = out;
+    if (x < nx) lines[set * lstride + (size_t)y * nx + x] = out;
 }
 
 // Spectral lines of L spaxels into an LDS tile, real FFT along z, half
@@ -634,13 +655,16 @@ __global__ __launch_bounds__(256) void sep_xpass(
     const float4* __restrict__ lines, long long lstride, int lpitch,
     gbkhip_grid g, const float* __restrict__ wx, int need_x, int need_z,
     int nxl, int mp, int zc, float* __restrict__ P1, long long p1stride,
-    int ny1, unsigned char* __restrict__ rowflag)
+    int ny1, const unsigned long long* __restrict__ rowmask)
 {
     extern __shared__ float4 M4[];        // [zc][mp] floats
     float* M = reinterpret_cast<float*>(M4);
     __shared__ float s_w[TX];
     constexpr int T = kSepThreads, J = sep_jx(TX);
     const int y = blockIdx.x, z0 = blockIdx.y * zc, set = blockIdx.z;
+    // a tile no line reaches filters to zeros: skipped (its rows are not
+    // written; the y pass reads the same mask and uses zeros)
+    if (!((rowmask[(size_t)set * ny1 + y] >> blockIdx.y) & 1ull)) return;
     const int zn = min(zc, need_z - z0);
     for (int i = threadIdx.x; i < TX; i += T) s_w[i] = wx[i];
     const int n4 = (zc * mp) >> 2;
@@ -650,7 +674,6 @@ __global__ __launch_bounds__(256) void sep_xpass(
     __syncthreads();
     // lines in form 1 (batch_lines_thin): amplitude and window precomputed
     const float4* lrow = lines + set * lstride + (size_t)y * lpitch;
-    int any = 0;
     for (int x = threadIdx.x; x < need_x; x += T) {
         const float4 V = lrow[x];
         if (V.x != 0.0f) {
@@ -664,16 +687,9 @@ __global__ __launch_bounds__(256) void sep_xpass(
             e.zero = g.spec_zero;
             e.step = g.spec_step;
             for (int k = k0; k <= k1; ++k) M[(k - z0) * mp + x] = e.at(k);
-            any |= k0 <= k1;
         }
     }
-    // a tile no line reaches filters to zeros: flag it for the y pass
-    // (which then reads zeros instead of the row) and skip it
-    const int nonzero = __syncthreads_or(any);
-    if (threadIdx.x == 0)
-        rowflag[((size_t)set * gridDim.y + blockIdx.y) * ny1 + y] =
-            (unsigned char)(nonzero != 0);
-    if (!nonzero) return;
+    __syncthreads();
     // outputs (z, X .. X + J - 1): inputs sx X + [0, sx (J - 1) + TX).
     // Lane l of a wave takes channel l % 16; its four 16-lane groups take
     // output blocks DXB apart, whose inputs start 16 banks apart (SX J DXB
@@ -723,7 +739,8 @@ __global__ __launch_bounds__(256) void sep_ypass(
     const float* __restrict__ P1, long long p1stride, const float* wy,
     int nxl, int nyl, int ny1, int need_z, int yblk,
     float* __restrict__ P2, long long p2stride,
-    const unsigned char* __restrict__ rowflag, int zc, int nchunks)
+    const unsigned long long* __restrict__ rowmask, int zc,
+    unsigned char* __restrict__ gflag, int ngy)
 {
     constexpr int T = kSepThreads, J = sep_jy(TY), XW = kSepXW;
     constexpr int PITCH = kSepYPitch, G = T / XW;
@@ -737,14 +754,26 @@ __global__ __launch_bounds__(256) void sep_ypass(
     __shared__ unsigned char s_flag[kSepMaxRows];
     for (int i = threadIdx.x; i < TY; i += T) s_w[i] = wy[i];
     const float* src = P1 + set * p1stride + (size_t)z * ny1 * nxl;
-    // rows the x pass flagged as all-zero are not read (never written)
-    const unsigned char* fl =
-        rowflag + ((size_t)set * nchunks + z / zc) * ny1;
+    // rows whose x-pass tile no line reaches are zero (never written)
+    const unsigned long long* rm = rowmask + (size_t)set * ny1;
+    const int chunk = z / zc;
     for (int r = threadIdx.x; r < nrows; r += T) {
         const int y = SY * Y0 + r;
-        s_flag[r] = y < ny1 ? fl[y] : 0;
+        s_flag[r] = y < ny1 ? (unsigned char)((rm[y] >> chunk) & 1ull) : 0;
     }
     __syncthreads();
+    // an output group whose input rows are all zero is zero: not computed
+    // and not written; its flag tells the z pass to read zeros instead
+    __shared__ unsigned char s_glive[kSepMaxRows];
+    int any = 0;
+    for (int gi = threadIdx.x; gi < ngrp; gi += T) {
+        unsigned char live = 0;
+        for (int i = 0; i < NIN; ++i) live |= s_flag[SY * J * gi + i];
+        s_glive[gi] = live;
+        any |= live;
+        gflag[((size_t)set * need_z + z) * ngy + Y0 / J + gi] = live;
+    }
+    if (!__syncthreads_or(any)) return;      // every group of the block is zero
     // staging: U independent loads in flight per thread before the stores
     constexpr int U = 8;
     if ((nxl & 3) == 0 && X0 + XW <= nxl) {
@@ -796,6 +825,7 @@ __global__ __launch_bounds__(256) void sep_ypass(
     const int c = threadIdx.x % XW, X = X0 + c;
     float* dst = P2 + set * p2stride + (size_t)z * nyl * nxl + X;
     for (int gi = threadIdx.x / XW; gi < ngrp; gi += G) {
+        if (!s_glive[gi]) continue;
         const float* in = tile + SY * J * gi * PITCH + c;
         float v[NIN];
 #pragma unroll
@@ -823,27 +853,53 @@ __global__ __launch_bounds__(256) void sep_zpass(
     const float* __restrict__ P2, long long p2stride, const float* wz,
     int nxl, int nyl, int nzl, int need_z, int zblocks, DataArgs da,
     float* __restrict__ lo_out, long long lstride,
-    double* __restrict__ partials, int nparts)
+    double* __restrict__ partials, int nparts,
+    const unsigned char* __restrict__ gflag, int ngy, int jy,
+    const double* __restrict__ zero_part)
 {
     constexpr int T = kSepThreads, J = kSepJZ, TXB = 64, TYB = T / TXB;
     constexpr int NIN = SZ * (J - 1) + TZ;
     __shared__ float s_w[TZ];
     __shared__ double red[T / 64];
+    __shared__ unsigned char s_live[TYB][NIN];
     const int set = blockIdx.z / zblocks, zb = blockIdx.z % zblocks;
     const int X = blockIdx.x * TXB + threadIdx.x % TXB;
     const int Y = blockIdx.y * TYB + threadIdx.x / TXB;
     const int Z0 = zb * J;
     for (int i = threadIdx.x; i < TZ; i += T) s_w[i] = wz[i];
-    __syncthreads();
+    // which (channel, row) inputs the y pass wrote: the others are zero
+    int any = 0;
+    for (int i = threadIdx.x; i < TYB * NIN; i += T) {
+        const int yy = i / NIN, k = i - yy * NIN;
+        const int z = SZ * Z0 + k, Yr = blockIdx.y * TYB + yy;
+        s_live[yy][k] = (z < need_z && Yr < nyl)
+            ? gflag[((size_t)set * need_z + z) * ngy + Yr / jy] : 0;
+        any |= s_live[yy][k];
+    }
+    const int part = (blockIdx.y * gridDim.x + blockIdx.x) * zblocks + zb;
+    // a block whose model is zero everywhere has the data's own partial
+    // sum, computed once at set-up by this kernel with no live input (same
+    // arithmetic, so the same bits): the data are not read again
+    if (!__syncthreads_or(any) && zero_part) {
+        if (threadIdx.x == 0)
+            partials[(size_t)set * nparts + part] = zero_part[part];
+        if (lo_out && X < nxl && Y < nyl) {
+            const size_t plane = (size_t)nyl * nxl, col = (size_t)Y * nxl + X;
+            for (int j = 0; j < J && Z0 + j < nzl; ++j)
+                lo_out[set * lstride + (size_t)(Z0 + j) * plane + col] = 0.0f;
+        }
+        return;
+    }
     double acc = 0.0;
     if (X < nxl && Y < nyl) {
         const size_t plane = (size_t)nyl * nxl, col = (size_t)Y * nxl + X;
         const float* src = P2 + set * p2stride + col;
+        const unsigned char* lv = s_live[threadIdx.x / TXB];
         float v[NIN];
 #pragma unroll
         for (int i = 0; i < NIN; ++i) {
             const int z = SZ * Z0 + i;
-            v[i] = z < need_z ? src[(size_t)z * plane] : 0.0f;
+            v[i] = lv[i] ? src[(size_t)z * plane] : 0.0f;
         }
         float o[J] = {};
 #pragma unroll
@@ -872,7 +928,6 @@ __global__ __launch_bounds__(256) void sep_zpass(
     if (threadIdx.x == 0) {
         double t = 0.0;
         for (int w = 0; w < T / 64; ++w) t += red[w];
-        const int part = (blockIdx.y * gridDim.x + blockIdx.x) * zblocks + zb;
         partials[(size_t)set * nparts + part] = t;
     }
 }
@@ -887,7 +942,10 @@ struct Sep {
     int yblk_full = 0;            // ... when one block covers every Y
     DevBuf wx, wy, wz;            // taps
     DevBuf P1, P2;                // work: per set (z, y, X) and (z, Y, X)
-    DevBuf flags;                 // work: per set, nonzero x-pass tiles
+    DevBuf masks;                 // work: per set and row, live chunks
+    DevBuf gflags;                // work: per set, channel, y-pass group
+    DevBuf zero_part;             // z-pass partials of a zero model (data)
+    int ngy = 0;                  // y-pass output groups per channel
     int nchunks = 0;              // S1 channel chunks
     int last_sets = 0;            // sets of the last chunk run on lane 0
     long long p1stride = 0, p2stride = 0;
@@ -1036,7 +1094,8 @@ static size_t work_bytes(const Batch& B)
     if (B.sep.on) {
         n += mb * (size_t)B.sep.need[1] * B.sep.need[0] * sizeof(float4);
         n += mb * (size_t)(B.sep.p1stride + B.sep.p2stride) * sizeof(float);
-        n += mb * (size_t)B.sep.nchunks * B.sep.need[1];
+        n += mb * (size_t)B.sep.need[1] * sizeof(uint64_t);
+        n += mb * (size_t)B.sep.need[2] * B.sep.ngy;
     } else if (B.fused.on) {
         n += mb * (size_t)B.fused.sstride * sizeof(float2);
         n += mb * (size_t)B.fused.lstride * sizeof(float4);
@@ -1064,7 +1123,8 @@ static void alloc_work(Batch& B)
                             sizeof(float4));
         B.sep.P1.alloc(mb * (size_t)B.sep.p1stride * sizeof(float));
         B.sep.P2.alloc(mb * (size_t)B.sep.p2stride * sizeof(float));
-        B.sep.flags.alloc(mb * (size_t)B.sep.nchunks * B.sep.need[1]);
+        B.sep.masks.alloc(mb * (size_t)B.sep.need[1] * sizeof(uint64_t));
+        B.sep.gflags.alloc(mb * (size_t)B.sep.need[2] * B.sep.ngy);
     } else if (B.fused.on) {
         B.fused.S.alloc(mb * B.fused.sstride * sizeof(float2));
         B.fused.lines.alloc(mb * B.fused.lstride * sizeof(float4));
@@ -1092,7 +1152,8 @@ static void release_work(Batch& B)
     B.fused.lines.release();
     B.sep.P1.release();
     B.sep.P2.release();
-    B.sep.flags.release();
+    B.sep.masks.release();
+    B.sep.gflags.release();
     B.hi.release();
     B.spec.release();
     B.icube.release();
@@ -1321,7 +1382,8 @@ static void run_chunk_fused(Batch& b, const float* params, int set0,
         dim3 g0(ceil_div(F.nx, 64), ceil_div(F.ny, 4), nsets);
         hipLaunchKernelGGL(batch_lines_thin, g0, dim3(64, 4), 0, c.stream,
                            b.tmpl, b.lay, P, D.pstride, D.grid, F.need[0],
-                           F.need[1], F.nx, F.ny, lines, F.lstride, 0);
+                           F.need[1], F.nx, F.ny, lines, F.lstride, 0,
+                           nullptr, 1, 0);
         GBK_HIP(hipGetLastError());
         mark(b, ci, 1);
         dim3 grid(ceil_div(F.nx, 1 << F.fz.lshift), F.ny, nsets);
@@ -1473,12 +1535,14 @@ static bool plan_sep(Batch& B)
     const int nchunks = (P.need[2] + zmax - 1) / zmax;
     P.zc = (P.need[2] + nchunks - 1) / nchunks;
     P.nchunks = (P.need[2] + P.zc - 1) / P.zc;
+    if (P.nchunks > 64) return false;      // 64-bit row masks
     // Y outputs per S2 block: all of them when the rows fit 48 KB
     const int max_rows = kSepMaxRows;
     const int gmax = ((max_rows - P.ty) / sy + 1) / jy;
     if (gmax < 1) return false;
     P.yblk = std::min((nyl + jy - 1) / jy, gmax) * jy;
     P.yblk_full = P.yblk;
+    P.ngy = (nyl + jy - 1) / jy;
     P.p1stride = (long long)P.need[2] * P.need[1] * nxl;
     P.p2stride = (long long)P.need[2] * nyl * nxl;
     P.zblocks = (nzl + kSepJZ - 1) / kSepJZ;
@@ -1557,14 +1621,20 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
     float* P1 = P.P1.as<float>() + mb * P.p1stride;
     float* P2 = P.P2.as<float>() + mb * P.p2stride;
     double* partials = b.partials.as<double>() + mb * b.nparts;
-    unsigned char* flags = P.flags.as<unsigned char>() +
-                           mb * (size_t)P.nchunks * ny;
+    unsigned long long* masks = P.masks.as<unsigned long long>() +
+                                mb * (size_t)ny;
+    unsigned char* gflags = P.gflags.as<unsigned char>() +
+                            mb * (size_t)nz * P.ngy;
     if (lane == 0) b.sep.last_sets = nsets;
     {
+        hipLaunchKernelGGL(sep_zero_masks, dim3(ceil_div(nsets * ny, 256)),
+                           dim3(256), 0, c.stream, masks,
+                           (long long)nsets * ny);
+        GBK_HIP(hipGetLastError());
         dim3 g0(ceil_div(nx, 64), ceil_div(ny, 4), nsets);
         hipLaunchKernelGGL(batch_lines_thin, g0, dim3(64, 4), 0, c.stream,
                            b.tmpl, b.lay, par, D.pstride, D.grid, nx, ny, nx,
-                           ny, lines, lstride, 1);
+                           ny, lines, lstride, 1, masks, P.zc, nz);
         GBK_HIP(hipGetLastError());
     }
     mark(b, ci, 1);
@@ -1574,7 +1644,7 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
         GBK_SEP_DISPATCH(sep_xpass, D.scale[0], P.tx, grid, dim3(kSepThreads),
                          lds, c.stream, lines, lstride, nx, D.grid,
                          P.wx.as<float>(), nx, nz, nxl, P.mp, P.zc,
-                         P1, P.p1stride, ny, flags);
+                         P1, P.p1stride, ny, masks);
         GBK_HIP(hipGetLastError());
     }
     mark(b, ci, 2);
@@ -1585,7 +1655,7 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
         GBK_SEP_DISPATCH(sep_ypass, D.scale[1], P.ty, grid, dim3(kSepThreads),
                          lds, c.stream, P1, P.p1stride,
                          P.wy.as<float>(), nxl, nyl, ny, nz, P.yblk,
-                         P2, P.p2stride, flags, P.zc, P.nchunks);
+                         P2, P.p2stride, masks, P.zc, gflags, P.ngy);
         GBK_HIP(hipGetLastError());
     }
     mark(b, ci, 3);
@@ -1602,7 +1672,8 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
                          0, c.stream, P2, P.p2stride,
                          P.wz.as<float>(), nxl, nyl, nzl, nz, P.zblocks, da,
                          cubes ? cubes + (size_t)set0 * b.n_lo : nullptr,
-                         b.n_lo, partials, b.nparts);
+                         b.n_lo, partials, b.nparts, gflags, P.ngy,
+                         sep_jy(P.ty), P.zero_part.as<double>());
         GBK_HIP(hipGetLastError());
     }
     mark(b, ci, 4);
@@ -1610,6 +1681,36 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
                        partials, b.nparts, chi2, set0);
     GBK_HIP(hipGetLastError());
     mark(b, ci, 5);
+}
+
+// The z pass's per-block partial sums of r^2 for a model that is zero
+// everywhere (data only): blocks with no live input take them instead of
+// reading the data again.  Computed once by the z pass itself with every
+// input flagged dead, so the bits are those it would produce.
+static void compute_zero_partials(Batch& b)
+{
+    Context& c = *b.ctx;
+    const gbkhip_batch_desc& D = b.desc;
+    Sep& P = b.sep;
+    const int nxl = D.size_lo[0], nyl = D.size_lo[1], nzl = D.size_lo[2];
+    const int nz = P.need[2];
+    P.zero_part.alloc((size_t)b.nparts * sizeof(double));
+    DevBuf dead((size_t)nz * P.ngy);
+    GBK_HIP(hipMemsetAsync(dead.ptr, 0, (size_t)nz * P.ngy, c.stream));
+    DataArgs da;
+    da.obs = D.obs;
+    da.err = D.err;
+    da.mask = D.mask;
+    da.err_scalar = D.err_scalar;
+    da.weight = D.weight;
+    dim3 grid(ceil_div(nxl, 64), ceil_div(nyl, kSepThreads / 64), P.zblocks);
+    GBK_SEP_DISPATCH(sep_zpass, D.scale[2], P.tz, grid, dim3(kSepThreads), 0,
+                     c.stream, P.P2.as<float>(), P.p2stride,
+                     P.wz.as<float>(), nxl, nyl, nzl, nz, P.zblocks, da,
+                     nullptr, b.n_lo, P.zero_part.as<double>(), b.nparts,
+                     dead.as<unsigned char>(), P.ngy, sep_jy(P.ty), nullptr);
+    GBK_HIP(hipGetLastError());
+    GBK_HIP(hipStreamSynchronize(c.stream));
 }
 
 static void run_chunk(Batch& b, const float* params, int set0, int nsets,
@@ -1818,7 +1919,7 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
                 B.lanes = 2;
                 ensure_lane_streams(B);
                 alloc_work(B);
-                GBK_HIP(hipStreamSynchronize(c->stream));
+                compute_zero_partials(B);
                 *out = b;
                 return;
             }
@@ -2001,7 +2102,8 @@ int gbkhip_batch_memory(gbkhip_batch* b, unsigned long long* resident,
         size_t r = B.ints.bytes + B.rnodes.bytes + B.spectrum.bytes +
                    F.phat.bytes + F.lhat.bytes + F.twz.bytes + F.twy.bytes +
                    F.twx.bytes + F.tpz.bytes + F.twsz.bytes + F.twsy.bytes +
-                   F.twsx.bytes;
+                   F.twsx.bytes + B.sep.wx.bytes + B.sep.wy.bytes +
+                   B.sep.wz.bytes + B.sep.zero_part.bytes;
         *resident = r;
         *work = work_bytes(B);
         *allocated = B.work ? 1 : 0;
@@ -2171,26 +2273,53 @@ int gbkhip_group_evaluate(gbkhip_group* g, const float* const* params,
     });
 }
 
-int gbkhip_batch_tile_stats(gbkhip_batch* b, long long* nonzero,
-                            long long* total)
+int gbkhip_batch_tile_stats(gbkhip_batch* b, long long* counts)
 {
     return guarded([&] {
-        GBK_CHECK(b && nonzero && total, GBKHIP_ERR_ARG, "null argument");
+        GBK_CHECK(b && counts, GBKHIP_ERR_ARG, "null argument");
         Batch& B = b->impl;
-        *nonzero = *total = 0;
+        for (int i = 0; i < 4; ++i) counts[i] = 0;
         if (!B.sep.on || !B.work || B.sep.last_sets == 0) return;
         GBK_HIP(hipSetDevice(B.device));
         const hipStream_t s = B.stream ? B.stream : B.ctx->stream;
         GBK_HIP(hipStreamSynchronize(s));
-        const size_t n = (size_t)B.sep.last_sets * B.sep.nchunks *
-                         B.sep.need[1];
-        std::vector<unsigned char> h(n);
-        GBK_HIP(hipMemcpy(h.data(), B.sep.flags.ptr, n,
+        const size_t n = (size_t)B.sep.last_sets * B.sep.need[1];
+        std::vector<unsigned long long> h(n);
+        GBK_HIP(hipMemcpy(h.data(), B.sep.masks.ptr, n * sizeof(uint64_t),
                           hipMemcpyDeviceToHost));
-        long long c = 0;
-        for (unsigned char v : h) c += v != 0;
-        *nonzero = c;
-        *total = (long long)n;
+        for (unsigned long long v : h) counts[0] += __builtin_popcountll(v);
+        counts[1] = (long long)n * B.sep.nchunks;
+        const size_t ng = (size_t)B.sep.last_sets * B.sep.need[2] *
+                          B.sep.ngy;
+        std::vector<unsigned char> f(ng);
+        GBK_HIP(hipMemcpy(f.data(), B.sep.gflags.ptr, ng,
+                          hipMemcpyDeviceToHost));
+        for (unsigned char v : f) counts[2] += v != 0;
+        counts[3] = (long long)ng;
+        // z-pass blocks (per set, 16-channel output block and 4-row block)
+        // with a live input; the others take the zero-model partials
+        const gbkhip_batch_desc& D = B.desc;
+        const int nz = B.sep.need[2], nyl = D.size_lo[1];
+        const int sz = D.scale[2], jy = sep_jy(B.sep.ty);
+        const int nin = sz * (kSepJZ - 1) + B.sep.tz;
+        const int yb = ceil_div(nyl, kSepThreads / 64);
+        for (int st = 0; st < B.sep.last_sets; ++st)
+            for (int zb = 0; zb < B.sep.zblocks; ++zb)
+                for (int by = 0; by < yb; ++by) {
+                    bool live = false;
+                    for (int yy = 0; yy < kSepThreads / 64 && !live; ++yy) {
+                        const int Y = by * (kSepThreads / 64) + yy;
+                        if (Y >= nyl) break;
+                        for (int k = 0; k < nin && !live; ++k) {
+                            const int z = sz * zb * kSepJZ + k;
+                            live = z < nz &&
+                                   f[((size_t)st * nz + z) * B.sep.ngy +
+                                     Y / jy] != 0;
+                        }
+                    }
+                    counts[4] += live;
+                    counts[5] += 1;
+                }
     });
 }
 

@@ -287,12 +287,15 @@ int gbkhip_group_evaluate(gbkhip_group* group, const float* const* params,
                           double* const* chi2, float* const* cubes,
                           int use_graph);
 
-/* Direct pipeline: x-pass tiles (one hi-res row x one channel chunk of one
- * set) of the last chunk evaluated on the first lane that some line reaches
- * (*nonzero) out of all (*total); the others are skipped (their outputs are
- * zero).  Waits for the batch's stream.  Zeros for other pipelines. */
-int gbkhip_batch_tile_stats(gbkhip_batch* batch, long long* nonzero,
-                            long long* total);
+/* Direct pipeline, last chunk evaluated on the first lane: counts[0] of
+ * counts[1] x-pass tiles (one hi-res row x one channel chunk of one set)
+ * that some line reaches, counts[2] of counts[3] y-pass output groups (one
+ * channel x 16 (or 4) rows of one set) with a nonzero input, and counts[4]
+ * of counts[5] z-pass blocks (16 channels x 4 rows of one set) with a
+ * nonzero input; the others are skipped (their outputs are zero; a skipped
+ * z-pass block takes its precomputed data-only partial sum).  Waits for
+ * the batch's stream.  Zeros for other pipelines. */
+int gbkhip_batch_tile_stats(gbkhip_batch* batch, long long* counts);
 /* Stage timing: with timing on, evaluate() launches directly (no graph) and
  * records events around the smdisk, convolution and downscale+chi^2 stages
  * of every chunk; stage_times() waits for them and returns the summed stage

@@ -100,9 +100,10 @@ def test_pipeline_selection(setup):
     # the outermost ring node and channels beyond the line-of-sight
     # velocities); the oracle tests cover their zero outputs
     ev.evaluate([P.gparams(p) for p in sets[:2]], use_graph=False)
-    live, total = ev.tile_stats()
-    print('live x-pass tiles', live, 'of', total)
-    assert 0 < live < total
+    (live, total), (glive, gtotal), (blive, btotal) = ev.tile_stats()
+    print('live x-pass tiles', live, 'of', total, '; live y-pass groups',
+          glive, 'of', gtotal, '; live z-pass blocks', blive, 'of', btotal)
+    assert 0 < live < total and 0 < glive < gtotal and 0 < blive <= btotal
     fused = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock,
                            error=0.01, max_batch=2, path='fused')
     assert fused.info()['pipeline'] == 'fused'
