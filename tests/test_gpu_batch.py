@@ -218,3 +218,37 @@ def test_mcdisk_batch_matches_plain_path():
     g = ev.evaluate(sets, use_graph=True)
     assert np.array_equal(g, chi2)
     ev.close()
+
+
+@pytest.mark.parametrize('path', ['direct', 'fused', 'padded'])
+def test_error_cube_mask_weight(path):
+    """chi^2 with an error cube, a mask and a weight (objective.hpp:7-33)
+    against the oracle's residual on every pipeline; on the direct pipeline
+    this also covers the data-only partial sums of z-pass blocks with no
+    live input, which fold in the same error and mask."""
+    from gbkfit_amd.batch import BatchEvaluator
+    drv = H.driver()
+    p = dict(P.c1_params(), size=(48, 44, 32))
+    model, base = P.build_product(drv, p)
+    ref, amb, _ = P.oracle_cube(p)
+    rng = np.random.default_rng(21)
+    mock = (ref + rng.normal(0, 0.01, ref.shape)).astype(np.float32)
+    err = rng.uniform(0.005, 0.02, ref.shape).astype(np.float32)
+    mask = (rng.uniform(size=ref.shape) > 0.2).astype(np.float32)
+    ev = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock,
+                        error=err, mask=mask, weight=0.7, max_batch=2,
+                        path=path)
+    assert ev.info()['pipeline'] == path
+    sets = [dict(p, incl=i, posa=a) for i, a in ((35, 10), (62, 150),
+                                                  (80, 300))]
+    chi2 = ev.evaluate([P.gparams(s) for s in sets], use_graph=True)
+    for i, s in enumerate(sets):
+        r_cube, r_amb, _ = P.oracle_cube(s)
+        r = oracle.residual(mock, err, mask, r_cube, None, None,
+                            0.7).reshape(r_cube.shape)
+        c_ref = oracle.residual_sum(r, True)
+        slack = np.sum(mask * 0.7 * (2 * np.abs(r) * r_amb / err +
+                                     0.7 * (r_amb / err) ** 2))
+        assert abs(chi2[i] - c_ref) <= 1e-5 * c_ref + slack, \
+            (path, i, chi2[i], c_ref)
+    ev.close()
