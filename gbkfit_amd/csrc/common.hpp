@@ -104,6 +104,7 @@ struct Context {
         work;
     DevBuf reduce_scratch;   // partial sums of reductions
     DevBuf int_scratch;      // int64 fixed-point accumulation (mcdisk)
+    DevBuf mc_scratch;       // binned mcdisk: records, ranks, tile offsets
     std::mutex mu;
 };
 

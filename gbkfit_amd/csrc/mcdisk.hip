@@ -23,6 +23,165 @@ __global__ __launch_bounds__(256) void mcdisk_kernel(
         mc_cloud(ci, mp, d, g, o, icube);
 }
 
+// ---- Binned deposits (scube only): clouds -> records ranked per spatial
+// tile -> tile offsets -> records sorted by tile -> per-tile LDS sums.
+__global__ void mc_zero_i32(int* p, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// With ntiles <= kMcBinLds the block counts its clouds per tile in LDS and
+// reserves one range per tile with a single global atomic, instead of one
+// global atomic per cloud: the central tiles of a disk take most clouds,
+// and same-address atomics serialise.
+constexpr int kMcBinLds = 4096;
+constexpr int kMcBinBlocks = 8192;
+
+__global__ __launch_bounds__(256) void mcdisk_bin_kernel(
+    McParams mp, gbkhip_disk d, gbkhip_grid g, McBin bin, int ntiles)
+{
+    extern __shared__ int s_csum[];
+    __shared__ float s_nodes[kMcNodesLds];
+    __shared__ int s_cnt[kMcBinLds];
+    mp.csum = mc_stage_csum(mp.csum, mp.csum_len, s_csum);
+    d.rnodes = mc_stage_nodes(d.rnodes, d.nrnodes, s_nodes);
+    const bool local = ntiles <= kMcBinLds;
+    if (local) {
+        for (int i = threadIdx.x; i < ntiles; i += blockDim.x) s_cnt[i] = 0;
+        __syncthreads();
+    }
+    gbkhip_outputs o;
+    memset(&o, 0, sizeof(o));
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long ci = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+         ci < mp.nclouds; ci += stride) {
+        const int tile = mc_cloud(ci, mp, d, g, o, nullptr, &bin);
+        // rank within the block's clouds of the tile (the block reserves
+        // its ranges afterwards), or directly global.  Wave-aggregated: the
+        // clouds of a disk crowd into its central tiles, and atomics on one
+        // address serialise, so each distinct tile of the wave takes one
+        // atomic for all its lanes, which rank by lane order.
+        const bool has = tile >= 0;
+        const int lane = __lane_id();
+        const unsigned long long below = (1ull << lane) - 1ull;
+        unsigned long long todo = __ballot(has);
+        int rank = 0;
+        while (todo) {
+            const int leader = __builtin_ctzll(todo);
+            const int t = __shfl(tile, leader);
+            const unsigned long long same = __ballot(has && tile == t);
+            int base = 0;
+            if (lane == leader) {
+                const int n = __popcll(same);
+                base = local ? atomicAdd(&s_cnt[t], n)
+                             : atomicAdd(&bin.tcount[t], n);
+            }
+            base = __shfl(base, leader);
+            if (has && tile == t) rank = base + __popcll(same & below);
+            todo &= ~same;
+        }
+        bin.tr[ci] = make_int2(tile, rank);
+    }
+    if (!local) return;
+    __syncthreads();
+    // one reservation per tile this block reached: s_cnt <- its base
+    for (int i = threadIdx.x; i < ntiles; i += blockDim.x) {
+        const int n = s_cnt[i];
+        s_cnt[i] = n ? atomicAdd(&bin.tcount[i], n) : 0;
+    }
+    __syncthreads();
+    for (long long ci = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+         ci < mp.nclouds; ci += stride) {
+        const int2 t = bin.tr[ci];
+        if (t.x >= 0) bin.tr[ci] = make_int2(t.x, t.y + s_cnt[t.x]);
+    }
+}
+
+// off[t] = sum of count[0..t), off[n] = total: one block of 1024 threads.
+__global__ __launch_bounds__(1024) void mc_scan_tiles(const int* count,
+                                                      int* off, int n)
+{
+    __shared__ int s[1024];
+    const int t = threadIdx.x, per = (n + 1023) / 1024;
+    const int lo = min(t * per, n), hi = min(lo + per, n);
+    int sum = 0;
+    for (int i = lo; i < hi; ++i) sum += count[i];
+    s[t] = sum;
+    __syncthreads();
+    if (t == 0) {                      // exclusive scan of 1024 partials
+        int run = 0;
+        for (int i = 0; i < 1024; ++i) {
+            const int v = s[i];
+            s[i] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    int run = s[t];
+    for (int i = lo; i < hi; ++i) {
+        off[i] = run;
+        run += count[i];
+    }
+    if (t == 1023) off[n] = run;
+}
+
+__global__ void mc_scatter(const McDep* __restrict__ rec,
+                           const int2* __restrict__ tr,
+                           const int* __restrict__ off,
+                           McDep* __restrict__ sorted, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+         i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int2 t = tr[i];
+        if (t.x >= 0) sorted[off[t.x] + t.y] = rec[i];
+    }
+}
+
+// One block per tile: the deposits of its clouds summed in LDS with int64
+// atomics (the same quantised values the global path adds), then cube +=
+// sum / 2^S as fixed_to_float does.
+__global__ __launch_bounds__(256) void mc_tile_deposit(
+    const McDep* __restrict__ sorted, const int* __restrict__ off,
+    gbkhip_grid g, int tw, int th, int ntx, double qscale,
+    float* __restrict__ cube)
+{
+    extern __shared__ unsigned long long acc[];      // [nz][tw * th]
+    const int tile = blockIdx.x;
+    const int lo = off[tile], hi = off[tile + 1];
+    if (lo == hi) return;
+    const int nx = g.spat_size[0], ny = g.spat_size[1], nz = g.spec_size;
+    const int tc = tw * th;
+    const size_t plane = (size_t)nx * ny;
+    for (int i = threadIdx.x; i < nz * tc; i += blockDim.x) acc[i] = 0ull;
+    __syncthreads();
+    const int tx0 = (tile % ntx) * tw, ty0 = (tile / ntx) * th;
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const McDep r = sorted[i];
+        const int y = r.col / nx, x = r.col - y * nx;
+        const int lc = (y - ty0) * tw + (x - tx0);
+        int k0, k1;
+        line_window(g, r.v, r.s, k0, k1);
+        if (k0 > k1) continue;
+        const LineEval e = line_eval(g, r.orvalue, r.v, r.s);
+        for (int k = k0; k <= k1; ++k) {
+            const long long q = __double2ll_rn((double)e.at(k) * qscale);
+            if (q) atomicAdd(&acc[k * tc + lc], (unsigned long long)q);
+        }
+    }
+    __syncthreads();
+    const double inv = 1.0 / qscale;
+    for (int i = threadIdx.x; i < nz * tc; i += blockDim.x) {
+        const long long q = (long long)acc[i];
+        if (!q) continue;
+        const int k = i / tc, lc = i - k * tc;
+        const int x = tx0 + lc % tw, y = ty0 + lc / tw;
+        if (x < nx && y < ny)
+            cube[k * plane + (size_t)y * nx + x] += (float)((double)q * inv);
+    }
+}
+
 // cube += icube / 2^S, and icube <- 0 for the next call.
 __global__ void fixed_to_float(unsigned long long* icube, float* cube,
                                long long n, double inv_scale)
@@ -69,8 +228,18 @@ void launch_mcdisk(Context& c, float cflux, int nclouds, const int* csum,
               "mcdisk needs density polar and height traits");
     const long long ncube = (long long)g.spat_size[0] * g.spat_size[1] *
                             g.spec_size;
+    // binned deposits when the spectral cube is the only output and a tile
+    // column of the spectrum fits LDS
+    const int nz = g.spec_size;
+    int tw = 8;
+    while (tw > 1 && (size_t)tw * tw * nz * 8 > 64 * 1024) tw /= 2;
+    const bool only_scube = o.scube && !o.image && !o.wdata &&
+        !o.wdata_cmp && !o.rdata && !o.rdata_cmp && !o.ordata &&
+        !o.ordata_cmp && !o.vdata_cmp && !o.ddata_cmp;
+    const bool binned = only_scube && ncube > 0 &&
+                        (size_t)tw * tw * nz * 8 <= 64 * 1024;
     unsigned long long* icube = nullptr;
-    if (o.scube && ncube > 0) {
+    if (o.scube && ncube > 0 && !binned) {
         const size_t bytes = (size_t)ncube * sizeof(unsigned long long);
         if (c.int_scratch.bytes < bytes) {
             c.int_scratch.alloc(bytes);
@@ -83,6 +252,7 @@ void launch_mcdisk(Context& c, float cflux, int nclouds, const int* csum,
     McParams mp;
     mp.cflux = cflux;
     mp.nclouds = nclouds;
+    const size_t lds = csum_len <= kMcCsumLds ? csum_len * sizeof(int) : 0;
     mp.csum = csum;
     mp.csum_len = csum_len;
     mp.hoi = hoi;
@@ -90,7 +260,43 @@ void launch_mcdisk(Context& c, float cflux, int nclouds, const int* csum,
     mp.key1 = (uint32_t)(seed >> 32);
     mp.qscale = mcdisk_fixed_scale(cflux, nclouds, g);
     const int blocks = std::max(1, std::min(ceil_div(nclouds, 256), 8192));
-    const size_t lds = csum_len <= kMcCsumLds ? csum_len * sizeof(int) : 0;
+    if (binned) {
+        const int nx = g.spat_size[0], ny = g.spat_size[1];
+        const int ntx = ceil_div(nx, tw), nty = ceil_div(ny, tw);
+        const int ntiles = ntx * nty;
+        const size_t nc = (size_t)nclouds;
+        const size_t need = nc * (2 * sizeof(McDep) + sizeof(int2)) +
+                            (2 * (size_t)ntiles + 1) * sizeof(int);
+        if (c.mc_scratch.bytes < need) c.mc_scratch.alloc(need);
+        char* base = static_cast<char*>(c.mc_scratch.ptr);
+        McBin bin;
+        bin.rec = reinterpret_cast<McDep*>(base);
+        McDep* sorted = bin.rec + nc;
+        bin.tr = reinterpret_cast<int2*>(sorted + nc);
+        bin.tcount = reinterpret_cast<int*>(bin.tr + nc);
+        int* toff = bin.tcount + ntiles;
+        bin.tw = bin.th = tw;
+        bin.ntx = ntx;
+        hipLaunchKernelGGL(mc_zero_i32, dim3(ceil_div(ntiles, 256)),
+                           dim3(256), 0, c.stream, bin.tcount, ntiles);
+        // each block makes one global reservation per tile it reached (the
+        // central tiles are reached by every block); 8192 blocks balance
+        // that against the parallelism of the cloud loop (1024 measured 2x
+        // slower)
+        const int bin_blocks =
+            std::max(1, std::min(ceil_div(nclouds, 256), kMcBinBlocks));
+        hipLaunchKernelGGL(mcdisk_bin_kernel, dim3(bin_blocks), dim3(256),
+                           lds, c.stream, mp, d, g, bin, ntiles);
+        hipLaunchKernelGGL(mc_scan_tiles, dim3(1), dim3(1024), 0, c.stream,
+                           bin.tcount, toff, ntiles);
+        hipLaunchKernelGGL(mc_scatter, dim3(blocks), dim3(256), 0, c.stream,
+                           bin.rec, bin.tr, toff, sorted, (long long)nc);
+        hipLaunchKernelGGL(mc_tile_deposit, dim3(ntiles), dim3(256),
+                           (size_t)tw * tw * nz * 8, c.stream, sorted, toff,
+                           g, tw, tw, ntx, mp.qscale, o.scube);
+        GBK_HIP(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(mcdisk_kernel, dim3(blocks), dim3(256), lds, c.stream,
                        mp, d, g, o, icube);
     GBK_HIP(hipGetLastError());

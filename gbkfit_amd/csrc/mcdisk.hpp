@@ -107,6 +107,22 @@ __device__ inline float rh_trait_rnd(int uid, const float* c, const float* q,
     }
 }
 
+// Binned deposits: instead of adding a cloud's channels to the int64 cube
+// with global atomics, the cloud writes a record and takes a rank inside
+// its spatial tile (tw x th spaxels); a per-tile pass then sums the tile's
+// deposits in LDS.  Integer sums: the cube is the same, bit for bit.
+struct McDep {
+    int col;                 // spaxel y * nx + x
+    float orvalue, v, s;     // line flux, velocity, dispersion
+};
+
+struct McBin {
+    McDep* rec;              // per cloud id
+    int2* tr;                // per cloud id: (tile or -1, rank in tile)
+    int* tcount;             // per tile: clouds so far
+    int tw, th, ntx;         // tile width / height (spaxels), tiles along x
+};
+
 struct McParams {
     float cflux;
     int nclouds;
@@ -144,11 +160,14 @@ __device__ __forceinline__ const float* mc_stage_nodes(const float* nodes,
     return lds;
 }
 
-__device__ __forceinline__ void mc_cloud(long long ci, const McParams& mp,
-                                         const gbkhip_disk& d,
-                                         const gbkhip_grid& g,
-                                         const gbkhip_outputs& o,
-                                         unsigned long long* icube)
+// Returns the cloud's tile when `bin` is given and the cloud lands in the
+// cube (its record written), else -1.
+__device__ __forceinline__ int mc_cloud(long long ci, const McParams& mp,
+                                        const gbkhip_disk& d,
+                                        const gbkhip_grid& g,
+                                        const gbkhip_outputs& o,
+                                        unsigned long long* icube,
+                                        const McBin* bin = nullptr)
 {
     const int nx = g.spat_size[0], ny = g.spat_size[1], nz = g.spat_size[2];
     const size_t plane = (size_t)nx * ny;
@@ -161,7 +180,7 @@ __device__ __forceinline__ void mc_cloud(long long ci, const McParams& mp,
     // ring of the cloud (gmodels.hpp:401-416)
     int rnidx = 0, tidx;
     while (rnidx < mp.csum_len && ci >= mp.csum[rnidx]) rnidx++;
-    if (rnidx >= mp.csum_len) return;     // cloud beyond the budget
+    if (rnidx >= mp.csum_len) return -1;     // cloud beyond the budget
     const float* rpc = d.rp.cvalues;
     const float* rpp = d.rp.pvalues;
     const float* rhc = d.rh.cvalues;
@@ -175,7 +194,7 @@ __device__ __forceinline__ void mc_cloud(long long ci, const McParams& mp,
         rhc += d.rh.ccounts[tidx];
         rhp += d.rh.pcounts[tidx];
     }
-    if (tidx >= d.rp.n) return;           // budget longer than traits
+    if (tidx >= d.rp.n) return -1;           // budget longer than traits
     // rp_trait_rnd (traits.hpp:310-326,1333-1435)
     const int uid = d.rp.uids[tidx];
     const int u = uid >= 1000 ? uid - 1000 : uid;
@@ -215,7 +234,7 @@ __device__ __forceinline__ void mc_cloud(long long ci, const McParams& mp,
     float st, ct;
     sincosf(theta, &st, &ct);
     const float xd = rd * ct, yd = rd * st;
-    if (!disk_info(rnidx, rd, d.nrnodes, d.rnodes)) return;
+    if (!disk_info(rnidx, rd, d.nrnodes, d.rnodes)) return -1;
 
     PPoint P;
     P.rnidx = rnidx;
@@ -232,7 +251,7 @@ __device__ __forceinline__ void mc_cloud(long long ci, const McParams& mp,
         float sv = 0;
         p_traits<TK_SP>(d.sp, P, pt);
         for (int i = 0; i < d.sp.n && i < kTraitMax; ++i) sv += pt[i];
-        if (!sv) return;
+        if (!sv) return -1;
     }
     float zd = rh_trait_rnd(d.rh.uids[tidx], rhc, rhp, P, S);
     if (d.zp.uids) {
@@ -268,12 +287,12 @@ __device__ __forceinline__ void mc_cloud(long long ci, const McParams& mp,
     // returns NaN for |u| > 1/2) lands nowhere: on the host driver
     // rint(NaN) converts to INT_MIN, outside the cube, while the GPU's
     // conversion would give 0.
-    if (isnan(xr) || isnan(yr) || isnan(zn)) return;
+    if (isnan(xr) || isnan(yr) || isnan(zn)) return -1;
     const int x = (int)rintf((xr - g.spat_zero[0]) / g.spat_step[0]);
     const int y = (int)rintf((yr - g.spat_zero[1]) / g.spat_step[1]);
     const int z = (int)rintf((zn - g.spat_zero[2]) / g.spat_step[2]);
     if (x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz)
-        return;
+        return -1;
 
     float vvalue = 0, dvalue = 0, wvalue = 1;
     if (d.vp.uids) p_traits<TK_VP>(d.vp, P, pt);
@@ -295,6 +314,15 @@ __device__ __forceinline__ void mc_cloud(long long ci, const McParams& mp,
                 d.opacity[x + (size_t)y * nx + (size_t)oz * plane];
 
     const size_t col = (size_t)y * nx + x;
+    if (bin) {               // record; the caller ranks it in its tile
+        McDep r;
+        r.col = (int)col;
+        r.orvalue = orvalue;
+        r.v = vvalue;
+        r.s = dvalue;
+        bin->rec[ci] = r;
+        return (y / bin->th) * bin->ntx + x / bin->tw;
+    }
     if (o.image) atomicAdd(&o.image[col], orvalue);
     if (icube) {
         int k0, k1;
@@ -319,6 +347,7 @@ __device__ __forceinline__ void mc_cloud(long long ci, const McParams& mp,
     if (o.ordata_cmp) atomicAdd(&o.ordata_cmp[idx], orvalue);
     if (o.vdata_cmp) o.vdata_cmp[idx] = vvalue;
     if (o.ddata_cmp) o.ddata_cmp[idx] = dvalue;
+    return -1;
 }
 
 }  // namespace gbkhip
