@@ -59,6 +59,7 @@ N.register_signatures({
                                    _c.POINTER(_c.c_double)], _c.c_int),
     'gbkhip_batch_info': ([_c.c_void_p, _c.POINTER(_c.c_int)], _c.c_int),
     'gbkhip_batch_set_tiles': ([_c.c_void_p, _c.c_int, _c.c_int], _c.c_int),
+    'gbkhip_batch_reserve_clouds': ([_c.c_void_p, _c.c_longlong], _c.c_int),
     'gbkhip_batch_tile_stats': ([_c.c_void_p, _c.POINTER(_c.c_longlong)],
                                 _c.c_int),
     'gbkhip_group_create': ([_c.c_void_p, _c.POINTER(_c.c_void_p), _c.c_int,
@@ -285,6 +286,9 @@ class BatchEvaluator:
         N.call('gbkhip_batch_create', self._ctx.handle,
                _c.byref(d), _c.byref(h))
         self.handle = h
+        self._mc_cap = 0
+        if self.mc:   # capacity for the template's budget, with headroom
+            self.reserve_clouds(int(1.5 * max(int(csum0[-1]), 1)))
         self._params_dev = None
         self._chi2_dev = None
         self._cubes_dev = None
@@ -297,6 +301,14 @@ class BatchEvaluator:
         if self.mc:
             self._upload_clouds(params_list)
         return self.upload_packed(host)
+
+    def reserve_clouds(self, clouds):
+        """mcdisk: per-set cloud capacity of the binned deposits (grown,
+        never shrunk; growing frees the work buffers and cached graphs)."""
+        if clouds > self._mc_cap:
+            N.call('gbkhip_batch_reserve_clouds', self.handle,
+                   _c.c_longlong(int(clouds)))
+            self._mc_cap = int(clouds)
 
     def cloud_budgets(self, params_list):
         """mcdisk: each set's cumulative cloud counts (int32, nsets x
@@ -312,6 +324,10 @@ class BatchEvaluator:
             N.call('gbkhip_mcdisk_fixed_scale', _c.c_float(self.desc.cflux),
                    int(c[-1]), _c.byref(self.grid), _c.byref(out))
             qs[i] = out.value
+        if n:
+            top = int(csum[:, -1].max())
+            if top > self._mc_cap:
+                self.reserve_clouds(int(1.25 * top))
         return csum, qs
 
     def _upload_clouds(self, params_list):

@@ -583,6 +583,95 @@ __global__ __launch_bounds__(256) void batch_mcdisk(
         mc_cloud(ci, mp, d, g, o, ic);
 }
 
+// Batched binned mcdisk: blockIdx.y = set.  The same cloud body writes a
+// record per cloud (at most cap per set) and ranks it in its tile, with the
+// wave-aggregated LDS ranks and per-block reservations of the plain path.
+constexpr int kMcBatchBinLds = 4096;
+
+__global__ __launch_bounds__(256) void batch_mcdisk_bin(
+    gbkhip_disk tmpl, SetLayout lay, const float* __restrict__ params,
+    int pstride, gbkhip_grid g, float cflux, const int* __restrict__ csum,
+    int csum_len, const unsigned char* hoi, uint32_t key0, uint32_t key1,
+    McBin bin, long long cap, int ntiles)
+{
+    extern __shared__ int s_csum[];
+    __shared__ float s_nodes[kMcNodesLds];
+    __shared__ int s_cnt[kMcBatchBinLds];
+    const int set = blockIdx.y;
+    gbkhip_disk d = set_disk(tmpl, lay, params + (size_t)set * pstride);
+    d.rnodes = mc_stage_nodes(d.rnodes, d.nrnodes, s_nodes);
+    McParams mp;
+    mp.cflux = cflux;
+    mp.csum = mc_stage_csum(csum + (size_t)set * csum_len, csum_len, s_csum);
+    mp.csum_len = csum_len;
+    mp.nclouds = (int)min((long long)mp.csum[csum_len - 1], cap);
+    mp.hoi = hoi;
+    mp.key0 = key0;
+    mp.key1 = key1;
+    mp.qscale = 1.0;                 // unused by the record path
+    bin.rec += set * cap;
+    bin.tr += set * cap;
+    bin.tcount += (size_t)set * ntiles;
+    const bool local = ntiles <= kMcBatchBinLds;
+    if (local) {
+        for (int i = threadIdx.x; i < ntiles; i += blockDim.x) s_cnt[i] = 0;
+        __syncthreads();
+    }
+    gbkhip_outputs o;
+    memset(&o, 0, sizeof(o));
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const int lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (long long ci = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+         ci < mp.nclouds; ci += stride) {
+        const int tile = mc_cloud(ci, mp, d, g, o, nullptr, &bin);
+        const bool has = tile >= 0;
+        unsigned long long todo = __ballot(has);
+        int rank = 0;
+        while (todo) {
+            const int leader = __builtin_ctzll(todo);
+            const int t = __shfl(tile, leader);
+            const unsigned long long same = __ballot(has && tile == t);
+            int base = 0;
+            if (lane == leader) {
+                const int n = __popcll(same);
+                base = local ? atomicAdd(&s_cnt[t], n)
+                             : atomicAdd(&bin.tcount[t], n);
+            }
+            base = __shfl(base, leader);
+            if (has && tile == t) rank = base + __popcll(same & below);
+            todo &= ~same;
+        }
+        bin.tr[ci] = make_int2(tile, rank);
+    }
+    if (!local) return;
+    __syncthreads();
+    for (int i = threadIdx.x; i < ntiles; i += blockDim.x) {
+        const int n = s_cnt[i];
+        s_cnt[i] = n ? atomicAdd(&bin.tcount[i], n) : 0;
+    }
+    __syncthreads();
+    for (long long ci = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+         ci < mp.nclouds; ci += stride) {
+        const int2 t = bin.tr[ci];
+        if (t.x >= 0) bin.tr[ci] = make_int2(t.x, t.y + s_cnt[t.x]);
+    }
+}
+
+__global__ void batch_zero_f32(float* p, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+         i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0.0f;
+}
+
+__global__ void batch_zero_i32(int* p, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+         i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
 // hi = icube / 2^S per set (0 where nothing landed), icube <- 0.
 __global__ void batch_fixed_to_hi(unsigned long long* icube, float* hi,
                                   long long n, const double* qscale)
@@ -994,6 +1083,12 @@ struct Batch {
     DevBuf spec;            // chunk x complex work spectrum
     DevBuf partials;
     DevBuf icube;           // mcdisk: chunk x int64 fixed-point cube
+    // binned mcdisk deposits (when a tile spectrum fits LDS): per set of a
+    // chunk, mc_cap records, ranks and sorted records, tile counts, offsets
+    int mc_tw = 0, mc_ntx = 0, mc_ntiles = 0;
+    long long mc_cap = 0;
+    DevBuf mcbin;
+    unsigned long long work_gen = 0;    // bumped whenever work is allocated
     const int* csum = nullptr;       // mcdisk inputs of the current call
     const double* qscale = nullptr;
     int nparts = 0;
@@ -1102,7 +1197,11 @@ static size_t work_bytes(const Batch& B)
     } else {
         n += mb * (size_t)B.n_hi * sizeof(float);
         if (B.desc.has_conv) n += mb * (size_t)B.n_c * sizeof(float2);
-        if (B.desc.mcdisk) n += mb * (size_t)B.n_hi * sizeof(uint64_t);
+        if (B.desc.mcdisk && B.mc_tw)
+            n += mb * (size_t)(B.mc_cap * (2 * sizeof(McDep) + sizeof(int2)) +
+                               (2 * (size_t)B.mc_ntiles + 1) * sizeof(int));
+        else if (B.desc.mcdisk)
+            n += mb * (size_t)B.n_hi * sizeof(uint64_t);
     }
     return n;
 }
@@ -1131,7 +1230,12 @@ static void alloc_work(Batch& B)
     } else {
         B.hi.alloc(mb * B.n_hi * sizeof(float));
         if (B.desc.has_conv) B.spec.alloc(mb * B.n_c * sizeof(float2));
-        if (B.desc.mcdisk) {
+        if (B.desc.mcdisk && B.mc_tw) {
+            B.mcbin.alloc(mb * (size_t)(B.mc_cap * (2 * sizeof(McDep) +
+                                                    sizeof(int2)) +
+                                        (2 * (size_t)B.mc_ntiles + 1) *
+                                            sizeof(int)));
+        } else if (B.desc.mcdisk) {
             // zeroed once; the conversion re-zeroes what it reads
             B.icube.alloc(mb * B.n_hi * sizeof(uint64_t));
             hipLaunchKernelGGL(batch_zero_u64, dim3(4096), dim3(256), 0,
@@ -1142,6 +1246,7 @@ static void alloc_work(Batch& B)
         }
     }
     B.work = true;
+    ++B.work_gen;
 }
 
 static void release_work(Batch& B)
@@ -1157,6 +1262,7 @@ static void release_work(Batch& B)
     B.hi.release();
     B.spec.release();
     B.icube.release();
+    B.mcbin.release();
     B.work = false;
 }
 
@@ -1731,7 +1837,40 @@ static void run_chunk(Batch& b, const float* params, int set0, int nsets,
     const gbkhip_grid& g = D.grid;
     const float* P = params + (size_t)set0 * D.pstride;
     float* hi = b.hi.as<float>();
-    if (D.mcdisk) {
+    if (D.mcdisk && b.mc_tw) {
+        GBK_CHECK(b.mc_cap > 0, GBKHIP_ERR_ARG,
+                  "mcdisk batch: reserve cloud capacity first");
+        const long long cap = b.mc_cap;
+        const int nt = b.mc_ntiles;
+        char* base = static_cast<char*>(b.mcbin.ptr);
+        const size_t mb = (size_t)D.max_batch;
+        McBin bin;
+        bin.rec = reinterpret_cast<McDep*>(base);
+        McDep* sorted = bin.rec + mb * cap;
+        bin.tr = reinterpret_cast<int2*>(sorted + mb * cap);
+        bin.tcount = reinterpret_cast<int*>(bin.tr + mb * cap);
+        int* toff = bin.tcount + mb * nt;
+        bin.tw = bin.th = b.mc_tw;
+        bin.ntx = b.mc_ntx;
+        hipLaunchKernelGGL(batch_zero_i32, dim3(ceil_div(nsets * nt, 256)),
+                           dim3(256), 0, c.stream, bin.tcount,
+                           (long long)nsets * nt);
+        hipLaunchKernelGGL(batch_zero_f32, dim3(4096), dim3(256), 0,
+                           c.stream, hi, (long long)nsets * b.n_hi);
+        const size_t lds = D.csum_len <= kMcCsumLds
+                               ? D.csum_len * sizeof(int) : 0;
+        const int bx = std::max(1, std::min(ceil_div(cap, 256),
+                                            8192 / std::max(nsets, 1)));
+        hipLaunchKernelGGL(batch_mcdisk_bin, dim3(bx, nsets), dim3(256), lds,
+                           c.stream, b.tmpl, b.lay, P, D.pstride, g, D.cflux,
+                           b.csum + (size_t)set0 * D.csum_len, D.csum_len,
+                           D.hoi, (uint32_t)D.seed, (uint32_t)(D.seed >> 32),
+                           bin, cap, nt);
+        GBK_HIP(hipGetLastError());
+        launch_mc_binned_tail(c, bin, sorted, toff, nt, nsets, 0, cap,
+                              b.csum + (size_t)set0 * D.csum_len, D.csum_len,
+                              g, b.qscale + set0, 1.0, hi, b.n_hi);
+    } else if (D.mcdisk) {
         // the same 256-block x 256-thread cloud loop for every set
         const size_t lds = D.csum_len <= kMcCsumLds
                                ? D.csum_len * sizeof(int) : 0;
@@ -1905,6 +2044,12 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
                           "mcdisk needs one density height trait per polar "
                           "trait");
                 B.thin = false;
+                B.mc_tw = mc_tile_width(D.grid.spec_size);
+                if (B.mc_tw) {
+                    B.mc_ntx = ceil_div(D.grid.spat_size[0], B.mc_tw);
+                    B.mc_ntiles = B.mc_ntx *
+                                  ceil_div(D.grid.spat_size[1], B.mc_tw);
+                }
             }
             plan_fused(*c, B);
             if ((D.path == 0 || D.path == 3) && plan_sep(B) &&
@@ -2243,6 +2388,8 @@ int gbkhip_group_evaluate(gbkhip_group* g, const float* const* params,
         }
         std::vector<uintptr_t> key;
         for (int i = 0; i < n; ++i) {
+            // a member whose work buffers were re-allocated needs a new graph
+            key.push_back((uintptr_t)g->members[i]->impl.work_gen);
             key.push_back((uintptr_t)params[i]);
             key.push_back((uintptr_t)nsets[i]);
             key.push_back((uintptr_t)chi2[i]);
@@ -2270,6 +2417,22 @@ int gbkhip_group_evaluate(gbkhip_group* g, const float* const* params,
             it = g->graphs.emplace(key, exec).first;
         }
         GBK_HIP(hipGraphLaunch(it->second, target));
+    });
+}
+
+int gbkhip_batch_reserve_clouds(gbkhip_batch* b, long long clouds)
+{
+    return guarded([&] {
+        GBK_CHECK(b != nullptr, GBKHIP_ERR_ARG, "null batch");
+        Batch& B = b->impl;
+        GBK_CHECK(B.desc.mcdisk, GBKHIP_ERR_ARG, "not an mcdisk batch");
+        GBK_CHECK(clouds >= 0 && clouds < (1LL << 31), GBKHIP_ERR_ARG,
+                  "cloud capacity out of range");
+        if (!B.mc_tw || clouds <= B.mc_cap) return;
+        GBK_HIP(hipSetDevice(B.device));
+        std::lock_guard<std::mutex> lk(B.ctx->mu);
+        release_work(B);             // re-allocated for the new capacity
+        B.mc_cap = (clouds + 1023) / 1024 * 1024;
     });
 }
 

@@ -99,11 +99,14 @@ __global__ __launch_bounds__(256) void mcdisk_bin_kernel(
     }
 }
 
-// off[t] = sum of count[0..t), off[n] = total: one block of 1024 threads.
+// off[t] = sum of count[0..t), off[n] = total: one block of 1024 threads
+// per set (blockIdx.x).
 __global__ __launch_bounds__(1024) void mc_scan_tiles(const int* count,
                                                       int* off, int n)
 {
     __shared__ int s[1024];
+    count += (size_t)blockIdx.x * n;
+    off += (size_t)blockIdx.x * (n + 1);
     const int t = threadIdx.x, per = (n + 1023) / 1024;
     const int lo = min(t * per, n), hi = min(lo + per, n);
     int sum = 0;
@@ -127,11 +130,22 @@ __global__ __launch_bounds__(1024) void mc_scan_tiles(const int* count,
     if (t == 1023) off[n] = run;
 }
 
+// Records to tile order; blockIdx.y = set.  A set's cloud count is its last
+// cumulative count (csum, per set) or n, clamped to the capacity.
 __global__ void mc_scatter(const McDep* __restrict__ rec,
                            const int2* __restrict__ tr,
                            const int* __restrict__ off,
-                           McDep* __restrict__ sorted, long long n)
+                           McDep* __restrict__ sorted, long long n,
+                           long long cap, const int* __restrict__ csum,
+                           int csum_len, int ntiles)
 {
+    const int set = blockIdx.y;
+    if (csum) n = min((long long)csum[(size_t)set * csum_len + csum_len - 1],
+                      cap);
+    rec += set * cap;
+    tr += set * cap;
+    sorted += set * cap;
+    off += (size_t)set * (ntiles + 1);
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
          i < n; i += (long long)gridDim.x * blockDim.x) {
         const int2 t = tr[i];
@@ -139,16 +153,21 @@ __global__ void mc_scatter(const McDep* __restrict__ rec,
     }
 }
 
-// One block per tile: the deposits of its clouds summed in LDS with int64
-// atomics (the same quantised values the global path adds), then cube +=
-// sum / 2^S as fixed_to_float does.
+// One block per (tile, set): the deposits of the tile's clouds summed in
+// LDS with int64 atomics (the same quantised values the global path adds),
+// then cube += sum / 2^S as fixed_to_float does.
 __global__ __launch_bounds__(256) void mc_tile_deposit(
     const McDep* __restrict__ sorted, const int* __restrict__ off,
-    gbkhip_grid g, int tw, int th, int ntx, double qscale,
-    float* __restrict__ cube)
+    gbkhip_grid g, int tw, int th, int ntx, const double* __restrict__ qs,
+    double qscale, float* __restrict__ cube, long long cap,
+    long long cstride)
 {
     extern __shared__ unsigned long long acc[];      // [nz][tw * th]
-    const int tile = blockIdx.x;
+    const int tile = blockIdx.x, set = blockIdx.y, ntiles = gridDim.x;
+    off += (size_t)set * (ntiles + 1);
+    sorted += set * cap;
+    cube += set * cstride;
+    if (qs) qscale = qs[set];
     const int lo = off[tile], hi = off[tile + 1];
     if (lo == hi) return;
     const int nx = g.spat_size[0], ny = g.spat_size[1], nz = g.spec_size;
@@ -180,6 +199,32 @@ __global__ __launch_bounds__(256) void mc_tile_deposit(
         if (x < nx && y < ny)
             cube[k * plane + (size_t)y * nx + x] += (float)((double)q * inv);
     }
+}
+
+int mc_tile_width(int nz)
+{
+    int tw = 8;
+    while (tw > 1 && (size_t)tw * tw * nz * 8 > 64 * 1024) tw /= 2;
+    return (size_t)tw * tw * nz * 8 <= 64 * 1024 ? tw : 0;
+}
+
+void launch_mc_binned_tail(Context& c, const McBin& bin, McDep* sorted,
+                           int* toff, int ntiles, int nsets, long long n,
+                           long long cap, const int* csum, int csum_len,
+                           const gbkhip_grid& g, const double* qs,
+                           double qscale, float* cube, long long cstride)
+{
+    hipLaunchKernelGGL(mc_scan_tiles, dim3(nsets), dim3(1024), 0, c.stream,
+                       bin.tcount, toff, ntiles);
+    const int sb = std::max(1, std::min(ceil_div(cap, 256), 8192 / nsets));
+    hipLaunchKernelGGL(mc_scatter, dim3(sb, nsets), dim3(256), 0, c.stream,
+                       bin.rec, bin.tr, toff, sorted, n, cap, csum, csum_len,
+                       ntiles);
+    hipLaunchKernelGGL(mc_tile_deposit, dim3(ntiles, nsets), dim3(256),
+                       (size_t)bin.tw * bin.th * g.spec_size * 8, c.stream,
+                       sorted, toff, g, bin.tw, bin.th, bin.ntx, qs, qscale,
+                       cube, cap, cstride);
+    GBK_HIP(hipGetLastError());
 }
 
 // cube += icube / 2^S, and icube <- 0 for the next call.
@@ -231,13 +276,11 @@ void launch_mcdisk(Context& c, float cflux, int nclouds, const int* csum,
     // binned deposits when the spectral cube is the only output and a tile
     // column of the spectrum fits LDS
     const int nz = g.spec_size;
-    int tw = 8;
-    while (tw > 1 && (size_t)tw * tw * nz * 8 > 64 * 1024) tw /= 2;
+    const int tw = mc_tile_width(nz);
     const bool only_scube = o.scube && !o.image && !o.wdata &&
         !o.wdata_cmp && !o.rdata && !o.rdata_cmp && !o.ordata &&
         !o.ordata_cmp && !o.vdata_cmp && !o.ddata_cmp;
-    const bool binned = only_scube && ncube > 0 &&
-                        (size_t)tw * tw * nz * 8 <= 64 * 1024;
+    const bool binned = only_scube && ncube > 0 && tw > 0;
     unsigned long long* icube = nullptr;
     if (o.scube && ncube > 0 && !binned) {
         const size_t bytes = (size_t)ncube * sizeof(unsigned long long);
@@ -287,13 +330,9 @@ void launch_mcdisk(Context& c, float cflux, int nclouds, const int* csum,
             std::max(1, std::min(ceil_div(nclouds, 256), kMcBinBlocks));
         hipLaunchKernelGGL(mcdisk_bin_kernel, dim3(bin_blocks), dim3(256),
                            lds, c.stream, mp, d, g, bin, ntiles);
-        hipLaunchKernelGGL(mc_scan_tiles, dim3(1), dim3(1024), 0, c.stream,
-                           bin.tcount, toff, ntiles);
-        hipLaunchKernelGGL(mc_scatter, dim3(blocks), dim3(256), 0, c.stream,
-                           bin.rec, bin.tr, toff, sorted, (long long)nc);
-        hipLaunchKernelGGL(mc_tile_deposit, dim3(ntiles), dim3(256),
-                           (size_t)tw * tw * nz * 8, c.stream, sorted, toff,
-                           g, tw, tw, ntx, mp.qscale, o.scube);
+        launch_mc_binned_tail(c, bin, sorted, toff, ntiles, 1, (long long)nc,
+                              (long long)nc, nullptr, 0, g, nullptr,
+                              mp.qscale, o.scube, 0);
         GBK_HIP(hipGetLastError());
         return;
     }

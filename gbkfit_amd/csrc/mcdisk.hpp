@@ -123,6 +123,19 @@ struct McBin {
     int tw, th, ntx;         // tile width / height (spaxels), tiles along x
 };
 
+struct Context;
+// Tile width (spaxels) of binned deposits for nz channels: the largest
+// power of two whose tile spectrum (tw^2 nz int64) fits 64 KB of LDS, or 0.
+int mc_tile_width(int nz);
+// Tile offsets, tile-ordered records and per-tile LDS deposits for nsets
+// sets whose records, ranks and tile counts were written (per set strides:
+// cap records, ntiles counts, ntiles + 1 offsets); cube += the deposits.
+void launch_mc_binned_tail(Context& c, const McBin& bin, McDep* sorted,
+                           int* toff, int ntiles, int nsets, long long n,
+                           long long cap, const int* csum, int csum_len,
+                           const gbkhip_grid& g, const double* qs,
+                           double qscale, float* cube, long long cstride);
+
 struct McParams {
     float cflux;
     int nclouds;

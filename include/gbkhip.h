@@ -258,6 +258,11 @@ int gbkhip_batch_evaluate_clouds(gbkhip_batch* batch, const float* params,
                                  const int* csum, const double* qscale,
                                  int nsets, double* chi2, float* cubes,
                                  int use_graph);
+/* Cloud capacity per set of an mcdisk batch (its binned deposits keep one
+ * record per cloud): at least the largest total cloud count of the sets
+ * evaluated next.  Growing it frees the work buffers (re-allocated by the
+ * next evaluation) and the cached graphs. */
+int gbkhip_batch_reserve_clouds(gbkhip_batch* batch, long long clouds);
 /* Fixed-point scale 2^S of the int64 cube of an mcdisk evaluation of nclouds
  * clouds of flux cflux on grid: the value gbkhip_mcdisk_evaluate uses. */
 int gbkhip_mcdisk_fixed_scale(float cflux, int nclouds,
