@@ -619,30 +619,16 @@ __global__ __launch_bounds__(256) void batch_mcdisk_bin(
     }
     gbkhip_outputs o;
     memset(&o, 0, sizeof(o));
+    __shared__ int s_wscr[4][128];
+    int* wscr = s_wscr[threadIdx.x >> 6];
     const long long stride = (long long)gridDim.x * blockDim.x;
-    const int lane = __lane_id();
-    const unsigned long long below = (1ull << lane) - 1ull;
-    for (long long ci = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-         ci < mp.nclouds; ci += stride) {
-        const int tile = mc_cloud(ci, mp, d, g, o, nullptr, &bin);
-        const bool has = tile >= 0;
-        unsigned long long todo = __ballot(has);
-        int rank = 0;
-        while (todo) {
-            const int leader = __builtin_ctzll(todo);
-            const int t = __shfl(tile, leader);
-            const unsigned long long same = __ballot(has && tile == t);
-            int base = 0;
-            if (lane == leader) {
-                const int n = __popcll(same);
-                base = local ? atomicAdd(&s_cnt[t], n)
-                             : atomicAdd(&bin.tcount[t], n);
-            }
-            base = __shfl(base, leader);
-            if (has && tile == t) rank = base + __popcll(same & below);
-            todo &= ~same;
-        }
-        bin.tr[ci] = make_int2(tile, rank);
+    for (long long b0 = (long long)blockIdx.x * blockDim.x; b0 < mp.nclouds;
+         b0 += stride) {
+        const long long ci = b0 + threadIdx.x;
+        const int tile = ci < mp.nclouds
+                             ? mc_cloud(ci, mp, d, g, o, nullptr, &bin) : -1;
+        const int rank = wave_rank(tile, local ? s_cnt : bin.tcount, wscr);
+        if (ci < mp.nclouds) bin.tr[ci] = make_int2(tile, rank);
     }
     if (!local) return;
     __syncthreads();

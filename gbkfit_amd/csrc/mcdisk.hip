@@ -54,35 +54,20 @@ __global__ __launch_bounds__(256) void mcdisk_bin_kernel(
     }
     gbkhip_outputs o;
     memset(&o, 0, sizeof(o));
+    __shared__ int s_wscr[4][128];
+    int* wscr = s_wscr[threadIdx.x >> 6];
     const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long ci = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-         ci < mp.nclouds; ci += stride) {
-        const int tile = mc_cloud(ci, mp, d, g, o, nullptr, &bin);
+    // every lane of the block runs every round (the rank is a wave-wide
+    // operation); lanes past the last cloud carry no key
+    for (long long b0 = (long long)blockIdx.x * blockDim.x; b0 < mp.nclouds;
+         b0 += stride) {
+        const long long ci = b0 + threadIdx.x;
+        const int tile = ci < mp.nclouds
+                             ? mc_cloud(ci, mp, d, g, o, nullptr, &bin) : -1;
         // rank within the block's clouds of the tile (the block reserves
-        // its ranges afterwards), or directly global.  Wave-aggregated: the
-        // clouds of a disk crowd into its central tiles, and atomics on one
-        // address serialise, so each distinct tile of the wave takes one
-        // atomic for all its lanes, which rank by lane order.
-        const bool has = tile >= 0;
-        const int lane = __lane_id();
-        const unsigned long long below = (1ull << lane) - 1ull;
-        unsigned long long todo = __ballot(has);
-        int rank = 0;
-        while (todo) {
-            const int leader = __builtin_ctzll(todo);
-            const int t = __shfl(tile, leader);
-            const unsigned long long same = __ballot(has && tile == t);
-            int base = 0;
-            if (lane == leader) {
-                const int n = __popcll(same);
-                base = local ? atomicAdd(&s_cnt[t], n)
-                             : atomicAdd(&bin.tcount[t], n);
-            }
-            base = __shfl(base, leader);
-            if (has && tile == t) rank = base + __popcll(same & below);
-            todo &= ~same;
-        }
-        bin.tr[ci] = make_int2(tile, rank);
+        // its ranges afterwards), or directly global
+        const int rank = wave_rank(tile, local ? s_cnt : bin.tcount, wscr);
+        if (ci < mp.nclouds) bin.tr[ci] = make_int2(tile, rank);
     }
     if (!local) return;
     __syncthreads();

@@ -123,6 +123,51 @@ struct McBin {
     int tw, th, ntx;         // tile width / height (spaxels), tiles along x
 };
 
+// Rank of each lane's key among the lanes of its wave with the same key,
+// offset by a reservation of as many slots in counters[key]: the wave sorts
+// its 64 keys (bitonic, across lanes), each run of equal keys reserves with
+// one atomic at its last lane, and the ranks go back to their lanes through
+// 2 x 64 ints of LDS per wave (wscr).  Lanes with key < 0 get 0.  All lanes
+// of the wave must call it (inactive lanes pass key = -1).  Same-address
+// atomics serialise; this takes one per distinct key and wave, all in one
+// instruction.
+__device__ __forceinline__ int wave_rank(int key, int* counters, int* wscr)
+{
+    const int lane = __lane_id();
+    int k = key < 0 ? 0x7fffffff : key, idx = lane;
+#pragma unroll
+    for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+        for (int j = size >> 1; j > 0; j >>= 1) {
+            const int pk = __shfl_xor(k, j, 64), pi = __shfl_xor(idx, j, 64);
+            const bool up = (lane & size) == 0, lower = (lane & j) == 0;
+            const bool pless = pk < k || (pk == k && pi < idx);
+            if (lower == up ? pless : !pless) {
+                k = pk;
+                idx = pi;
+            }
+        }
+    }
+    // runs of equal keys: start lane by an inclusive max-scan of head lanes
+    const int prev = __shfl_up(k, 1, 64), next = __shfl_down(k, 1, 64);
+    const bool head = lane == 0 || prev != k;
+    const bool tail = lane == 63 || next != k;
+    int start = head ? lane : 0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(start, o, 64);
+        if (lane >= o) start = max(start, v);
+    }
+    if (tail && k != 0x7fffffff)
+        wscr[start] = atomicAdd(&counters[k], lane - start + 1);
+    __builtin_amdgcn_wave_barrier();
+    const int rank = k != 0x7fffffff ? wscr[start] + lane - start : 0;
+    __builtin_amdgcn_wave_barrier();
+    wscr[64 + idx] = rank;
+    __builtin_amdgcn_wave_barrier();
+    return wscr[64 + lane];
+}
+
 struct Context;
 // Tile width (spaxels) of binned deposits for nz channels: the largest
 // power of two whose tile spectrum (tw^2 nz int64) fits 64 KB of LDS, or 0.
