@@ -180,7 +180,8 @@ class BatchEvaluator:
 
     dmodel: DModelSCube (its DCube geometry, PSF and LSF are used);
     gmodel: GModelKinematics2D/3D with exactly one SpectralSMDisk or
-            SpectralMCDisk component (mcdisk: padded-cube pipeline, each
+            SpectralMCDisk component (thick smdisk and mcdisk: direct or
+            padded-cube pipeline from a deposited hi-res cube; mcdisk: each
             set's cloud budget computed on the host as the reference does,
             src/gbkfit/model/gmodels/_mcdisk.py:16-189);
     obs:    observed low-res cube (numpy, z-y-x); error: scalar or cube.
@@ -201,6 +202,9 @@ class BatchEvaluator:
         be_fft = driver.backends().fft(cube.dtype(), cube._padding)
         size_hi, step_hi, edge_hi, zero_hi = cube.geometry(be_fft)
         thin = getattr(comps[0], 'thin', True)
+        # thick disks and mcdisk deposit a hi-res cube that the direct
+        # pipeline's x pass reads (thin disks deposit into its LDS tiles)
+        self.cube_source = self.mc or not thin
         if thin:
             spat = ((size_hi[0], size_hi[1], 1), (step_hi[0], step_hi[1], 0),
                     (zero_hi[0], zero_hi[1], 0))
@@ -445,6 +449,8 @@ class BatchEvaluator:
     PADDED_STAGES = ('smdisk', 'fft_convolution', 'downscale_chi2')
     DIRECT_KERNELS = ('batch_lines_thin', 'sep_xpass', 'sep_ypass',
                       'sep_zpass', 'batch_chi2_final')
+    DIRECT_CUBE_KERNELS = ('model_deposit', 'sep_xpass_cube', 'sep_ypass',
+                           'sep_zpass', 'batch_chi2_final')
 
     def kernel_times(self):
         """{kernel: milliseconds} of the last timed evaluation, summed over
@@ -452,8 +458,10 @@ class BatchEvaluator:
         ms = (_c.c_double * 8)()
         n = _c.c_int()
         N.call('gbkhip_batch_kernel_times', self.handle, ms, 8, _c.byref(n))
-        names = {7: self.FUSED_KERNELS, 5: self.DIRECT_KERNELS}.get(
-            n.value, self.PADDED_STAGES)
+        direct = (self.DIRECT_CUBE_KERNELS if self.cube_source
+                  else self.DIRECT_KERNELS)
+        names = {7: self.FUSED_KERNELS, 5: direct}.get(n.value,
+                                                       self.PADDED_STAGES)
         return {k: ms[i] for i, k in enumerate(names[:n.value])}
 
     def close(self):
@@ -529,7 +537,8 @@ def autotune(driver, dmodel, gmodel, obs, sample_params, error=None,
 
     mbs = sorted({min(max(1, n), m) for m in max_batches})
     trials, best = [], None            # best: (s per set, path, mb, tiles)
-    for path in (('padded',) if mc else ('direct', 'fused', 'padded')):
+    for path in (('direct', 'padded') if mc
+                 else ('direct', 'fused', 'padded')):
         try:
             probe = make(path, 1)
         except N.GbkHipError:

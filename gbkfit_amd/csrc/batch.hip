@@ -720,11 +720,57 @@ __host__ __device__ constexpr int sep_dxb(int stride)
     return d;
 }
 
+// S1 filter: the block's LDS tile M [zc][mp] holds the hi-res values of
+// one row for channels [z0, z0 + zn); a thread filters J consecutive X
+// outputs of one channel into out (the row's P1 line of channel 0).
+// Outputs (z, X .. X + J - 1) read inputs sx X + [0, sx (J - 1) + TX).
+// Lane l of a wave takes channel l % 16; its four 16-lane groups take
+// output blocks DXB apart, whose inputs start 16 banks apart (SX J DXB = 16
+// mod 64), and the 16 channels of a group, mp (odd) apart, hit 16 distinct
+// banks: no bank conflicts with 32 or 64 banks.
+template <int SX, int TX>
+__device__ __forceinline__ void sep_xfilter(const float* M, const float* s_w,
+                                            int mp, int zn, int z0, int nxl,
+                                            int ny1, float* __restrict__ out)
+{
+    constexpr int T = kSepThreads, J = sep_jx(TX);
+    constexpr int NIN = SX * (J - 1) + TX;
+    constexpr int DXB = sep_dxb(SX * J);
+    const int nxb = (nxl + J - 1) / J;
+    const int lane = threadIdx.x & 63, zz = lane & 15, grp = lane >> 4;
+    for (int w = threadIdx.x >> 6;; w += T / 64) {
+        const int g = w / DXB, xb = g * 4 * DXB + w % DXB + grp * DXB;
+        if (g * 4 * DXB >= nxb) break;           // uniform over the wave
+        if (zz >= zn || xb >= nxb) continue;
+        const int X0 = xb * J;
+        const float* in = M + zz * mp + SX * X0;
+        float v[NIN];
+#pragma unroll
+        for (int i = 0; i < NIN; ++i) v[i] = in[i];
+        float o[J] = {};
+#pragma unroll
+        for (int u = 0; u < TX; ++u) {
+            const float w = s_w[u];
+#pragma unroll
+            for (int j = 0; j < J; ++j) o[j] = fmaf(w, v[SX * j + u], o[j]);
+        }
+        float* dst = out + (size_t)(z0 + zz) * ny1 * nxl + X0;
+        if (X0 + J <= nxl) {
+#pragma unroll
+            for (int j = 0; j < J; j += 4)
+                *reinterpret_cast<float4*>(dst + j) =
+                    make_float4(o[j], o[j + 1], o[j + 2], o[j + 3]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < J; ++j)
+                if (X0 + j < nxl) dst[j] = o[j];
+        }
+    }
+}
+
 // S1: block = (row y, channels [z0, z0 + zn), set).  The hi-res values of
 // the row are deposited into an LDS tile [zc][mp] (zero outside each
-// spaxel's rint window, gmodels.hpp:91-116); a thread then filters J
-// consecutive X outputs of one channel.  Lanes run over channels and mp is
-// odd, so the lanes of a wave read distinct banks.
+// spaxel's rint window, gmodels.hpp:91-116), then filtered along x.
 template <int SX, int TX>
 __global__ __launch_bounds__(256) void sep_xpass(
     const float4* __restrict__ lines, long long lstride, int lpitch,
@@ -735,7 +781,7 @@ __global__ __launch_bounds__(256) void sep_xpass(
     extern __shared__ float4 M4[];        // [zc][mp] floats
     float* M = reinterpret_cast<float*>(M4);
     __shared__ float s_w[TX];
-    constexpr int T = kSepThreads, J = sep_jx(TX);
+    constexpr int T = kSepThreads;
     const int y = blockIdx.x, z0 = blockIdx.y * zc, set = blockIdx.z;
     // a tile no line reaches filters to zeros: skipped (its rows are not
     // written; the y pass reads the same mask and uses zeros)
@@ -765,44 +811,58 @@ __global__ __launch_bounds__(256) void sep_xpass(
         }
     }
     __syncthreads();
-    // outputs (z, X .. X + J - 1): inputs sx X + [0, sx (J - 1) + TX).
-    // Lane l of a wave takes channel l % 16; its four 16-lane groups take
-    // output blocks DXB apart, whose inputs start 16 banks apart (SX J DXB
-    // = 16 mod 64), and the 16 channels of a group, mp (odd) apart, hit 16
-    // distinct banks: no bank conflicts with 32 or 64 banks.
-    constexpr int NIN = SX * (J - 1) + TX;
-    constexpr int DXB = sep_dxb(SX * J);
-    const int nxb = (nxl + J - 1) / J;
-    const int lane = threadIdx.x & 63, zz = lane & 15, grp = lane >> 4;
-    float* out = P1 + set * p1stride + (size_t)y * nxl;
-    for (int w = threadIdx.x >> 6;; w += T / 64) {
-        const int g = w / DXB, xb = g * 4 * DXB + w % DXB + grp * DXB;
-        if (g * 4 * DXB >= nxb) break;           // uniform over the wave
-        if (zz >= zn || xb >= nxb) continue;
-        const int X0 = xb * J;
-        const float* in = M + zz * mp + SX * X0;
-        float v[NIN];
+    sep_xfilter<SX, TX>(M, s_w, mp, zn, z0, nxl, ny1,
+                        P1 + set * p1stride + (size_t)y * nxl);
+}
+
+// S1 from a deposited hi-res cube (thick disks, mcdisk): block = (row y,
+// channels [z0, z0 + zn), set).  The row's need_x values of each channel
+// are read from the cube (a wave per channel row, 8 loads in flight per
+// lane) into the LDS tile, zero beyond need_x.  A tile that is zero
+// everywhere filters to zeros: it returns without writing P1 and leaves its
+// bit of the row mask clear (the mask starts at zero; the y pass reads it
+// and uses zeros for the row); otherwise it sets the bit and filters.
+template <int SX, int TX>
+__global__ __launch_bounds__(256) void sep_xpass_cube(
+    const float* __restrict__ hi, long long hstride, int hnx, int hny,
+    const float* __restrict__ wx, int need_x, int need_z, int nxl, int mp,
+    int zc, float* __restrict__ P1, long long p1stride, int ny1,
+    unsigned long long* __restrict__ rowmask)
+{
+    extern __shared__ float4 M4[];        // [zc][mp] floats
+    float* M = reinterpret_cast<float*>(M4);
+    __shared__ float s_w[TX];
+    constexpr int T = kSepThreads, U = 8;
+    const int y = blockIdx.x, z0 = blockIdx.y * zc, set = blockIdx.z;
+    const int zn = min(zc, need_z - z0);
+    for (int i = threadIdx.x; i < TX; i += T) s_w[i] = wx[i];
+    const int lane = threadIdx.x & 63;
+    const size_t plane = (size_t)hny * hnx;
+    int live = 0;
+    for (int zz = threadIdx.x >> 6; zz < zn; zz += T / 64) {
+        const float* src = hi + set * hstride + (size_t)(z0 + zz) * plane +
+                           (size_t)y * hnx;
+        float* dst = M + zz * mp;
+        for (int x0 = lane; x0 < mp; x0 += 64 * U) {
+            float t[U];
 #pragma unroll
-        for (int i = 0; i < NIN; ++i) v[i] = in[i];
-        float o[J] = {};
+            for (int u = 0; u < U; ++u) {
+                const int x = x0 + 64 * u;
+                t[u] = x < need_x ? src[x] : 0.0f;
+            }
 #pragma unroll
-        for (int u = 0; u < TX; ++u) {
-            const float w = s_w[u];
-#pragma unroll
-            for (int j = 0; j < J; ++j) o[j] = fmaf(w, v[SX * j + u], o[j]);
-        }
-        float* dst = out + (size_t)(z0 + zz) * ny1 * nxl + X0;
-        if (X0 + J <= nxl) {
-#pragma unroll
-            for (int j = 0; j < J; j += 4)
-                *reinterpret_cast<float4*>(dst + j) =
-                    make_float4(o[j], o[j + 1], o[j + 2], o[j + 3]);
-        } else {
-#pragma unroll
-            for (int j = 0; j < J; ++j)
-                if (X0 + j < nxl) dst[j] = o[j];
+            for (int u = 0; u < U; ++u) {
+                const int x = x0 + 64 * u;
+                if (x < mp) dst[x] = t[u];
+                live |= t[u] != 0.0f;
+            }
         }
     }
+    if (!__syncthreads_or(live)) return;
+    if (threadIdx.x == 0)
+        atomicOr(&rowmask[(size_t)set * ny1 + y], 1ull << blockIdx.y);
+    sep_xfilter<SX, TX>(M, s_w, mp, zn, z0, nxl, ny1,
+                        P1 + set * p1stride + (size_t)y * nxl);
 }
 
 // S2: P2[z][Y][X] = sum_v Wy[v] P1[z][sy Y + v][X].  Block = (32 X
@@ -1009,6 +1069,7 @@ __global__ __launch_bounds__(256) void sep_zpass(
 
 struct Sep {
     bool on = false;
+    bool cube = false;            // S1 reads a deposited hi-res cube
     int need[3] = {0, 0, 0};      // inputs that reach the outputs
     int tx = 0, ty = 0, tz = 0;   // tap buckets (zero-padded)
     int mp = 0;                   // S1 LDS row length (odd)
@@ -1168,12 +1229,24 @@ static void build_layout(Batch& b)
 // cubes, line parameters, chi^2 partials).  They can be released between
 // evaluations and are re-allocated by the next one; the resident state
 // (layout, kernel spectra, plans, twiddles) stays.
+// Binned mcdisk deposits of one lane (max_batch sets): records, ranks,
+// sorted records, tile counts and offsets, 256-byte aligned.
+static size_t mc_lane_bytes(const Batch& B)
+{
+    const size_t set = (size_t)B.mc_cap * (2 * sizeof(McDep) + sizeof(int2)) +
+                       (2 * (size_t)B.mc_ntiles + 1) * sizeof(int);
+    return ((size_t)B.desc.max_batch * set + 255) / 256 * 256;
+}
+
 static size_t work_bytes(const Batch& B)
 {
     const size_t mb = (size_t)B.desc.max_batch * B.lanes;
     size_t n = mb * (size_t)B.nparts * sizeof(double);
     if (B.sep.on) {
-        n += mb * (size_t)B.sep.need[1] * B.sep.need[0] * sizeof(float4);
+        if (B.sep.cube)
+            n += mb * (size_t)B.n_hi * sizeof(float);
+        else
+            n += mb * (size_t)B.sep.need[1] * B.sep.need[0] * sizeof(float4);
         n += mb * (size_t)(B.sep.p1stride + B.sep.p2stride) * sizeof(float);
         n += mb * (size_t)B.sep.need[1] * sizeof(uint64_t);
         n += mb * (size_t)B.sep.need[2] * B.sep.ngy;
@@ -1183,12 +1256,11 @@ static size_t work_bytes(const Batch& B)
     } else {
         n += mb * (size_t)B.n_hi * sizeof(float);
         if (B.desc.has_conv) n += mb * (size_t)B.n_c * sizeof(float2);
-        if (B.desc.mcdisk && B.mc_tw)
-            n += mb * (size_t)(B.mc_cap * (2 * sizeof(McDep) + sizeof(int2)) +
-                               (2 * (size_t)B.mc_ntiles + 1) * sizeof(int));
-        else if (B.desc.mcdisk)
-            n += mb * (size_t)B.n_hi * sizeof(uint64_t);
     }
+    if (B.desc.mcdisk && B.mc_tw)
+        n += (size_t)B.lanes * mc_lane_bytes(B);
+    else if (B.desc.mcdisk)
+        n += mb * (size_t)B.n_hi * sizeof(uint64_t);
     return n;
 }
 
@@ -1204,8 +1276,11 @@ static void alloc_work(Batch& B)
     const size_t mb = (size_t)B.desc.max_batch * B.lanes;
     B.partials.alloc(mb * B.nparts * sizeof(double));
     if (B.sep.on) {
-        B.fused.lines.alloc(mb * (size_t)B.sep.need[1] * B.sep.need[0] *
-                            sizeof(float4));
+        if (B.sep.cube)
+            B.hi.alloc(mb * B.n_hi * sizeof(float));
+        else
+            B.fused.lines.alloc(mb * (size_t)B.sep.need[1] * B.sep.need[0] *
+                                sizeof(float4));
         B.sep.P1.alloc(mb * (size_t)B.sep.p1stride * sizeof(float));
         B.sep.P2.alloc(mb * (size_t)B.sep.p2stride * sizeof(float));
         B.sep.masks.alloc(mb * (size_t)B.sep.need[1] * sizeof(uint64_t));
@@ -1216,20 +1291,16 @@ static void alloc_work(Batch& B)
     } else {
         B.hi.alloc(mb * B.n_hi * sizeof(float));
         if (B.desc.has_conv) B.spec.alloc(mb * B.n_c * sizeof(float2));
-        if (B.desc.mcdisk && B.mc_tw) {
-            B.mcbin.alloc(mb * (size_t)(B.mc_cap * (2 * sizeof(McDep) +
-                                                    sizeof(int2)) +
-                                        (2 * (size_t)B.mc_ntiles + 1) *
-                                            sizeof(int)));
-        } else if (B.desc.mcdisk) {
-            // zeroed once; the conversion re-zeroes what it reads
-            B.icube.alloc(mb * B.n_hi * sizeof(uint64_t));
-            hipLaunchKernelGGL(batch_zero_u64, dim3(4096), dim3(256), 0,
-                               B.ctx->stream,
-                               B.icube.as<unsigned long long>(),
-                               (long long)(mb * B.n_hi));
-            GBK_HIP(hipGetLastError());
-        }
+    }
+    if (B.desc.mcdisk && B.mc_tw) {
+        B.mcbin.alloc((size_t)B.lanes * mc_lane_bytes(B));
+    } else if (B.desc.mcdisk) {
+        // zeroed once; the conversion re-zeroes what it reads
+        B.icube.alloc(mb * B.n_hi * sizeof(uint64_t));
+        hipLaunchKernelGGL(batch_zero_u64, dim3(4096), dim3(256), 0,
+                           B.ctx->stream, B.icube.as<unsigned long long>(),
+                           (long long)(mb * B.n_hi));
+        GBK_HIP(hipGetLastError());
     }
     B.work = true;
     ++B.work_gen;
@@ -1301,7 +1372,7 @@ static void plan_fused(Context& c, Batch& B)
     const gbkhip_batch_desc& D = B.desc;
     Fused& F = B.fused;
     F.on = false;
-    if (!B.thin || !D.has_conv || D.path == 1) return;
+    if (!D.has_conv || D.path == 1) return;
     const int Nx = D.size_hi[0], Ny = D.size_hi[1], Nz = D.size_hi[2];
     const size_t N = (size_t)Nx * Ny * Nz;
     std::vector<float> pc(N);
@@ -1340,6 +1411,9 @@ static void plan_fused(Context& c, Batch& B)
     B.ka_psf = psf;
     B.ka_lsf = lsf;
     B.ka_total = total;
+    // the kernel analysis above also serves the direct pipeline of thick
+    // disks and mcdisk; the spectral-first deposit is for thin disks only
+    if (!B.thin || D.path == 3) return;
     for (int i = 0; i < 3; ++i)
         F.need[i] = D.size_lo[i] * D.scale[i] + 2 * D.edge_hi[i];
     // FFT sizes with engine shapes: x rows, y lines, z columns of 64 or 32
@@ -1697,6 +1771,73 @@ static bool sep_cheaper(const Batch& B)
         throw Error{GBKHIP_ERR_INTERNAL, "no separable kernel for shape"};   \
     }
 
+// Hi-res model cubes of sets [set0, set0 + nsets) of a chunk on `lane` into
+// hi (n_hi per set): mcdisk (binned deposits when a tile spectrum fits LDS,
+// else int64 atomics into a fixed-point cube) or a thick smdisk.
+static void deposit_hi(Batch& b, const float* P, int set0, int nsets,
+                       int lane, float* hi)
+{
+    Context& c = *b.ctx;
+    const gbkhip_batch_desc& D = b.desc;
+    const gbkhip_grid& g = D.grid;
+    if (D.mcdisk && b.mc_tw) {
+        GBK_CHECK(b.mc_cap > 0, GBKHIP_ERR_ARG,
+                  "mcdisk batch: reserve cloud capacity first");
+        const long long cap = b.mc_cap;
+        const int nt = b.mc_ntiles;
+        char* base = static_cast<char*>(b.mcbin.ptr) +
+                     (size_t)lane * mc_lane_bytes(b);
+        const size_t mb = (size_t)D.max_batch;
+        McBin bin;
+        bin.rec = reinterpret_cast<McDep*>(base);
+        McDep* sorted = bin.rec + mb * cap;
+        bin.tr = reinterpret_cast<int2*>(sorted + mb * cap);
+        bin.tcount = reinterpret_cast<int*>(bin.tr + mb * cap);
+        int* toff = bin.tcount + mb * nt;
+        bin.tw = bin.th = b.mc_tw;
+        bin.ntx = b.mc_ntx;
+        hipLaunchKernelGGL(batch_zero_i32, dim3(ceil_div(nsets * nt, 256)),
+                           dim3(256), 0, c.stream, bin.tcount,
+                           (long long)nsets * nt);
+        hipLaunchKernelGGL(batch_zero_f32, dim3(4096), dim3(256), 0,
+                           c.stream, hi, (long long)nsets * b.n_hi);
+        const size_t lds = D.csum_len <= kMcCsumLds
+                               ? D.csum_len * sizeof(int) : 0;
+        const int bx = std::max(1, std::min(ceil_div(cap, 256),
+                                            8192 / std::max(nsets, 1)));
+        hipLaunchKernelGGL(batch_mcdisk_bin, dim3(bx, nsets), dim3(256), lds,
+                           c.stream, b.tmpl, b.lay, P, D.pstride, g, D.cflux,
+                           b.csum + (size_t)set0 * D.csum_len, D.csum_len,
+                           D.hoi, (uint32_t)D.seed, (uint32_t)(D.seed >> 32),
+                           bin, cap, nt);
+        GBK_HIP(hipGetLastError());
+        launch_mc_binned_tail(c, bin, sorted, toff, nt, nsets, 0, cap,
+                              b.csum + (size_t)set0 * D.csum_len, D.csum_len,
+                              g, b.qscale + set0, 1.0, hi, b.n_hi);
+    } else if (D.mcdisk) {
+        // the same 256-block x 256-thread cloud loop for every set
+        unsigned long long* icube = b.icube.as<unsigned long long>() +
+                                    (size_t)lane * D.max_batch * b.n_hi;
+        const size_t lds = D.csum_len <= kMcCsumLds
+                               ? D.csum_len * sizeof(int) : 0;
+        hipLaunchKernelGGL(batch_mcdisk, dim3(256, nsets), dim3(256), lds,
+                           c.stream, b.tmpl, b.lay, P, D.pstride, g, D.cflux,
+                           b.csum + (size_t)set0 * D.csum_len, D.csum_len,
+                           D.hoi, (uint32_t)D.seed, (uint32_t)(D.seed >> 32),
+                           b.qscale + set0, icube, b.n_hi);
+        GBK_HIP(hipGetLastError());
+        hipLaunchKernelGGL(batch_fixed_to_hi, dim3(1024, nsets), dim3(256), 0,
+                           c.stream, icube, hi, b.n_hi, b.qscale + set0);
+    } else {
+        const int chunk = std::max(1, std::min(g.spec_size, 240));
+        dim3 grid(ceil_div(g.spat_size[0], 64), g.spat_size[1], nsets);
+        hipLaunchKernelGGL(batch_smdisk_thick, grid, dim3(256),
+                           (size_t)chunk * 64 * sizeof(float), c.stream,
+                           b.tmpl, b.lay, P, D.pstride, g, hi, b.n_hi, chunk);
+    }
+    GBK_HIP(hipGetLastError());
+}
+
 static void run_chunk_sep(Batch& b, const float* params, int set0,
                           int nsets, double* chi2, float* cubes, int ci,
                           int lane)
@@ -1709,7 +1850,8 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
     const int nx = P.need[0], ny = P.need[1], nz = P.need[2];
     const size_t mb = (size_t)D.max_batch * lane;    // sets before this lane
     const long long lstride = (long long)ny * nx;
-    float4* lines = b.fused.lines.as<float4>() + mb * lstride;
+    float4* lines = P.cube ? nullptr
+                           : b.fused.lines.as<float4>() + mb * lstride;
     float* P1 = P.P1.as<float>() + mb * P.p1stride;
     float* P2 = P.P2.as<float>() + mb * P.p2stride;
     double* partials = b.partials.as<double>() + mb * b.nparts;
@@ -1718,7 +1860,23 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
     unsigned char* gflags = P.gflags.as<unsigned char>() +
                             mb * (size_t)nz * P.ngy;
     if (lane == 0) b.sep.last_sets = nsets;
-    {
+    if (P.cube) {
+        // model deposited into the hi-res cube; the x pass sets the masks
+        float* hi = b.hi.as<float>() + mb * b.n_hi;
+        hipLaunchKernelGGL(sep_zero_masks, dim3(ceil_div(nsets * ny, 256)),
+                           dim3(256), 0, c.stream, masks,
+                           (long long)nsets * ny);
+        GBK_HIP(hipGetLastError());
+        deposit_hi(b, par, set0, nsets, lane, hi);
+        mark(b, ci, 1);
+        dim3 grid(ny, ceil_div(nz, P.zc), nsets);
+        const size_t lds = (size_t)P.zc * P.mp * sizeof(float);
+        GBK_SEP_DISPATCH(sep_xpass_cube, D.scale[0], P.tx, grid,
+                         dim3(kSepThreads), lds, c.stream, hi, b.n_hi,
+                         D.size_hi[0], D.size_hi[1], P.wx.as<float>(), nx,
+                         nz, nxl, P.mp, P.zc, P1, P.p1stride, ny, masks);
+        GBK_HIP(hipGetLastError());
+    } else {
         hipLaunchKernelGGL(sep_zero_masks, dim3(ceil_div(nsets * ny, 256)),
                            dim3(256), 0, c.stream, masks,
                            (long long)nsets * ny);
@@ -1728,9 +1886,7 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
                            b.tmpl, b.lay, par, D.pstride, D.grid, nx, ny, nx,
                            ny, lines, lstride, 1, masks, P.zc, nz);
         GBK_HIP(hipGetLastError());
-    }
-    mark(b, ci, 1);
-    {
+        mark(b, ci, 1);
         dim3 grid(ny, ceil_div(nz, P.zc), nsets);
         const size_t lds = (size_t)P.zc * P.mp * sizeof(float);
         GBK_SEP_DISPATCH(sep_xpass, D.scale[0], P.tx, grid, dim3(kSepThreads),
@@ -1823,66 +1979,15 @@ static void run_chunk(Batch& b, const float* params, int set0, int nsets,
     const gbkhip_grid& g = D.grid;
     const float* P = params + (size_t)set0 * D.pstride;
     float* hi = b.hi.as<float>();
-    if (D.mcdisk && b.mc_tw) {
-        GBK_CHECK(b.mc_cap > 0, GBKHIP_ERR_ARG,
-                  "mcdisk batch: reserve cloud capacity first");
-        const long long cap = b.mc_cap;
-        const int nt = b.mc_ntiles;
-        char* base = static_cast<char*>(b.mcbin.ptr);
-        const size_t mb = (size_t)D.max_batch;
-        McBin bin;
-        bin.rec = reinterpret_cast<McDep*>(base);
-        McDep* sorted = bin.rec + mb * cap;
-        bin.tr = reinterpret_cast<int2*>(sorted + mb * cap);
-        bin.tcount = reinterpret_cast<int*>(bin.tr + mb * cap);
-        int* toff = bin.tcount + mb * nt;
-        bin.tw = bin.th = b.mc_tw;
-        bin.ntx = b.mc_ntx;
-        hipLaunchKernelGGL(batch_zero_i32, dim3(ceil_div(nsets * nt, 256)),
-                           dim3(256), 0, c.stream, bin.tcount,
-                           (long long)nsets * nt);
-        hipLaunchKernelGGL(batch_zero_f32, dim3(4096), dim3(256), 0,
-                           c.stream, hi, (long long)nsets * b.n_hi);
-        const size_t lds = D.csum_len <= kMcCsumLds
-                               ? D.csum_len * sizeof(int) : 0;
-        const int bx = std::max(1, std::min(ceil_div(cap, 256),
-                                            8192 / std::max(nsets, 1)));
-        hipLaunchKernelGGL(batch_mcdisk_bin, dim3(bx, nsets), dim3(256), lds,
-                           c.stream, b.tmpl, b.lay, P, D.pstride, g, D.cflux,
-                           b.csum + (size_t)set0 * D.csum_len, D.csum_len,
-                           D.hoi, (uint32_t)D.seed, (uint32_t)(D.seed >> 32),
-                           bin, cap, nt);
-        GBK_HIP(hipGetLastError());
-        launch_mc_binned_tail(c, bin, sorted, toff, nt, nsets, 0, cap,
-                              b.csum + (size_t)set0 * D.csum_len, D.csum_len,
-                              g, b.qscale + set0, 1.0, hi, b.n_hi);
-    } else if (D.mcdisk) {
-        // the same 256-block x 256-thread cloud loop for every set
-        const size_t lds = D.csum_len <= kMcCsumLds
-                               ? D.csum_len * sizeof(int) : 0;
-        hipLaunchKernelGGL(batch_mcdisk, dim3(256, nsets), dim3(256), lds,
-                           c.stream, b.tmpl, b.lay, P, D.pstride, g, D.cflux,
-                           b.csum + (size_t)set0 * D.csum_len, D.csum_len,
-                           D.hoi, (uint32_t)D.seed, (uint32_t)(D.seed >> 32),
-                           b.qscale + set0,
-                           b.icube.as<unsigned long long>(), b.n_hi);
-        GBK_HIP(hipGetLastError());
-        hipLaunchKernelGGL(batch_fixed_to_hi, dim3(1024, nsets), dim3(256), 0,
-                           c.stream, b.icube.as<unsigned long long>(), hi,
-                           b.n_hi, b.qscale + set0);
-    } else if (b.thin) {
+    if (D.mcdisk || !b.thin) {
+        deposit_hi(b, P, set0, nsets, 0, hi);
+    } else {
         dim3 grid(ceil_div(g.spat_size[0], 64), ceil_div(g.spat_size[1], 4),
                   nsets);
         hipLaunchKernelGGL(batch_smdisk_thin, grid, dim3(64, 4), 0, c.stream,
                            b.tmpl, b.lay, P, D.pstride, g, hi, b.n_hi);
-    } else {
-        const int chunk = std::max(1, std::min(g.spec_size, 240));
-        dim3 grid(ceil_div(g.spat_size[0], 64), g.spat_size[1], nsets);
-        hipLaunchKernelGGL(batch_smdisk_thick, grid, dim3(256),
-                           (size_t)chunk * 64 * sizeof(float), c.stream,
-                           b.tmpl, b.lay, P, D.pstride, g, hi, b.n_hi, chunk);
+        GBK_HIP(hipGetLastError());
     }
-    GBK_HIP(hipGetLastError());
     mark(b, ci, 1);
     if (D.has_conv)
         fft_convolve_with(c, *b.plan, hi, b.spec.as<float2>(),
@@ -2019,8 +2124,9 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
                       "path must be 0 (automatic), 1 (padded-cube FFT), "
                       "2 (fused FFT) or 3 (direct separable)");
             if (D.mcdisk) {
-                GBK_CHECK(D.path <= 1, GBKHIP_ERR_UNSUPPORTED,
-                          "mcdisk batches run the padded-cube pipeline");
+                GBK_CHECK(D.path != 2, GBKHIP_ERR_UNSUPPORTED,
+                          "mcdisk batches run the direct or padded-cube "
+                          "pipeline");
                 GBK_CHECK(D.hoi != nullptr && D.csum_len > 0, GBKHIP_ERR_ARG,
                           "mcdisk batch needs hoi and csum_len");
                 // cloud i of trait t samples height trait t
@@ -2045,10 +2151,20 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
                                  ceil_div(D.size_lo[1], kSepThreads / 64) *
                                  B.sep.zblocks);
                 B.sep.on = true;
+                // thick disks and mcdisk deposit a hi-res cube first
+                B.sep.cube = D.mcdisk || !B.thin;
                 B.ka_psf.clear();
                 B.ka_lsf.clear();
-                B.lanes = 2;
-                ensure_lane_streams(B);
+                // two lanes overlap one chunk's compute-bound x pass with
+                // the memory-bound passes of the previous one; a deposited
+                // cube per lane costs memory, so those start with one
+                B.lanes = B.sep.cube ? 1 : 2;
+                if (B.lanes == 2) ensure_lane_streams(B);
+                if (!B.thin && !D.mcdisk)
+                    (void)hipFuncSetAttribute(
+                        (const void*)batch_smdisk_thick,
+                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                        96 * 1024);
                 alloc_work(B);
                 compute_zero_partials(B);
                 *out = b;
@@ -2056,8 +2172,8 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
             }
             GBK_CHECK(D.path != 3, GBKHIP_ERR_UNSUPPORTED,
                       "direct separable path unavailable for this model "
-                      "(thin disk and separable PSF x LSF within the edge "
-                      "needed)");
+                      "(a PSF x LSF that factors as lsf(z) px(x) py(y) "
+                      "within the edge needed)");
             GBK_CHECK(D.path != 2 || B.fused.on, GBKHIP_ERR_UNSUPPORTED,
                       "fused FFT path unavailable for this model");
             B.ka_psf.clear();

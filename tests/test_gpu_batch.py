@@ -165,8 +165,12 @@ def test_direct_pipeline_other_shapes(scale, size, psf, lsf):
     ev.close()
 
 
-def test_thick_disk_uses_padded_path():
-    """A thick disk runs the padded-cube path; chi^2 equals the plain path."""
+@pytest.mark.parametrize('path', ['auto', 'direct', 'padded'])
+def test_thick_disk_batch_matches_plain_path(path):
+    """A thick disk runs the direct pipeline from its deposited hi-res cube
+    (automatic choice) or the padded-cube one; chi^2 equals the plain path,
+    and two lanes of chunks give the chi^2 of one (within 1e-7: the thick
+    deposit adds into LDS with float atomics, so its sums vary with order)."""
     from gbkfit_amd.batch import BatchEvaluator
     from gbkfit_amd.objective import Data, Objective
     drv = H.driver()
@@ -176,8 +180,9 @@ def test_thick_disk_uses_padded_path():
     rng = np.random.default_rng(3)
     mock = (ref + rng.normal(0, 0.01, ref.shape)).astype(np.float32)
     ev = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock,
-                        error=0.01, max_batch=2)
-    assert ev.info()['pipeline'] == 'padded'
+                        error=0.01, max_batch=2, path=path)
+    want = 'direct' if path == 'auto' else path
+    assert ev.info()['pipeline'] == want
     sets = [P.gparams(dict(p, incl=i, posa=a)) for i, a in
             ((40, 10), (60, 200), (75, 300))]
     chi2 = ev.evaluate(sets, use_graph=False)
@@ -187,14 +192,21 @@ def test_thick_disk_uses_padded_path():
     for i, s in enumerate(sets):
         plain = -2 * obj.log_likelihood(s)[0]
         assert abs(plain - chi2[i]) <= 1e-5 * chi2[i], (i, plain, chi2[i])
+    if want == 'direct':
+        ev.set_tiles(2, 2)
+        two = ev.evaluate(sets, use_graph=True)
+        assert np.allclose(two, chi2, rtol=1e-7, atol=0), (two, chi2)
     ev.close()
 
 
-def test_mcdisk_batch_matches_plain_path():
-    """Batched mcdisk (padded-cube pipeline, per-set cloud budgets): model
+@pytest.mark.parametrize('path', ['auto', 'direct', 'padded'])
+def test_mcdisk_batch_matches_plain_path(path):
+    """Batched mcdisk (direct pipeline from the deposited hi-res cube, the
+    automatic choice, or padded-cube FFT; per-set cloud budgets): model
     cubes and chi^2 of every set against the plain one-model path (which the
     kernel tests pin to the oracle's Philox stream), a set alone equal to
-    the same set in the batch, and graph replay equal to direct launches."""
+    the same set in the batch, graph replay equal to direct launches and,
+    on the direct pipeline, two lanes of chunks equal to one."""
     from gbkfit_amd.batch import BatchEvaluator
     drv = H.driver()
     p = dict(P.c1_params(), size=(32, 32, 24), size_z=24)
@@ -203,8 +215,10 @@ def test_mcdisk_batch_matches_plain_path():
     rng = np.random.default_rng(5)
     mock = (ref + rng.normal(0, 0.01, ref.shape)).astype(np.float32)
     ev = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock,
-                        error=0.01, max_batch=2, template_params=base)
-    assert ev.info()['pipeline'] == 'padded'
+                        error=0.01, max_batch=2, template_params=base,
+                        path=path)
+    want = 'direct' if path == 'auto' else path
+    assert ev.info()['pipeline'] == want
     sets = [dict(base, incl=i, posa=a, bpt_s=s) for i, a, s in
             ((45, 30, 8.0), (60, 200, 6.0), (75, 300, 10.0))]
     chi2, cubes = ev.evaluate(sets, want_cubes=True, use_graph=False)
@@ -217,6 +231,11 @@ def test_mcdisk_batch_matches_plain_path():
     assert alone[0] == chi2[1]
     g = ev.evaluate(sets, use_graph=True)
     assert np.array_equal(g, chi2)
+    if want == 'direct':
+        ev.set_tiles(2, 2)
+        two = ev.evaluate(sets, use_graph=True)
+        assert np.array_equal(two, chi2)
+        print('mcdisk direct tiles', ev.tile_stats())
     ev.close()
 
 

@@ -1,6 +1,7 @@
 """Ragged groups (gbkhip_group_*) and the autotuner: galaxies of different
 sizes and models (thin smdisk on the direct pipeline, thick smdisk and
-mcdisk on the padded-cube one) evaluated in one launch.  Each member's chi^2
+mcdisk on the direct or padded-cube one, as timed) evaluated in one
+launch.  Each member's chi^2
 in the group equals its own evaluation (graph and direct launches): bit for
 bit, except for the thick smdisk, whose four waves per line of sight add
 into an LDS spectrum with float atomics, so its chi^2 varies run to run at
@@ -70,8 +71,9 @@ def test_autotuner_choices(group):
         assert ev.info()['pipeline'] == path
         assert rep['trials'] and min(t['s_per_set'] for t in
                                      rep['trials']) == rep['s_per_set']
+        tried = {t['path'] for t in rep['trials']}
         if kind == 'mc' or not p['thin']:
-            assert path == 'padded'
+            assert tried == {'direct', 'padded'}, tried
         print(kind, p['size'], p['scale'], rep['choice'],
               '%.1f us/set' % (1e6 * rep['s_per_set']))
 
