@@ -1230,12 +1230,11 @@ static void build_layout(Batch& b)
 // evaluations and are re-allocated by the next one; the resident state
 // (layout, kernel spectra, plans, twiddles) stays.
 // Binned mcdisk deposits of one lane (max_batch sets): records, ranks,
-// sorted records, tile counts and offsets, 256-byte aligned.
+// sorted records, tile counts, offsets and split-tile accumulators.
 static size_t mc_lane_bytes(const Batch& B)
 {
-    const size_t set = (size_t)B.mc_cap * (2 * sizeof(McDep) + sizeof(int2)) +
-                       (2 * (size_t)B.mc_ntiles + 1) * sizeof(int);
-    return ((size_t)B.desc.max_batch * set + 255) / 256 * 256;
+    return mc_bufs(nullptr, B.desc.max_batch, B.mc_cap, B.mc_ntiles,
+                   B.mc_tw, B.desc.grid.spec_size, nullptr);
 }
 
 static size_t work_bytes(const Batch& B)
@@ -1787,13 +1786,12 @@ static void deposit_hi(Batch& b, const float* P, int set0, int nsets,
         const int nt = b.mc_ntiles;
         char* base = static_cast<char*>(b.mcbin.ptr) +
                      (size_t)lane * mc_lane_bytes(b);
-        const size_t mb = (size_t)D.max_batch;
+        McBufs bb;
+        mc_bufs(base, D.max_batch, cap, nt, b.mc_tw, g.spec_size, &bb);
         McBin bin;
-        bin.rec = reinterpret_cast<McDep*>(base);
-        McDep* sorted = bin.rec + mb * cap;
-        bin.tr = reinterpret_cast<int2*>(sorted + mb * cap);
-        bin.tcount = reinterpret_cast<int*>(bin.tr + mb * cap);
-        int* toff = bin.tcount + mb * nt;
+        bin.rec = bb.rec;
+        bin.tr = bb.tr;
+        bin.tcount = bb.tcount;
         bin.tw = bin.th = b.mc_tw;
         bin.ntx = b.mc_ntx;
         hipLaunchKernelGGL(batch_zero_i32, dim3(ceil_div(nsets * nt, 256)),
@@ -1811,7 +1809,7 @@ static void deposit_hi(Batch& b, const float* P, int set0, int nsets,
                            D.hoi, (uint32_t)D.seed, (uint32_t)(D.seed >> 32),
                            bin, cap, nt);
         GBK_HIP(hipGetLastError());
-        launch_mc_binned_tail(c, bin, sorted, toff, nt, nsets, 0, cap,
+        launch_mc_binned_tail(c, bin, bb, nsets, 0,
                               b.csum + (size_t)set0 * D.csum_len, D.csum_len,
                               g, b.qscale + set0, 1.0, hi, b.n_hi);
     } else if (D.mcdisk) {

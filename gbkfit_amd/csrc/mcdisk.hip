@@ -84,35 +84,68 @@ __global__ __launch_bounds__(256) void mcdisk_bin_kernel(
     }
 }
 
-// off[t] = sum of count[0..t), off[n] = total: one block of 1024 threads
-// per set (blockIdx.x).
-__global__ __launch_bounds__(1024) void mc_scan_tiles(const int* count,
-                                                      int* off, int n)
+// Per set (blockIdx.x; one block of 1024 threads): off[t] = records of
+// tiles [0, t) (off[n] = all), soff[t] = slices of tiles [0, t) with
+// ceil(count / kMcSlice) slices per tile, and the tiles of more than one
+// slice numbered in order: sidx[t] (-1 for the others), stile[k], nsplit.
+__global__ __launch_bounds__(1024) void mc_scan_tiles(
+    const int* __restrict__ count, int* __restrict__ off,
+    int* __restrict__ soff, int* __restrict__ sidx, int* __restrict__ stile,
+    int* __restrict__ nsplit, int n, int maxsplit)
 {
-    __shared__ int s[1024];
-    count += (size_t)blockIdx.x * n;
-    off += (size_t)blockIdx.x * (n + 1);
+    __shared__ int s_r[1024], s_s[1024], s_p[1024];
+    const size_t set = blockIdx.x;
+    count += set * n;
+    off += set * (n + 1);
+    soff += set * (n + 1);
+    sidx += set * n;
+    stile += set * maxsplit;
     const int t = threadIdx.x, per = (n + 1023) / 1024;
     const int lo = min(t * per, n), hi = min(lo + per, n);
-    int sum = 0;
-    for (int i = lo; i < hi; ++i) sum += count[i];
-    s[t] = sum;
-    __syncthreads();
-    if (t == 0) {                      // exclusive scan of 1024 partials
-        int run = 0;
-        for (int i = 0; i < 1024; ++i) {
-            const int v = s[i];
-            s[i] = run;
-            run += v;
-        }
-    }
-    __syncthreads();
-    int run = s[t];
+    int r = 0, sl = 0, sp = 0;
     for (int i = lo; i < hi; ++i) {
-        off[i] = run;
-        run += count[i];
+        const int c = count[i], ns = (c + kMcSlice - 1) / kMcSlice;
+        r += c;
+        sl += ns;
+        sp += ns > 1;
     }
-    if (t == 1023) off[n] = run;
+    s_r[t] = r;
+    s_s[t] = sl;
+    s_p[t] = sp;
+    __syncthreads();
+    if (t == 0) {                      // exclusive scans of 1024 partials
+        int a = 0, b = 0, c = 0;
+        for (int i = 0; i < 1024; ++i) {
+            const int x = s_r[i], y = s_s[i], z = s_p[i];
+            s_r[i] = a;
+            s_s[i] = b;
+            s_p[i] = c;
+            a += x;
+            b += y;
+            c += z;
+        }
+        off[n] = a;
+        soff[n] = b;
+        nsplit[set] = c;
+    }
+    __syncthreads();
+    r = s_r[t];
+    sl = s_s[t];
+    sp = s_p[t];
+    for (int i = lo; i < hi; ++i) {
+        const int c = count[i], ns = (c + kMcSlice - 1) / kMcSlice;
+        off[i] = r;
+        soff[i] = sl;
+        if (ns > 1) {
+            sidx[i] = sp;
+            stile[sp] = i;
+            ++sp;
+        } else {
+            sidx[i] = -1;
+        }
+        r += c;
+        sl += ns;
+    }
 }
 
 // Records to tile order; blockIdx.y = set.  A set's cloud count is its last
@@ -138,23 +171,50 @@ __global__ void mc_scatter(const McDep* __restrict__ rec,
     }
 }
 
-// One block per (tile, set): the deposits of the tile's clouds summed in
-// LDS with int64 atomics (the same quantised values the global path adds),
-// then cube += sum / 2^S as fixed_to_float does.
+// Split tiles' int64 accumulators start at zero (blockIdx.x = split index,
+// blockIdx.y = set).
+__global__ __launch_bounds__(256) void mc_zero_split(
+    unsigned long long* __restrict__ gacc, const int* __restrict__ nsplit,
+    int maxsplit, int tile_elems)
+{
+    const int k = blockIdx.x, set = blockIdx.y;
+    if (k >= nsplit[set]) return;
+    unsigned long long* a = gacc + ((size_t)set * maxsplit + k) * tile_elems;
+    for (int i = threadIdx.x; i < tile_elems; i += blockDim.x) a[i] = 0ull;
+}
+
+// One block per (slice, set): the deposits of a slice of a tile's records
+// summed in LDS with int64 atomics (the same quantised values the global
+// path adds).  A tile of one slice then does cube += sum / 2^S as
+// fixed_to_float does; a split tile adds its LDS sums into the tile's int64
+// accumulator, converted by mc_split_finish.
 __global__ __launch_bounds__(256) void mc_tile_deposit(
     const McDep* __restrict__ sorted, const int* __restrict__ off,
-    gbkhip_grid g, int tw, int th, int ntx, const double* __restrict__ qs,
+    const int* __restrict__ soff, const int* __restrict__ sidx,
+    unsigned long long* __restrict__ gacc, int maxsplit, gbkhip_grid g,
+    int tw, int th, int ntx, int ntiles, const double* __restrict__ qs,
     double qscale, float* __restrict__ cube, long long cap,
     long long cstride)
 {
     extern __shared__ unsigned long long acc[];      // [nz][tw * th]
-    const int tile = blockIdx.x, set = blockIdx.y, ntiles = gridDim.x;
+    const int slice = blockIdx.x, set = blockIdx.y;
     off += (size_t)set * (ntiles + 1);
+    soff += (size_t)set * (ntiles + 1);
+    if (slice >= soff[ntiles]) return;
+    // the tile t with soff[t] <= slice < soff[t + 1] (the last such t;
+    // empty tiles have no slices)
+    int a = 0, b = ntiles - 1;
+    while (a < b) {
+        const int m = (a + b + 1) >> 1;
+        if (soff[m] <= slice) a = m; else b = m - 1;
+    }
+    const int tile = a;
+    const int ns = soff[tile + 1] - soff[tile];
+    const int lo = off[tile] + (slice - soff[tile]) * kMcSlice;
+    const int hi = min(lo + kMcSlice, off[tile + 1]);
     sorted += set * cap;
     cube += set * cstride;
     if (qs) qscale = qs[set];
-    const int lo = off[tile], hi = off[tile + 1];
-    if (lo == hi) return;
     const int nx = g.spat_size[0], ny = g.spat_size[1], nz = g.spec_size;
     const int tc = tw * th;
     const size_t plane = (size_t)nx * ny;
@@ -175,6 +235,16 @@ __global__ __launch_bounds__(256) void mc_tile_deposit(
         }
     }
     __syncthreads();
+    if (ns > 1) {
+        unsigned long long* ga =
+            gacc + ((size_t)set * maxsplit + sidx[(size_t)set * ntiles +
+                                                   tile]) * (nz * tc);
+        for (int i = threadIdx.x; i < nz * tc; i += blockDim.x) {
+            const unsigned long long q = acc[i];
+            if (q) atomicAdd(&ga[i], q);
+        }
+        return;
+    }
     const double inv = 1.0 / qscale;
     for (int i = threadIdx.x; i < nz * tc; i += blockDim.x) {
         const long long q = (long long)acc[i];
@@ -186,6 +256,38 @@ __global__ __launch_bounds__(256) void mc_tile_deposit(
     }
 }
 
+// cube += accumulator / 2^S for the split tiles (blockIdx.x = split index,
+// blockIdx.y = set).
+__global__ __launch_bounds__(256) void mc_split_finish(
+    const unsigned long long* __restrict__ gacc,
+    const int* __restrict__ stile, const int* __restrict__ nsplit,
+    int maxsplit, gbkhip_grid g, int tw, int th, int ntx,
+    const double* __restrict__ qs, double qscale, float* __restrict__ cube,
+    long long cstride)
+{
+    const int k = blockIdx.x, set = blockIdx.y;
+    if (k >= nsplit[set]) return;
+    const int tile = stile[(size_t)set * maxsplit + k];
+    const int nx = g.spat_size[0], ny = g.spat_size[1], nz = g.spec_size;
+    const int tc = tw * th;
+    const size_t plane = (size_t)nx * ny;
+    const unsigned long long* ga =
+        gacc + ((size_t)set * maxsplit + k) * (nz * tc);
+    cube += set * cstride;
+    if (qs) qscale = qs[set];
+    const double inv = 1.0 / qscale;
+    const int tx0 = (tile % ntx) * tw, ty0 = (tile / ntx) * th;
+    for (int i = threadIdx.x; i < nz * tc; i += blockDim.x) {
+        const long long q = (long long)ga[i];
+        if (!q) continue;
+        const int kk = i / tc, lc = i - kk * tc;
+        const int x = tx0 + lc % tw, y = ty0 + lc / tw;
+        if (x < nx && y < ny)
+            cube[kk * plane + (size_t)y * nx + x] +=
+                (float)((double)q * inv);
+    }
+}
+
 int mc_tile_width(int nz)
 {
     int tw = 8;
@@ -193,22 +295,74 @@ int mc_tile_width(int nz)
     return (size_t)tw * tw * nz * 8 <= 64 * 1024 ? tw : 0;
 }
 
-void launch_mc_binned_tail(Context& c, const McBin& bin, McDep* sorted,
-                           int* toff, int ntiles, int nsets, long long n,
-                           long long cap, const int* csum, int csum_len,
-                           const gbkhip_grid& g, const double* qs,
-                           double qscale, float* cube, long long cstride)
+size_t mc_bufs(char* base, int nsets, long long cap, int ntiles, int tw,
+               int nz, McBufs* out)
 {
+    McBufs b;
+    b.cap = cap;
+    b.ntiles = ntiles;
+    // a split tile holds more than kMcSlice records; slices beyond one per
+    // tile number at most cap / kMcSlice
+    const long long extra = (cap + kMcSlice - 1) / kMcSlice;
+    b.maxsplit = (int)std::min<long long>(ntiles, extra);
+    b.maxslices = (int)std::min<long long>((long long)ntiles + extra,
+                                           (1LL << 31) - 1);
+    b.tile_elems = tw * tw * nz;
+    const size_t ns = (size_t)nsets;
+    size_t at = 0;
+    auto take = [&](size_t bytes, size_t align) {
+        at = (at + align - 1) / align * align;
+        char* p = base ? base + at : nullptr;
+        at += bytes;
+        return p;
+    };
+    b.rec = reinterpret_cast<McDep*>(take(ns * cap * sizeof(McDep), 256));
+    b.sorted = reinterpret_cast<McDep*>(take(ns * cap * sizeof(McDep), 256));
+    b.tr = reinterpret_cast<int2*>(take(ns * cap * sizeof(int2), 256));
+    b.tcount = reinterpret_cast<int*>(take(ns * ntiles * sizeof(int), 256));
+    b.toff = reinterpret_cast<int*>(take(ns * (ntiles + 1) * sizeof(int),
+                                         256));
+    b.soff = reinterpret_cast<int*>(take(ns * (ntiles + 1) * sizeof(int),
+                                         256));
+    b.sidx = reinterpret_cast<int*>(take(ns * ntiles * sizeof(int), 256));
+    b.stile = reinterpret_cast<int*>(take(ns * b.maxsplit * sizeof(int),
+                                          256));
+    b.nsplit = reinterpret_cast<int*>(take(ns * sizeof(int), 256));
+    b.gacc = reinterpret_cast<unsigned long long*>(
+        take(ns * b.maxsplit * (size_t)b.tile_elems * 8, 256));
+    at = (at + 255) / 256 * 256;
+    if (out) *out = b;
+    return at;
+}
+
+void launch_mc_binned_tail(Context& c, const McBin& bin, const McBufs& bb,
+                           int nsets, long long n, const int* csum,
+                           int csum_len, const gbkhip_grid& g,
+                           const double* qs, double qscale, float* cube,
+                           long long cstride)
+{
+    const int nt = bb.ntiles;
     hipLaunchKernelGGL(mc_scan_tiles, dim3(nsets), dim3(1024), 0, c.stream,
-                       bin.tcount, toff, ntiles);
-    const int sb = std::max(1, std::min(ceil_div(cap, 256), 8192 / nsets));
+                       bin.tcount, bb.toff, bb.soff, bb.sidx, bb.stile,
+                       bb.nsplit, nt, bb.maxsplit);
+    const int sb = std::max(1, std::min(ceil_div(bb.cap, 256), 8192 / nsets));
     hipLaunchKernelGGL(mc_scatter, dim3(sb, nsets), dim3(256), 0, c.stream,
-                       bin.rec, bin.tr, toff, sorted, n, cap, csum, csum_len,
-                       ntiles);
-    hipLaunchKernelGGL(mc_tile_deposit, dim3(ntiles, nsets), dim3(256),
-                       (size_t)bin.tw * bin.th * g.spec_size * 8, c.stream,
-                       sorted, toff, g, bin.tw, bin.th, bin.ntx, qs, qscale,
-                       cube, cap, cstride);
+                       bin.rec, bin.tr, bb.toff, bb.sorted, n, bb.cap, csum,
+                       csum_len, nt);
+    if (bb.maxsplit > 0)
+        hipLaunchKernelGGL(mc_zero_split, dim3(bb.maxsplit, nsets), dim3(256),
+                           0, c.stream, bb.gacc, bb.nsplit, bb.maxsplit,
+                           bb.tile_elems);
+    hipLaunchKernelGGL(mc_tile_deposit, dim3(bb.maxslices, nsets), dim3(256),
+                       (size_t)bb.tile_elems * 8, c.stream, bb.sorted,
+                       bb.toff, bb.soff, bb.sidx, bb.gacc, bb.maxsplit, g,
+                       bin.tw, bin.th, bin.ntx, nt, qs, qscale, cube, bb.cap,
+                       cstride);
+    if (bb.maxsplit > 0)
+        hipLaunchKernelGGL(mc_split_finish, dim3(bb.maxsplit, nsets),
+                           dim3(256), 0, c.stream, bb.gacc, bb.stile,
+                           bb.nsplit, bb.maxsplit, g, bin.tw, bin.th, bin.ntx,
+                           qs, qscale, cube, cstride);
     GBK_HIP(hipGetLastError());
 }
 
@@ -292,17 +446,16 @@ void launch_mcdisk(Context& c, float cflux, int nclouds, const int* csum,
         const int nx = g.spat_size[0], ny = g.spat_size[1];
         const int ntx = ceil_div(nx, tw), nty = ceil_div(ny, tw);
         const int ntiles = ntx * nty;
-        const size_t nc = (size_t)nclouds;
-        const size_t need = nc * (2 * sizeof(McDep) + sizeof(int2)) +
-                            (2 * (size_t)ntiles + 1) * sizeof(int);
+        const long long nc = nclouds;
+        const size_t need = mc_bufs(nullptr, 1, nc, ntiles, tw, nz, nullptr);
         if (c.mc_scratch.bytes < need) c.mc_scratch.alloc(need);
-        char* base = static_cast<char*>(c.mc_scratch.ptr);
+        McBufs bb;
+        mc_bufs(static_cast<char*>(c.mc_scratch.ptr), 1, nc, ntiles, tw, nz,
+                &bb);
         McBin bin;
-        bin.rec = reinterpret_cast<McDep*>(base);
-        McDep* sorted = bin.rec + nc;
-        bin.tr = reinterpret_cast<int2*>(sorted + nc);
-        bin.tcount = reinterpret_cast<int*>(bin.tr + nc);
-        int* toff = bin.tcount + ntiles;
+        bin.rec = bb.rec;
+        bin.tr = bb.tr;
+        bin.tcount = bb.tcount;
         bin.tw = bin.th = tw;
         bin.ntx = ntx;
         hipLaunchKernelGGL(mc_zero_i32, dim3(ceil_div(ntiles, 256)),
@@ -315,8 +468,7 @@ void launch_mcdisk(Context& c, float cflux, int nclouds, const int* csum,
             std::max(1, std::min(ceil_div(nclouds, 256), kMcBinBlocks));
         hipLaunchKernelGGL(mcdisk_bin_kernel, dim3(bin_blocks), dim3(256),
                            lds, c.stream, mp, d, g, bin, ntiles);
-        launch_mc_binned_tail(c, bin, sorted, toff, ntiles, 1, (long long)nc,
-                              (long long)nc, nullptr, 0, g, nullptr,
+        launch_mc_binned_tail(c, bin, bb, 1, nc, nullptr, 0, g, nullptr,
                               mp.qscale, o.scube, 0);
         GBK_HIP(hipGetLastError());
         return;

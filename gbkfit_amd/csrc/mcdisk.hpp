@@ -172,14 +172,42 @@ struct Context;
 // Tile width (spaxels) of binned deposits for nz channels: the largest
 // power of two whose tile spectrum (tw^2 nz int64) fits 64 KB of LDS, or 0.
 int mc_tile_width(int nz);
+
+// Records per tile-deposit block: a tile with more records is split into
+// slices of at most kMcSlice, one block each, whose LDS sums meet in an int64
+// accumulator of the tile (integer sums: the result does not depend on the
+// split), converted afterwards.  The central tiles of a disk hold most
+// clouds; one block per tile serialised on them.
+constexpr int kMcSlice = 8192;
+
+// Binned deposit buffers of nsets sets, set-major within each array.
+struct McBufs {
+    McDep* rec = nullptr;        // cap per set, by cloud id
+    McDep* sorted = nullptr;     // cap per set, by tile
+    int2* tr = nullptr;          // cap per set: (tile or -1, rank in tile)
+    int* tcount = nullptr;       // ntiles per set
+    int* toff = nullptr;         // ntiles + 1 per set: record offsets
+    int* soff = nullptr;         // ntiles + 1 per set: slice offsets
+    int* sidx = nullptr;         // ntiles per set: split index or -1
+    int* stile = nullptr;        // maxsplit per set: tile of a split index
+    int* nsplit = nullptr;       // 1 per set
+    unsigned long long* gacc = nullptr;   // maxsplit x tile spectrum per set
+    long long cap = 0;
+    int ntiles = 0, maxsplit = 0, maxslices = 0;
+    int tile_elems = 0;          // tw^2 nz
+};
+// Carves the buffers for (nsets, cap, ntiles, tw, nz) out of base (null:
+// sizes only) and returns the bytes they take (256-byte aligned parts).
+size_t mc_bufs(char* base, int nsets, long long cap, int ntiles, int tw,
+               int nz, McBufs* out);
 // Tile offsets, tile-ordered records and per-tile LDS deposits for nsets
-// sets whose records, ranks and tile counts were written (per set strides:
-// cap records, ntiles counts, ntiles + 1 offsets); cube += the deposits.
-void launch_mc_binned_tail(Context& c, const McBin& bin, McDep* sorted,
-                           int* toff, int ntiles, int nsets, long long n,
-                           long long cap, const int* csum, int csum_len,
-                           const gbkhip_grid& g, const double* qs,
-                           double qscale, float* cube, long long cstride);
+// sets whose records, ranks and tile counts were written; cube += the
+// deposits.  A set's cloud count is its last cumulative count (csum) or n.
+void launch_mc_binned_tail(Context& c, const McBin& bin, const McBufs& bb,
+                           int nsets, long long n, const int* csum,
+                           int csum_len, const gbkhip_grid& g,
+                           const double* qs, double qscale, float* cube,
+                           long long cstride);
 
 struct McParams {
     float cflux;

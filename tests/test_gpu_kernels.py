@@ -241,6 +241,26 @@ def test_mcdisk_matches_oracle(drv, case):
         1e-5 * abs(ref['scube'].sum())
 
 
+@pytest.mark.parametrize('nclouds', [60000, 500000])
+def test_mcdisk_binned_split_tiles_equal_global_path(drv, nclouds):
+    """With the spectral cube the only output, clouds are binned by 8 x 8
+    spaxel tile and summed per tile in LDS; a tile of more than 8192 records
+    (the central tiles here: 16 tiles share the clouds) is split over
+    several blocks whose int64 sums meet in a tile accumulator.  With
+    another output requested, every deposit is a global int64 atomic.  Both
+    are the same integer sums: the cubes are bit-identical."""
+    case = dict(cases.MCDISK_CASES[0], nclouds=nclouds)
+    spec = cases.build_spec(case).rounded_f32()
+    grid = oracle.grid_f32(cases.build_grid(case))
+    cflux, csum, hoi = cases.cloud_budget(spec, case['traits'],
+                                          case['nclouds'])
+    binned = H.mcdisk(drv, spec, grid, cflux, csum, hoi, seed=7)['scube']
+    glob = H.mcdisk(drv, spec, grid, cflux, csum, hoi, seed=7,
+                    want=('scube', 'image'))['scube']
+    assert binned.any()
+    assert np.array_equal(binned, glob)
+
+
 def test_mcdisk_reruns_bit_identical(drv):
     """int64 fixed-point accumulation: the cube does not depend on the order
     in which clouds land, so reruns are bit-identical; another seed is not."""
