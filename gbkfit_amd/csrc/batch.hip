@@ -793,18 +793,28 @@ __global__ __launch_bounds__(256) void sep_xpass(
         M4[i] = make_float4(0, 0, 0, 0);
     for (int i = (n4 << 2) + threadIdx.x; i < zc * mp; i += T) M[i] = 0.0f;
     __syncthreads();
-    // lines in form 1 (batch_lines_thin): amplitude and window precomputed
+    // lines in form 1 (batch_lines_thin): amplitude and window precomputed;
+    // a thread's UX line loads are issued together
+    constexpr int UX = 2;
     const float4* lrow = lines + set * lstride + (size_t)y * lpitch;
-    for (int x = threadIdx.x; x < need_x; x += T) {
-        const float4 V = lrow[x];
-        if (V.x != 0.0f) {
-            const unsigned kw = __float_as_uint(V.w);
+    for (int x0 = threadIdx.x; x0 < need_x; x0 += T * UX) {
+        float4 V[UX];
+#pragma unroll
+        for (int u = 0; u < UX; ++u) {
+            const int x = x0 + u * T;
+            V[u] = x < need_x ? lrow[x] : make_float4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < UX; ++u) {
+            if (V[u].x == 0.0f) continue;
+            const int x = x0 + u * T;
+            const unsigned kw = __float_as_uint(V[u].w);
             const int k0 = max((int)(kw & 0xffffu), z0);
             const int k1 = min((int)(kw >> 16), z0 + zn - 1);
             LineEval e;
-            e.amp = V.x;
-            e.v = V.y;
-            e.inv2s2 = V.z;
+            e.amp = V[u].x;
+            e.v = V[u].y;
+            e.inv2s2 = V[u].z;
             e.zero = g.spec_zero;
             e.step = g.spec_step;
             for (int k = k0; k <= k1; ++k) M[(k - z0) * mp + x] = e.at(k);
