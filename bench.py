@@ -28,7 +28,7 @@ The plain path is the drop-in call sequence of the reference
 FFT convolution, downscale, residual, residual sum, scalar download).
 
 A serving leg (skip with --no-serve) replays a Poisson stream of
-single-set requests (default 1000/s for 30 s) over 64 resident galaxies with
+single-set requests (default 1000/s for 60 s) over 64 resident galaxies with
 cube sizes log-uniform from 64x64x32 to 256x256x128, half smdisk, half mcdisk
 with 2^20 clouds; each request returns chi^2 and moment maps 0/1/2 through
 gbkfit_amd.serve.Server (continuous batching over 4 streams, HBM-budget
@@ -402,8 +402,9 @@ def cpu_baseline(seconds, thick=False):
         return None
     R = refcore.ref(np.float32)
     nthreads = refcore.num_threads()
-    arrays = workload(64, seed=99)
-    sets = param_dicts(arrays, 64)
+    # enough sets that the time limit, not the sample, ends the loop
+    arrays = workload(1024, seed=99)
+    sets = param_dicts(arrays, 1024)
     base = P.c3_params(n=1)[0]
     base['thin'] = not thick
     geom = P.oracle_geom(base)
@@ -530,7 +531,7 @@ def main():
     ap.add_argument('--no-graph', action='store_true')
     ap.add_argument('--thick', action='store_true')
     ap.add_argument('--serve-galaxies', type=int, default=64)
-    ap.add_argument('--serve-seconds', type=float, default=30.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument('--serve-rate', type=float, default=1000.0)
     ap.add_argument('--no-serve', action='store_true')
     ap.add_argument('--ragged-galaxies', type=int, default=32)
