@@ -708,6 +708,9 @@ constexpr int kSepJZ = 16;          // outputs per thread, S3
 constexpr int kSepXW = 32;          // X columns per S2 block
 constexpr int kSepYPitch = kSepXW + 1;
 constexpr int kSepMaxRows = (48 * 1024) / (kSepYPitch * 4);   // S2 LDS rows
+// lines per thread an S1 row block keeps in registers (wider rows read the
+// rest again per chunk)
+constexpr int kSepXLines = 2;
 
 // outputs per thread of S1 and S2 (fewer for wide taps: registers)
 __host__ __device__ constexpr int sep_jx(int taps) { return taps <= 64 ? 8 : 4; }
@@ -768,9 +771,13 @@ __device__ __forceinline__ void sep_xfilter(const float* M, const float* s_w,
     }
 }
 
-// S1: block = (row y, channels [z0, z0 + zn), set).  The hi-res values of
-// the row are deposited into an LDS tile [zc][mp] (zero outside each
-// spaxel's rint window, gmodels.hpp:91-116), then filtered along x.
+// S1: block = (row y, set); the row's channel chunks [z0, z0 + zn) in
+// turn.  The hi-res values of the row are deposited into an LDS tile
+// [zc][mp] (zero outside each spaxel's rint window, gmodels.hpp:91-116),
+// then filtered along x.  A chunk no line of the row reaches filters to
+// zeros: it is skipped (its rows are not written; the y pass reads the same
+// mask and uses zeros), and a row no line reaches returns at once.  The
+// row's first T UX lines are loaded once for all its chunks.
 template <int SX, int TX>
 __global__ __launch_bounds__(256) void sep_xpass(
     const float4* __restrict__ lines, long long lstride, int lpitch,
@@ -781,48 +788,53 @@ __global__ __launch_bounds__(256) void sep_xpass(
     extern __shared__ float4 M4[];        // [zc][mp] floats
     float* M = reinterpret_cast<float*>(M4);
     __shared__ float s_w[TX];
-    constexpr int T = kSepThreads;
-    const int y = blockIdx.x, z0 = blockIdx.y * zc, set = blockIdx.z;
-    // a tile no line reaches filters to zeros: skipped (its rows are not
-    // written; the y pass reads the same mask and uses zeros)
-    if (!((rowmask[(size_t)set * ny1 + y] >> blockIdx.y) & 1ull)) return;
-    const int zn = min(zc, need_z - z0);
+    constexpr int T = kSepThreads, UX = kSepXLines;
+    const int y = blockIdx.x, set = blockIdx.z;
+    unsigned long long live = rowmask[(size_t)set * ny1 + y];
+    if (!live) return;
     for (int i = threadIdx.x; i < TX; i += T) s_w[i] = wx[i];
-    const int n4 = (zc * mp) >> 2;
-    for (int i = threadIdx.x; i < n4; i += T)
-        M4[i] = make_float4(0, 0, 0, 0);
-    for (int i = (n4 << 2) + threadIdx.x; i < zc * mp; i += T) M[i] = 0.0f;
-    __syncthreads();
-    // lines in form 1 (batch_lines_thin): amplitude and window precomputed;
-    // a thread's UX line loads are issued together
-    constexpr int UX = 2;
+    // lines in form 1 (batch_lines_thin): amplitude and window precomputed
     const float4* lrow = lines + set * lstride + (size_t)y * lpitch;
-    for (int x0 = threadIdx.x; x0 < need_x; x0 += T * UX) {
-        float4 V[UX];
+    float4 V[UX];
 #pragma unroll
-        for (int u = 0; u < UX; ++u) {
-            const int x = x0 + u * T;
-            V[u] = x < need_x ? lrow[x] : make_float4(0, 0, 0, 0);
-        }
-#pragma unroll
-        for (int u = 0; u < UX; ++u) {
-            if (V[u].x == 0.0f) continue;
-            const int x = x0 + u * T;
-            const unsigned kw = __float_as_uint(V[u].w);
+    for (int u = 0; u < UX; ++u) {
+        const int x = threadIdx.x + u * T;
+        V[u] = x < need_x ? lrow[x] : make_float4(0, 0, 0, 0);
+    }
+    const int n4 = (zc * mp) >> 2;
+    float* out = P1 + set * p1stride + (size_t)y * nxl;
+    while (live) {                            // uniform over the block
+        const int ch = __ffsll((long long)live) - 1;
+        live &= live - 1ull;
+        const int z0 = ch * zc, zn = min(zc, need_z - z0);
+        for (int i = threadIdx.x; i < n4; i += T)
+            M4[i] = make_float4(0, 0, 0, 0);
+        for (int i = (n4 << 2) + threadIdx.x; i < zc * mp; i += T)
+            M[i] = 0.0f;
+        __syncthreads();
+        auto deposit = [&](const float4& L, int x) {
+            const unsigned kw = __float_as_uint(L.w);
             const int k0 = max((int)(kw & 0xffffu), z0);
             const int k1 = min((int)(kw >> 16), z0 + zn - 1);
             LineEval e;
-            e.amp = V[u].x;
-            e.v = V[u].y;
-            e.inv2s2 = V[u].z;
+            e.amp = L.x;
+            e.v = L.y;
+            e.inv2s2 = L.z;
             e.zero = g.spec_zero;
             e.step = g.spec_step;
             for (int k = k0; k <= k1; ++k) M[(k - z0) * mp + x] = e.at(k);
+        };
+#pragma unroll
+        for (int u = 0; u < UX; ++u)
+            if (V[u].x != 0.0f) deposit(V[u], threadIdx.x + u * T);
+        for (int x = threadIdx.x + UX * T; x < need_x; x += T) {
+            const float4 L = lrow[x];
+            if (L.x != 0.0f) deposit(L, x);
         }
+        __syncthreads();
+        sep_xfilter<SX, TX>(M, s_w, mp, zn, z0, nxl, ny1, out);
+        __syncthreads();            // the tile is free for the next chunk
     }
-    __syncthreads();
-    sep_xfilter<SX, TX>(M, s_w, mp, zn, z0, nxl, ny1,
-                        P1 + set * p1stride + (size_t)y * nxl);
 }
 
 // S1 from a deposited hi-res cube (thick disks, mcdisk): block = (row y,
@@ -1895,7 +1907,7 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
                            ny, lines, lstride, 1, masks, P.zc, nz);
         GBK_HIP(hipGetLastError());
         mark(b, ci, 1);
-        dim3 grid(ny, ceil_div(nz, P.zc), nsets);
+        dim3 grid(ny, 1, nsets);
         const size_t lds = (size_t)P.zc * P.mp * sizeof(float);
         GBK_SEP_DISPATCH(sep_xpass, D.scale[0], P.tx, grid, dim3(kSepThreads),
                          lds, c.stream, lines, lstride, nx, D.grid,

@@ -133,11 +133,13 @@ def test_pipelines_agree(setup, path):
     ((1, 1, 1), (48, 40, 32), 2.0, 25.0),
     ((3, 2, 2), (24, 30, 20), 2.5, 30.0),
     ((2, 3, 3), (20, 18, 16), 1.5, 60.0),
+    ((3, 1, 1), (184, 12, 16), 1.5, 25.0),
 ])
 def test_direct_pipeline_other_shapes(scale, size, psf, lsf):
     """Direct pipeline at other downscale factors and tap buckets: model
     cubes against the FP64 oracle (1e-5 normwise + ambiguity bound) and
-    chi^2 against the fused FFT pipeline."""
+    chi^2 against the fused FFT pipeline.  The last shape has hi-res rows
+    wider than the 512 lines an x-pass block keeps in registers."""
     from gbkfit_amd.batch import BatchEvaluator
     drv = H.driver()
     p = dict(P.c1_params(), size=size, scale=scale, psf_fwhm=psf,
