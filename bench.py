@@ -443,6 +443,31 @@ def cpu_baseline(seconds, thick=False):
                        f"is absent)")
 
 
+def smdisk_op_count(drv, model, sets):
+    """On-disk spaxels V and channel deposits sum n_ch of the thin smdisk
+    per parameter set, counted on the plain path's hi-res cube (the spaxels
+    and voxels with a nonzero deposit; the rint windows are the
+    reference's, gmodels.hpp:91-116), and the op count W = 60 V + 12 sum
+    n_ch with each transcendental as 4 ops (means over `sets`)."""
+    import torch
+    dm, g = model.dmodel(), model.gmodel()
+    c = dm.dcube()
+    vs, ns = [], []
+    for p in sets:
+        model.model_d(p)                  # prepares the scratch cube
+        drv.mem_fill(c.scratch_dcube(), 0)
+        g.evaluate_scube(drv, p, c.scratch_dcube(), c.scratch_wcube(),
+                         c.scratch_size(), c.scratch_step(),
+                         c.scratch_zero(), c.rota(), c.dtype(), None)
+        nz = c.scratch_dcube() != 0
+        vs.append(int(nz.any(0).sum()))
+        ns.append(int(nz.sum()))
+    torch.cuda.synchronize()
+    v, n = float(np.mean(vs)), float(np.mean(ns))
+    return dict(v_disk=v, sum_nch=n, ops=60.0 * v + 12.0 * n,
+                sets=len(sets))
+
+
 def algorithmic_bytes(ev, info, live=1.0, glive=1.0, blive=1.0):
     """Compulsory HBM bytes per parameter set of each kernel (or stage).
     Direct pipeline: `live` is the fraction of x-pass tiles (row x channel
@@ -698,6 +723,28 @@ def main():
     dom = max(kern_ms, key=kern_ms.get)
     dk = kernels[dom]
     traffic = measured_traffic(dom)
+    # op roofline of the smdisk work: per-spaxel geometry and traits in
+    # batch_lines_thin, the channel deposits in the x pass (which also runs
+    # the x filter, so this understates the deposit's own rate)
+    smdisk = None
+    if info['pipeline'] == 'direct' and not args.thick:
+        smdisk = smdisk_op_count(drv, model, param_dicts(arrays, 16))
+        t_ms = sum(kernels[k]['avg_ms_per_launch']
+                   for k in ('batch_lines_thin', 'sep_xpass') if k in kernels)
+        if t_ms > 0:
+            tfs = smdisk['ops'] * per_chunk / (t_ms * 1e-3) / 1e12
+            xf = aflops.get('sep_xpass', 0.0)
+            both = (smdisk['ops'] + xf) * per_chunk / (t_ms * 1e-3) / 1e12
+            smdisk.update(
+                kernels='batch_lines_thin + sep_xpass',
+                achieved_tflops=tfs, frac=tfs / FP32_PEAK_TFLOPS,
+                with_x_filter_tflops=both,
+                with_x_filter_frac=both / FP32_PEAK_TFLOPS,
+                note='W = 60 V + 12 sum n_ch FP32 ops per set (a '
+                     'transcendental counts 4), V and n_ch counted on the '
+                     'hi-res cube of 16 sets of the workload; the time is '
+                     'both kernels per 64-set launch, the x pass also '
+                     'filtering (with_x_filter adds its useful flops)')
     # the binding roof of the dominant kernel: FP32 compute when its flop
     # rate is the larger fraction of peak, else HBM bandwidth
     if dk.get('frac_fp32', 0) > dk.get('frac', 0):
@@ -776,6 +823,7 @@ def main():
                                 convolution=stage_ms[1] / args.steps,
                                 downscale_chi2=stage_ms[2] / args.steps),
         kernels=kernels,
+        smdisk_ops=smdisk,
         roofline=dict(roofline, sets_per_launch=per_chunk),
         serving=serving,
         ragged=ragged,

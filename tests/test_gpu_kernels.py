@@ -31,6 +31,26 @@ def test_smdisk_cases(drv, case):
     assert nbad == 0
 
 
+@pytest.mark.parametrize('name', ['thin_basic', 'thin_profiles_a'])
+def test_smdisk_deposit_counts_match_oracle(drv, name):
+    """The smdisk op count bench.py reports (spaxels with a line and voxels
+    with a nonzero deposit on the hi-res cube, 60 V + 12 sum n_ch) equals
+    the FP64 oracle's wherever no FP32 near-tie (amb) can move a deposit."""
+    case = [c for c in cases.SMDISK_CASES if c['name'] == name][0]
+    spec = cases.build_spec(case).rounded_f32()
+    grid = oracle.grid_f32(cases.build_grid(case))
+    ref, _ = oracle.smdisk(spec, grid, want=('scube', 'amb'))
+    got = H.smdisk(drv, spec, grid)['scube']
+    sure = ref['amb'] == 0
+    assert sure.mean() > 0.9
+    assert np.array_equal((got != 0)[sure], (ref['scube'] != 0)[sure])
+    col_sure = sure.all(axis=0)
+    assert np.array_equal((got != 0).any(axis=0)[col_sure],
+                          (ref['scube'] != 0).any(axis=0)[col_sure])
+    print(name, 'V', int((got != 0).any(axis=0).sum()), 'sum n_ch',
+          int((got != 0).sum()))
+
+
 @pytest.mark.parametrize('name', ['thin_basic', 'thick_sech2'])
 def test_smdisk_side_outputs(drv, name):
     case = [c for c in cases.SMDISK_CASES if c['name'] == name][0]
