@@ -586,23 +586,22 @@ __global__ __launch_bounds__(256) void batch_mcdisk(
 // Batched binned mcdisk: blockIdx.y = set.  The same cloud body writes a
 // record per cloud (at most cap per set) and ranks it in its tile, with the
 // wave-aggregated LDS ranks and per-block reservations of the plain path.
-constexpr int kMcBatchBinLds = 4096;
-
 __global__ __launch_bounds__(256) void batch_mcdisk_bin(
     gbkhip_disk tmpl, SetLayout lay, const float* __restrict__ params,
     int pstride, gbkhip_grid g, float cflux, const int* __restrict__ csum,
     int csum_len, const unsigned char* hoi, uint32_t key0, uint32_t key1,
     McBin bin, long long cap, int ntiles)
 {
-    extern __shared__ int s_csum[];
-    __shared__ float s_nodes[kMcNodesLds];
-    __shared__ int s_cnt[kMcBatchBinLds];
+    extern __shared__ int s_dyn[];            // mc_bin_lds layout
+    const McBinLds L = mc_bin_lds(csum_len, tmpl.nrnodes, ntiles);
+    float* s_nodes = reinterpret_cast<float*>(s_dyn + L.csum);
+    int* s_cnt = s_dyn + L.csum + L.nodes;
     const int set = blockIdx.y;
     gbkhip_disk d = set_disk(tmpl, lay, params + (size_t)set * pstride);
     d.rnodes = mc_stage_nodes(d.rnodes, d.nrnodes, s_nodes);
     McParams mp;
     mp.cflux = cflux;
-    mp.csum = mc_stage_csum(csum + (size_t)set * csum_len, csum_len, s_csum);
+    mp.csum = mc_stage_csum(csum + (size_t)set * csum_len, csum_len, s_dyn);
     mp.csum_len = csum_len;
     mp.nclouds = (int)min((long long)mp.csum[csum_len - 1], cap);
     mp.hoi = hoi;
@@ -612,7 +611,7 @@ __global__ __launch_bounds__(256) void batch_mcdisk_bin(
     bin.rec += set * cap;
     bin.tr += set * cap;
     bin.tcount += (size_t)set * ntiles;
-    const bool local = ntiles <= kMcBatchBinLds;
+    const bool local = L.cnt > 0;
     if (local) {
         for (int i = threadIdx.x; i < ntiles; i += blockDim.x) s_cnt[i] = 0;
         __syncthreads();
@@ -1821,8 +1820,8 @@ static void deposit_hi(Batch& b, const float* P, int set0, int nsets,
                            (long long)nsets * nt);
         hipLaunchKernelGGL(batch_zero_f32, dim3(4096), dim3(256), 0,
                            c.stream, hi, (long long)nsets * b.n_hi);
-        const size_t lds = D.csum_len <= kMcCsumLds
-                               ? D.csum_len * sizeof(int) : 0;
+        const size_t lds =
+            mc_bin_lds(D.csum_len, D.nrnodes, nt).total() * sizeof(int);
         const int bx = std::max(1, std::min(ceil_div(cap, 256),
                                             8192 / std::max(nsets, 1)));
         hipLaunchKernelGGL(batch_mcdisk_bin, dim3(bx, nsets), dim3(256), lds,

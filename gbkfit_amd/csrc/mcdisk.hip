@@ -36,18 +36,18 @@ __global__ void mc_zero_i32(int* p, int n)
 // reserves one range per tile with a single global atomic, instead of one
 // global atomic per cloud: the central tiles of a disk take most clouds,
 // and same-address atomics serialise.
-constexpr int kMcBinLds = 4096;
 constexpr int kMcBinBlocks = 8192;
 
 __global__ __launch_bounds__(256) void mcdisk_bin_kernel(
     McParams mp, gbkhip_disk d, gbkhip_grid g, McBin bin, int ntiles)
 {
-    extern __shared__ int s_csum[];
-    __shared__ float s_nodes[kMcNodesLds];
-    __shared__ int s_cnt[kMcBinLds];
-    mp.csum = mc_stage_csum(mp.csum, mp.csum_len, s_csum);
+    extern __shared__ int s_dyn[];            // mc_bin_lds layout
+    const McBinLds L = mc_bin_lds(mp.csum_len, d.nrnodes, ntiles);
+    float* s_nodes = reinterpret_cast<float*>(s_dyn + L.csum);
+    int* s_cnt = s_dyn + L.csum + L.nodes;
+    mp.csum = mc_stage_csum(mp.csum, mp.csum_len, s_dyn);
     d.rnodes = mc_stage_nodes(d.rnodes, d.nrnodes, s_nodes);
-    const bool local = ntiles <= kMcBinLds;
+    const bool local = L.cnt > 0;
     if (local) {
         for (int i = threadIdx.x; i < ntiles; i += blockDim.x) s_cnt[i] = 0;
         __syncthreads();
@@ -466,8 +466,10 @@ void launch_mcdisk(Context& c, float cflux, int nclouds, const int* csum,
         // slower)
         const int bin_blocks =
             std::max(1, std::min(ceil_div(nclouds, 256), kMcBinBlocks));
+        const size_t lds_bin =
+            mc_bin_lds(csum_len, d.nrnodes, ntiles).total() * sizeof(int);
         hipLaunchKernelGGL(mcdisk_bin_kernel, dim3(bin_blocks), dim3(256),
-                           lds, c.stream, mp, d, g, bin, ntiles);
+                           lds_bin, c.stream, mp, d, g, bin, ntiles);
         launch_mc_binned_tail(c, bin, bb, 1, nc, nullptr, 0, g, nullptr,
                               mp.qscale, o.scube, 0);
         GBK_HIP(hipGetLastError());

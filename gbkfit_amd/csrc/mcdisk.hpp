@@ -246,6 +246,27 @@ __device__ __forceinline__ const float* mc_stage_nodes(const float* nodes,
     return lds;
 }
 
+// Binned cloud kernels: per-block tile counts in LDS when there are at most
+// this many tiles (else ranks come from global atomics).
+constexpr int kMcBinLds = 4096;
+
+// Their dynamic LDS, in ints: staged cumulative counts, staged sub-ring
+// nodes and per-tile counts, each present only when it fits, so a block
+// takes no more LDS than its cube needs (more blocks per CU).
+struct McBinLds {
+    int csum, nodes, cnt;
+    __host__ __device__ int total() const { return csum + nodes + cnt; }
+};
+__host__ __device__ inline McBinLds mc_bin_lds(int csum_len, int nrnodes,
+                                               int ntiles)
+{
+    McBinLds l;
+    l.csum = csum_len <= kMcCsumLds ? csum_len : 0;
+    l.nodes = nrnodes <= kMcNodesLds ? nrnodes : 0;
+    l.cnt = ntiles <= kMcBinLds ? ntiles : 0;
+    return l;
+}
+
 // Returns the cloud's tile when `bin` is given and the cloud lands in the
 // cube (its record written), else -1.
 __device__ __forceinline__ int mc_cloud(long long ci, const McParams& mp,
