@@ -33,7 +33,7 @@ cube sizes log-uniform from 64x64x32 to 256x256x128, half smdisk, half mcdisk
 with 2^20 clouds; each request returns chi^2 and moment maps 0/1/2 through
 gbkfit_amd.serve.Server (continuous batching over 4 streams, HBM-budget
 admission), in steady state: before the stream every galaxy has captured
-the graphs of batches of 1-4 sets and holds its work buffers while they fit
+the graphs of batches of 1-8 sets and holds its work buffers while they fit
 the budget.  Its throughput and p50/p99 latencies go into the `serving`
 object.
 
@@ -195,7 +195,7 @@ def serve_benchmark(drv, ngal, seconds, rate, slots=4, max_batch=16,
                  dpt_a=float(rng.uniform(5, 50)),
                  **({'bht_s': 1.0} if kinds[t] == 'mcdisk' else {}))
             for t in targets]
-    # steady state: every galaxy's graphs for batches of 1-4 captured, and
+    # steady state: every galaxy's graphs for batches of 1-8 captured, and
     # its work buffers kept while they fit the budget (the server counts
     # them and releases idle ones, least recently used first, for others)
     used = 0
@@ -204,7 +204,7 @@ def serve_benchmark(drv, ngal, seconds, rate, slots=4, max_batch=16,
             if i in set(targets.tolist()) else None
         if p0 is None:
             continue
-        for k in (1, 2, 3, 4):
+        for k in range(1, 9):
             g.results(g.launch([p0] * k, 0), k)
         if used + g.work_bytes() <= budget:
             used += g.work_bytes()
@@ -563,6 +563,9 @@ def main():
     ap.add_argument('--ragged-reps', type=int, default=5)
     ap.add_argument('--no-ragged', action='store_true')
     ap.add_argument('--no-mcdisk', action='store_true')
+    ap.add_argument('--tiles', default=None,
+                    help='direct pipeline tiles "lanes,ysplit" (default: '
+                         'the evaluator\'s, 2,1)')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only copy a 1 GiB device buffer 3 times (known '
                          'bytes for scaling FETCH_SIZE / WRITE_SIZE)')
@@ -621,6 +624,8 @@ def main():
 
     ev = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock, error=0.01,
                         max_batch=args.chunk, template_params=truth)
+    if args.tiles:
+        ev.set_tiles(*(int(v) for v in args.tiles.split(',')))
     use_graph = not args.no_graph
 
     def step():
