@@ -273,3 +273,64 @@ def test_error_cube_mask_weight(path):
         assert abs(chi2[i] - c_ref) <= 1e-5 * c_ref + slack, \
             (path, i, chi2[i], c_ref)
     ev.close()
+
+
+@pytest.mark.parametrize('path', ['direct', 'fused', 'padded'])
+def test_sets_with_no_model_flux(path):
+    """Edge cases in a batch: a disk entirely outside the field and a disk
+    of zero surface brightness give an all-zero model, so chi^2 is the
+    data's own sum of (obs / err)^2 (on the direct pipeline every tile is
+    skipped and every z-pass block takes the precomputed data-only partial);
+    the normal sets around them keep their chi^2 from a batch of their
+    own."""
+    from gbkfit_amd.batch import BatchEvaluator
+    drv = H.driver()
+    p = dict(P.c1_params(), size=(48, 44, 32))
+    model, base = P.build_product(drv, p)
+    ref, _, _ = P.oracle_cube(p)
+    rng = np.random.default_rng(31)
+    mock = (ref + rng.normal(0, 0.01, ref.shape)).astype(np.float32)
+    ev = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock,
+                        error=0.01, max_batch=4, path=path)
+    assert ev.info()['pipeline'] == path
+    normal = [P.gparams(dict(p, incl=i, posa=a)) for i, a in ((45, 20),
+                                                             (70, 250))]
+    outside = dict(normal[0], xpos=1e4, ypos=-1e4)
+    dark = dict(normal[1], bpt_a=0.0)
+    chi2 = ev.evaluate([normal[0], outside, dark, normal[1]],
+                       use_graph=True)
+    data_only = float(np.sum((mock.astype(np.float64) / 0.01) ** 2))
+    for c in chi2[1:3]:
+        assert abs(c - data_only) <= 1e-5 * data_only, (path, c, data_only)
+    own = ev.evaluate(normal, use_graph=False)
+    assert np.array_equal(own, chi2[[0, 3]]), (own, chi2)
+    if path == 'direct':
+        ev.evaluate([outside], use_graph=False)
+        (live, _), (glive, _), (blive, _) = ev.tile_stats()
+        assert live == 0 and glive == 0 and blive == 0
+    ev.close()
+
+
+def test_mcdisk_set_with_no_clouds():
+    """A zero-brightness mcdisk set has a cloud budget of 0: its model is
+    zero and its chi^2 the data-only sum, beside normal sets in the same
+    batch (direct pipeline, binned deposits)."""
+    from gbkfit_amd.batch import BatchEvaluator
+    drv = H.driver()
+    p = dict(P.c1_params(), size=(32, 32, 24), size_z=24)
+    model, base = P.build_mc_product(drv, p, cflux=0.01)
+    ref = model.model_h(base)['scube']['d']
+    rng = np.random.default_rng(41)
+    mock = (ref + rng.normal(0, 0.01, ref.shape)).astype(np.float32)
+    ev = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock,
+                        error=0.01, max_batch=3, template_params=base)
+    assert ev.info()['pipeline'] == 'direct'
+    dark = dict(base, bpt_a=0.0)
+    csum, _ = ev.cloud_budgets([dark])
+    assert csum[0][-1] == 0
+    chi2 = ev.evaluate([base, dark, dict(base, incl=70.0)], use_graph=True)
+    data_only = float(np.sum((mock.astype(np.float64) / 0.01) ** 2))
+    assert abs(chi2[1] - data_only) <= 1e-5 * data_only, (chi2, data_only)
+    alone = ev.evaluate([base], use_graph=False)
+    assert alone[0] == chi2[0]
+    ev.close()
