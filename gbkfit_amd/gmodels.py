@@ -156,32 +156,60 @@ class Disk:
         return out
 
     def _upload(self, driver, packed):
-        """Copies packed arrays into persistent device buffers."""
+        """Copies the packed arrays of one parameter set to the device in one
+        transfer: they lie back to back (16-byte aligned) in a pinned host
+        staging buffer and in one device buffer whose views are the native
+        arguments.  The copy runs on the current stream, before the kernels
+        that read it; the staging buffer is rewritten only once the previous
+        copy has read it."""
+        from .driver.hip import torch as _torch, _tdtype
+        t = _torch()
         if self._driver is not driver or self._dev is None:
             self._driver = driver
             self._dev = {}
             self._dev['rnodes'] = driver.mem_copy_h2d(
                 self.subrnodes.astype(np.float32))
+            self._layout = None
         dev = self._dev
-
-        def put(key, arr):
-            if arr is None:
-                dev[key] = None
-                return
-            cur = dev.get(key)
-            if cur is None or tuple(cur.shape) != arr.shape or \
-                    str(cur.dtype).split('.')[-1] != arr.dtype.name:
-                cur = driver.mem_alloc_d(arr.shape, arr.dtype)
-                dev[key] = cur
-            driver.mem_copy_h2d(arr, cur)
-
-        for k in ('vsys', 'xpos', 'ypos', 'posa', 'incl'):
-            put(k, packed[k])
+        items = [(k, packed[k])
+                 for k in ('vsys', 'xpos', 'ypos', 'posa', 'incl')]
         for k in _TRAIT_SLOTS:
             tr = packed[k]
             for j, name in enumerate(('uids', 'cvalues', 'ccounts', 'pvalues',
                                       'pcounts')):
-                put(f'{k}_{name}', None if tr is None else tr[j])
+                items.append((f'{k}_{name}', None if tr is None else tr[j]))
+        layout = tuple((k, None if a is None else (a.shape, a.dtype.str))
+                       for k, a in items)
+        if layout != self._layout:
+            offs, at = {}, 0
+            for k, a in items:
+                if a is not None:
+                    offs[k] = at
+                    at += (a.nbytes + 15) // 16 * 16
+            total = max(at, 16)
+            self._hbuf = t.empty(total, dtype=t.uint8, pin_memory=True)
+            self._dbuf = t.empty(total, dtype=t.uint8, device=driver.device)
+            self._hnp = self._hbuf.numpy()
+            self._offs = offs
+            self._copied = None
+            for k, a in items:
+                if a is None:
+                    dev[k] = None
+                    continue
+                o = offs[k]
+                dev[k] = self._dbuf[o:o + a.nbytes].view(
+                    _tdtype(a.dtype)).view(a.shape)
+            self._layout = layout
+        if self._copied is not None:
+            self._copied.synchronize()
+        for k, a in items:
+            if a is not None:
+                o = self._offs[k]
+                self._hnp[o:o + a.nbytes] = \
+                    np.ascontiguousarray(a).view(np.uint8).ravel()
+        self._dbuf.copy_(self._hbuf, non_blocking=True)
+        self._copied = t.cuda.Event()
+        self._copied.record()
         return dev
 
     def native_args(self, driver, params):
