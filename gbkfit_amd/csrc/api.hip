@@ -2,6 +2,7 @@
 // context and cache management.  Kernels live in the other .hip files.
 #include "fft_plan.hpp"
 #include "launch.hpp"
+#include "sepconv.hpp"
 
 #include <cstring>
 
@@ -210,11 +211,22 @@ int gbkhip_fft_convolve_cached(gbkhip_ctx* ctx, const int size[3],
         check_ptr(data1_r, "data1_r");
         check_ptr(data2_r, "data2_r");
         std::lock_guard<std::mutex> lk(c->mu);
-        auto p = get_plan(*c, size[0], size[1], size[2]);
-        const size_t nc = (size_t)p->nxh * p->ny * p->nz;
         const auto skey = std::make_tuple(size[0], size[1], size[2],
                                           (uintptr_t)data2_r,
                                           data2_generation);
+        // a kernel that factors into short 1-D kernels (the PSF x LSF cube
+        // DCube builds): three direct circular passes instead of the FFTs
+        auto& sc = c->sepconvs[skey];
+        if (!sc) {
+            sc = std::make_shared<SepConv>();
+            sepconv_plan(*c, size, data2_r, *sc);
+        }
+        if (sc->ok) {
+            sepconv_run(*c, size, *sc, data1_r);
+            return;
+        }
+        auto p = get_plan(*c, size[0], size[1], size[2]);
+        const size_t nc = (size_t)p->nxh * p->ny * p->nz;
         auto& spec = c->spectra[skey];
         if (!spec) {
             spec = std::make_shared<DevBuf>(nc * sizeof(float2));
@@ -237,6 +249,7 @@ int gbkhip_fft_clear_cache(gbkhip_ctx* ctx)
         std::lock_guard<std::mutex> lk(c->mu);
         GBK_HIP(hipStreamSynchronize(c->stream));
         c->spectra.clear();
+        c->sepconvs.clear();
         c->work.clear();
         c->plans.clear();
     });

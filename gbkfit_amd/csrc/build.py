@@ -18,7 +18,8 @@ ROOT = os.path.dirname(PKG)
 BUILD = os.path.join(PKG, '_build')
 OUT = os.path.join(PKG, 'libgbkhip.so')
 
-SOURCES = ['api.hip', 'smdisk.hip', 'mcdisk.hip', 'fft.hip', 'dmodel.hip',
+SOURCES = ['api.hip', 'smdisk.hip', 'mcdisk.hip', 'fft.hip', 'sepconv.hip',
+           'dmodel.hip',
            'objective.hip', 'batch.hip']
 
 FLAGS = ['--offload-arch=gfx950', '-O3', '-fPIC', '-std=c++17',

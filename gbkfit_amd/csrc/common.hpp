@@ -90,6 +90,7 @@ struct DevBuf {
 };
 
 struct FFTPlan;   // fft.hip
+struct SepConv;   // sepconv.hpp
 
 struct Context {
     int device = 0;
@@ -99,6 +100,9 @@ struct Context {
     // cached kernel spectra keyed by (nx, ny, nz, pointer, generation)
     std::map<std::tuple<int, int, int, uintptr_t, uint64_t>,
              std::shared_ptr<DevBuf>> spectra;
+    // separable-kernel analyses, keyed like the spectra
+    std::map<std::tuple<int, int, int, uintptr_t, uint64_t>,
+             std::shared_ptr<SepConv>> sepconvs;
     // per-pointer scratch spectra for fft_convolve_cached
     std::map<std::tuple<int, int, int, uintptr_t>, std::shared_ptr<DevBuf>>
         work;
