@@ -32,6 +32,14 @@ int cv_bucket(int n)
     return 0;
 }
 
+// i mod n for i in [-n, ...): the staged halo wraps at most a few times
+__device__ __forceinline__ int wrap(int i, int n)
+{
+    if (i < 0) i += n;
+    while (i >= n) i -= n;
+    return i;
+}
+
 // Rows: block = kCvRows consecutive rows of length nx.  The LDS tile holds
 // s[i] = m((i - e) mod nx) per row, so output q reads s[q .. q + 2e]
 // (taps W[u] = k(e - u)); the shared row filter writes the row in place.
@@ -50,9 +58,7 @@ __global__ __launch_bounds__(256) void cv_rows(float* __restrict__ data,
     float* rows = data + r0 * nx;
     for (int i = threadIdx.x; i < nr * mp; i += T) {
         const int r = i / mp, c = i - r * mp;
-        int x = (c - e) % nx;
-        if (x < 0) x += nx;
-        M[i] = rows[(size_t)r * nx + x];
+        M[i] = rows[(size_t)r * nx + wrap(c - e, nx)];
     }
     __syncthreads();
     sep_xfilter<1, TAPS>(M, s_w, mp, nr, 0, nx, 1, rows);
@@ -83,9 +89,8 @@ __global__ __launch_bounds__(256) void cv_cols(float* __restrict__ data,
     const int cw = std::min(kCvW, ncols - X0);
     for (int i = threadIdx.x; i < rows * kCvW; i += T) {
         const int r = i / kCvW, c = i - r * kCvW;
-        int y = (r - e) % n;
-        if (y < 0) y += n;
-        tile[r * kCvPitch + c] = c < cw ? base[y * rstride + c] : 0.0f;
+        tile[r * kCvPitch + c] =
+            c < cw ? base[wrap(r - e, n) * rstride + c] : 0.0f;
     }
     __syncthreads();
     const int c = threadIdx.x % kCvW;
