@@ -69,7 +69,9 @@ FP32_PEAK_TFLOPS = 157.3
 SIG = 1 / math.sqrt(8 * math.log(2))
 
 
-def workload(nsets, seed=1234):
+def workload(nsets, seed=1234, thick=False):
+    """Parameter arrays of `nsets` sets; a thick disk adds its sech^2
+    scale height (2 px, the thick variant of the 64^3 case)."""
     rng = np.random.default_rng(seed)
     p = dict(incl=rng.uniform(20, 80, nsets), posa=rng.uniform(0, 360, nsets),
              vt=rng.uniform(100, 300, nsets), rt=rng.uniform(2, 20, nsets),
@@ -78,6 +80,8 @@ def workload(nsets, seed=1234):
                   vsys=np.zeros(nsets), posa=p['posa'], incl=p['incl'],
                   bpt_a=np.ones(nsets), bpt_s=np.full(nsets, 8.0),
                   vpt_rt=p['rt'], vpt_vt=p['vt'], dpt_a=p['disp'])
+    if thick:
+        arrays['bht_s'] = np.full(nsets, 2.0)
     return arrays
 
 
@@ -615,7 +619,7 @@ def main():
 
     drv = DriverHip()
     model = build_model(drv, args.thick)
-    arrays = workload(args.nsets)
+    arrays = workload(args.nsets, thick=args.thick)
     # mock data: the model of the first set plus seeded noise
     truth = param_dicts(arrays, 1)[0]
     cube0 = model.model_h(truth)['scube']['d'].astype(np.float32)
@@ -666,7 +670,12 @@ def main():
     torch.cuda.synchronize()
     t_timed = time.perf_counter() - t1
     ev.set_timing(False)
-    assert np.array_equal(chi2_t, chi2)
+    # the two regions give the same chi^2: bit for bit, except for a thick
+    # disk, whose deposit adds into LDS with float atomics (~1e-9 relative)
+    if args.thick:
+        assert np.allclose(chi2_t, chi2, rtol=1e-7, atol=0)
+    else:
+        assert np.array_equal(chi2_t, chi2)
     info = ev.info()
 
     # plain drop-in path, one parameter set per call
