@@ -113,25 +113,24 @@ __global__ __launch_bounds__(1024) void mc_scan_tiles(
     s_s[t] = sl;
     s_p[t] = sp;
     __syncthreads();
-    if (t == 0) {                      // exclusive scans of 1024 partials
-        int a = 0, b = 0, c = 0;
-        for (int i = 0; i < 1024; ++i) {
-            const int x = s_r[i], y = s_s[i], z = s_p[i];
-            s_r[i] = a;
-            s_s[i] = b;
-            s_p[i] = c;
-            a += x;
-            b += y;
-            c += z;
-        }
-        off[n] = a;
-        soff[n] = b;
-        nsplit[set] = c;
+    for (int d = 1; d < 1024; d <<= 1) {      // inclusive scans
+        const int x = t >= d ? s_r[t - d] : 0;
+        const int y = t >= d ? s_s[t - d] : 0;
+        const int z = t >= d ? s_p[t - d] : 0;
+        __syncthreads();
+        s_r[t] += x;
+        s_s[t] += y;
+        s_p[t] += z;
+        __syncthreads();
     }
-    __syncthreads();
-    r = s_r[t];
-    sl = s_s[t];
-    sp = s_p[t];
+    if (t == 1023) {
+        off[n] = s_r[t];
+        soff[n] = s_s[t];
+        nsplit[set] = s_p[t];
+    }
+    r = s_r[t] - r;                           // exclusive
+    sl = s_s[t] - sl;
+    sp = s_p[t] - sp;
     for (int i = lo; i < hi; ++i) {
         const int c = count[i], ns = (c + kMcSlice - 1) / kMcSlice;
         off[i] = r;
