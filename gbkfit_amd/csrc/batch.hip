@@ -91,11 +91,14 @@ __global__ __launch_bounds__(256) void batch_smdisk_thin(
 }
 
 // Thick disk: 4 waves split the line of sight of 64 columns, spectrum tile
-// in LDS (chunked over channels), tile written in full.
+// in LDS (chunked over channels), tile written in full.  Only columns
+// x < nxe and channels k < nze are evaluated and written, on the rows the
+// grid launches (the direct pipeline reads nothing beyond its inputs; the
+// padded-cube pipeline passes the whole grid).
 __global__ __launch_bounds__(256) void batch_smdisk_thick(
     gbkhip_disk tmpl, SetLayout lay, const float* __restrict__ params,
     int pstride, gbkhip_grid g, float* __restrict__ cubes, long long cstride,
-    int chunk)
+    int chunk, int nxe, int nze)
 {
     extern __shared__ float tile[];
     const int lane = threadIdx.x & 63;
@@ -109,12 +112,12 @@ __global__ __launch_bounds__(256) void batch_smdisk_thick(
     const bool need_theta = disk_needs_theta(d);
     const NodeGeom ng = node_geom(d, 0);
     float* cube = cubes + set * cstride;
-    for (int c0 = 0; c0 < g.spec_size; c0 += chunk) {
-        const int c1 = min(c0 + chunk, g.spec_size);
+    for (int c0 = 0; c0 < nze; c0 += chunk) {
+        const int c1 = min(c0 + chunk, nze);
         for (int i = threadIdx.x; i < chunk * 64; i += blockDim.x)
             tile[i] = 0.0f;
         __syncthreads();
-        if (x < nx) {
+        if (x < nxe) {
             for (int z = wave; z < nz; z += 4) {
                 const VoxelLine L = smdisk_voxel(d, g, ng, x, y, z,
                                                  need_theta, false);
@@ -133,7 +136,7 @@ __global__ __launch_bounds__(256) void batch_smdisk_thick(
         for (int i = threadIdx.x; i < (c1 - c0) * 64; i += blockDim.x) {
             const int k = c0 + i / 64, l = i % 64;
             const int xx = blockIdx.x * 64 + l;
-            if (xx < nx)
+            if (xx < nxe)
                 cube[k * plane + (size_t)y * nx + xx] = tile[i];
         }
         __syncthreads();
@@ -1789,11 +1792,17 @@ static void deposit_hi(Batch& b, const float* P, int set0, int nsets,
         hipLaunchKernelGGL(batch_fixed_to_hi, dim3(1024, nsets), dim3(256), 0,
                            c.stream, icube, hi, b.n_hi, b.qscale + set0);
     } else {
-        const int chunk = std::max(1, std::min(g.spec_size, 240));
-        dim3 grid(ceil_div(g.spat_size[0], 64), g.spat_size[1], nsets);
+        // the direct pipeline reads only its inputs [0, need) of each axis
+        const bool part = b.sep.on && b.sep.cube;
+        const int nxe = part ? b.sep.need[0] : g.spat_size[0];
+        const int nye = part ? b.sep.need[1] : g.spat_size[1];
+        const int nze = part ? b.sep.need[2] : g.spec_size;
+        const int chunk = std::max(1, std::min(nze, 240));
+        dim3 grid(ceil_div(nxe, 64), nye, nsets);
         hipLaunchKernelGGL(batch_smdisk_thick, grid, dim3(256),
                            (size_t)chunk * 64 * sizeof(float), c.stream,
-                           b.tmpl, b.lay, P, D.pstride, g, hi, b.n_hi, chunk);
+                           b.tmpl, b.lay, P, D.pstride, g, hi, b.n_hi, chunk,
+                           nxe, nze);
     }
     GBK_HIP(hipGetLastError());
 }
