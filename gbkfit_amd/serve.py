@@ -18,7 +18,14 @@ galaxy) and runs a scheduler that
     overlaps on the device;
   * completes requests by polling each slot's event, then resolves their
     futures with chi^2 (and moment maps when the galaxy was added with
-    orders).
+    orders);
+  * keeps Python's cyclic garbage collector off the resident state: the
+    objects alive when the server starts (galaxies, models, buffers) are
+    frozen out of its generations (gc.freeze) until stop(), so collections
+    during serving walk only request-time objects.  Measured on 64
+    resident galaxies at 1000 requests/s, collections of the whole heap
+    otherwise paused the scheduler thread and set the p99 latency (2.3-3.6
+    ms against 1.0 ms frozen).
 
 A request's result does not depend on which batch it joined: each set's
 chi^2 is reduced in a fixed order from its own partial sums, and the
@@ -30,6 +37,7 @@ the CPU with stand-in galaxies; GpuGalaxy implements it on BatchEvaluator.
 """
 import collections
 import concurrent.futures
+import gc
 import threading
 import time
 
@@ -62,16 +70,21 @@ class Server:
 
     hbm_budget: bytes the work buffers of all galaxies may occupy together
     (None: no limit); slots: concurrent batched evaluations; max_batch:
-    requests per launch (at most the galaxies' own max_batch).
+    requests per launch (at most the galaxies' own max_batch); freeze_gc:
+    freeze the objects alive at start() out of the cyclic collector until
+    stop() (a process-wide setting: pass False to leave the collector
+    alone).
     """
 
     def __init__(self, hbm_budget=None, slots=4, max_batch=16,
-                 clock=time.perf_counter):
+                 clock=time.perf_counter, freeze_gc=True):
         if slots < 1 or max_batch < 1:
             raise ValueError("slots and max_batch must be >= 1")
         self.hbm_budget = hbm_budget
         self.nslots = slots
         self.max_batch = max_batch
+        self.freeze_gc = freeze_gc
+        self._frozen = False
         self.clock = clock
         self._entries = collections.OrderedDict()
         self._slots = [None] * slots          # (entry, requests, token)
@@ -222,6 +235,10 @@ class Server:
 
     def start(self):
         if self._thread is None:
+            if self.freeze_gc and not self._frozen:
+                gc.collect()
+                gc.freeze()
+                self._frozen = True
             self._stop = False
             self._thread = threading.Thread(target=self._loop, daemon=True)
             self._thread.start()
@@ -240,6 +257,9 @@ class Server:
             self._cv.notify()
         self._thread.join()
         self._thread = None
+        if self._frozen:
+            gc.unfreeze()
+            self._frozen = False
 
     def latency_percentiles(self):
         lat = np.asarray(self.latencies) * 1e3

@@ -181,3 +181,33 @@ def test_threaded_clients_all_resolve():
     assert s.stats['completed'] == 200
     assert all(g.max_inflight == 1 for g in gals.values())
     assert len(s.latencies) == 200
+
+
+def test_start_freezes_resident_objects_until_stop():
+    """start() moves the objects alive at that point (resident galaxies and
+    their state) out of the cyclic collector's generations, so collections
+    while serving do not walk them; stop() puts them back.  freeze_gc=False
+    leaves the collector alone."""
+    import gc
+    gc.unfreeze()
+    s = Server(slots=1)
+    g = FakeGalaxy(1)
+    g.auto = True
+    s.add('a', g)
+    s.start()
+    try:
+        assert gc.get_freeze_count() > 0
+        assert s.submit('a', 1.0).result(5) is not None
+    finally:
+        s.stop()
+    assert gc.get_freeze_count() == 0
+    s2 = Server(slots=1, freeze_gc=False)
+    g2 = FakeGalaxy(1)
+    g2.auto = True
+    s2.add('b', g2)
+    s2.start()
+    try:
+        assert gc.get_freeze_count() == 0
+        assert s2.submit('b', 2.0).result(5) is not None
+    finally:
+        s2.stop()
