@@ -254,13 +254,16 @@ __global__ __launch_bounds__(256) void batch_lines_thin(
     const int x = blockIdx.x * 64 + threadIdx.x;
     const int y = blockIdx.y * 4 + threadIdx.y;
     const int set = blockIdx.z;
+    const gbkhip_disk d = set_disk(tmpl, lay, params + (size_t)set * pstride);
+    // the set's node-0 geometry (two sincos) once per block
+    __shared__ NodeGeom s_ng;
+    if (threadIdx.x == 0 && threadIdx.y == 0) s_ng = node_geom(d, 0);
+    __syncthreads();
     if (y >= ny) return;               // whole waves: a wave is one row
     float4 out = make_float4(0, 0, 1, 0);
     if (x < nx && x < need_x && y < need_y) {
-        const gbkhip_disk d = set_disk(tmpl, lay,
-                                       params + (size_t)set * pstride);
         const bool need_theta = disk_needs_theta(d);
-        const NodeGeom ng = node_geom(d, 0);
+        const NodeGeom ng = s_ng;
         const VoxelLine V = smdisk_voxel(d, g, ng, x, y, 0, need_theta, false);
         if (V.on) out = make_float4(V.orvalue, V.v, V.s, 0);
     }
