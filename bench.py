@@ -553,7 +553,7 @@ def main():
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--nsets', type=int, default=512)
-    ap.add_argument('--chunk', type=int, default=64)
+    ap.add_argument('--chunk', type=int, default=256)
     ap.add_argument('--plain-sets', type=int, default=128)
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--no-cpu', action='store_true')
@@ -702,7 +702,7 @@ def main():
     launches = args.steps * max(chunks, 1)
     per_chunk = args.nsets / max(chunks, 1)
     # the last timed step ran one lane: its tile statistics are the last
-    # chunk's (64 sets of the workload)
+    # chunk's (a chunk of sets of the workload)
     live_tiles = live_groups = live_blocks = None
     if info['pipeline'] == 'direct':
         (nzt, tot), (gz, gt), (bz, bt) = ev.tile_stats()
@@ -757,7 +757,7 @@ def main():
                 note='W = 60 V + 12 sum n_ch FP32 ops per set (a '
                      'transcendental counts 4), V and n_ch counted on the '
                      'hi-res cube of 16 sets of the workload; the time is '
-                     'both kernels per 64-set launch, the x pass also '
+                     'both kernels per launch, the x pass also '
                      'filtering (with_x_filter adds its useful flops)')
     # the binding roof of the dominant kernel: FP32 compute when its flop
     # rate is the larger fraction of peak, else HBM bandwidth
