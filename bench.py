@@ -13,10 +13,10 @@ PSF FWHM 3 px, LSF FWHM 2 channels (10 km/s), reference DCube padding
 its automatic choice, the direct separable pipeline: x, y and z filters
 with the block mean folded into the taps over the 296x296x110 inputs that
 reach the outputs, skipping tiles no line reaches (the fused FFT pipeline,
-300x300x120, gives the same result and is timed in the tests).  The timed
-region covers packing the parameter block,
-the upload, the batched native evaluation (one HIP graph replay per chunk
-sequence) and the download of the chi^2 vector.
+300x300x120, gives the same result and is timed in the tests), in chunks
+of 256 sets (`--chunk`).  The timed region covers packing the parameter
+block, the upload, the batched native evaluation (one HIP graph replay per
+chunk sequence) and the download of the chi^2 vector.
 
 A second timed region repeats the steps with direct launches and HIP events
 after every kernel (on the kernels' own stream): the per-kernel times give
