@@ -940,10 +940,17 @@ __global__ __launch_bounds__(256) void sep_ypass(
 #pragma unroll
             for (int j = 0; j < J; ++j) o[j] = fmaf(w, v[SY * j + u], o[j]);
         }
+        // full groups (all but the last rows' group) store without a
+        // bounds check per output
+        const int Yb = Y0 + gi * J;
+        float* d = dst + (size_t)Yb * nxl;
+        if (X < nxl && Yb + J <= nyl) {
 #pragma unroll
-        for (int j = 0; j < J; ++j) {
-            const int Y = Y0 + gi * J + j;
-            if (X < nxl && Y < nyl) dst[(size_t)Y * nxl] = o[j];
+            for (int j = 0; j < J; ++j) d[(size_t)j * nxl] = o[j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < J; ++j)
+                if (X < nxl && Yb + j < nyl) d[(size_t)j * nxl] = o[j];
         }
     }
 }
