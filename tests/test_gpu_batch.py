@@ -334,3 +334,50 @@ def test_mcdisk_set_with_no_clouds():
     alone = ev.evaluate([base], use_graph=False)
     assert alone[0] == chi2[0]
     ev.close()
+
+
+@pytest.mark.parametrize('path', ['direct', 'fused', 'padded'])
+def test_loose_tilted_disk_matches_plain_path(path):
+    """A loose and tilted thin disk (node-wise centre, systemic velocity,
+    position angle and inclination; ring_info, gmodels.hpp:217-323) on each
+    batched pipeline: chi^2 of every set within 1e-5 of the plain one-model
+    path, and the model cubes within 1e-5 normwise."""
+    from gbkfit_amd import gmodels, psflsf
+    from gbkfit_amd.batch import BatchEvaluator
+    from gbkfit_amd.dmodels import DModelSCube
+    from gbkfit_amd.objective import Data, Model, Objective
+    from gbkfit_amd.traits import make
+    drv = H.driver()
+    rn = np.arange(0, 22, 1.5)
+    cmp = gmodels.SpectralSMDisk2D(
+        True, True, make('bpt', 'exponential'), make('vpt', 'tan_arctan'),
+        make('dpt', 'uniform'), rnodes=rn)
+    dm = DModelSCube((48, 44, 32), (1, 1, 10), scale=(2, 2, 1),
+                     psf=psflsf.PSFGauss(1.1), lsf=psflsf.LSFGauss(8.5))
+    model = Model(drv, dm, gmodels.GModelKinematics2D([cmp]))
+    n = rn.size
+
+    def params(i):
+        t = np.linspace(0.0, 1.0, n)
+        return dict(xpos=0.5 * np.sin(3 * t + i), ypos=-0.4 * t,
+                    vsys=5.0 * t, posa=30.0 + 40.0 * t + 10 * i,
+                    incl=40.0 + 25.0 * t + 5 * i, bpt_a=1.0, bpt_s=6.0,
+                    vpt_rt=4.0, vpt_vt=180.0 + 20 * i, dpt_a=25.0)
+    sets = [params(i) for i in range(3)]
+    ref = model.model_h(sets[0])['scube']['d']
+    rng = np.random.default_rng(51)
+    mock = (ref + rng.normal(0, 0.01, ref.shape)).astype(np.float32)
+    ev = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock,
+                        error=0.01, max_batch=2, template_params=sets[0],
+                        path=path)
+    assert ev.info()['pipeline'] == path
+    chi2, cubes = ev.evaluate(sets, want_cubes=True, use_graph=True)
+    obj = Objective([dict(scube=Data(mock, error=np.full(mock.shape, 0.01,
+                                                          np.float32)))],
+                    [model])
+    for i, s in enumerate(sets):
+        plain = -2 * obj.log_likelihood(s)[0]
+        assert abs(plain - chi2[i]) <= 1e-5 * chi2[i], (path, i, plain,
+                                                        chi2[i])
+        assert H.normwise(cubes[i], model.model_h(s)['scube']['d']) < 1e-5
+    ev.close()
