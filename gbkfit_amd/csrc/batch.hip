@@ -233,11 +233,14 @@ __global__ void batch_chi2_final(const double* __restrict__ partials,
 // size) reach no stored output through a kernel of half-width edge_i, so
 // they are zero and the FFT sizes are the smallest 5-smooth ones above it.
 // ---------------------------------------------------------------------------
-__global__ void sep_zero_masks(unsigned long long* m, long long n)
+__global__ void sep_zero_masks(unsigned long long* m, long long n,
+                               unsigned long long* m2)
 {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-         i < n; i += (long long)gridDim.x * blockDim.x)
+         i < n; i += (long long)gridDim.x * blockDim.x) {
         m[i] = 0ull;
+        if (m2) m2[i] = 0ull;
+    }
 }
 
 // Line of sight of each spaxel (thin disk): the line it deposits, flux 0
@@ -249,7 +252,8 @@ __global__ __launch_bounds__(256) void batch_lines_thin(
     gbkhip_disk tmpl, SetLayout lay, const float* __restrict__ params,
     int pstride, gbkhip_grid g, int need_x, int need_y, int nx, int ny,
     float4* __restrict__ lines, long long lstride, int form,
-    unsigned long long* __restrict__ rowmask, int zc, int need_z)
+    unsigned long long* __restrict__ rowmask, int zc, int need_z,
+    unsigned long long* __restrict__ segmask)
 {
     const int x = blockIdx.x * 64 + threadIdx.x;
     const int y = blockIdx.y * 4 + threadIdx.y;
@@ -294,8 +298,17 @@ __global__ __launch_bounds__(256) void batch_lines_thin(
                    << c0;
         }
         for (int o = 32; o > 0; o >>= 1) bits |= __shfl_xor(bits, o, 64);
-        if (threadIdx.x == 0 && bits)
+        if (threadIdx.x == 0 && bits) {
             atomicOr(&rowmask[(size_t)set * ny + y], bits);
+            // the row's 64-spaxel segment (this wave) carries a line
+            if (segmask)
+                atomicOr(&segmask[(size_t)set * ny + y],
+                         1ull << min((int)blockIdx.x, 63));
+        }
+        // a segment below the 63rd with no line reaching the x pass's
+        // channels is not written: the x pass reads only live segments
+        // (and every segment from the 63rd on) and takes zeros for the rest
+        if (segmask && !bits && blockIdx.x < 63) return;
     }
     if (x < nx) lines[set * lstride + (size_t)y * nx + x] = out;
 }
@@ -729,7 +742,8 @@ __global__ __launch_bounds__(256) void sep_xpass(
     const float4* __restrict__ lines, long long lstride, int lpitch,
     gbkhip_grid g, const float* __restrict__ wx, int need_x, int need_z,
     int nxl, int mp, int zc, float* __restrict__ P1, long long p1stride,
-    int ny1, const unsigned long long* __restrict__ rowmask)
+    int ny1, const unsigned long long* __restrict__ rowmask,
+    const unsigned long long* __restrict__ segmask)
 {
     extern __shared__ float4 M4[];        // [zc][mp] floats
     float* M = reinterpret_cast<float*>(M4);
@@ -738,6 +752,13 @@ __global__ __launch_bounds__(256) void sep_xpass(
     const int y = blockIdx.x, set = blockIdx.z;
     unsigned long long live = rowmask[(size_t)set * ny1 + y];
     if (!live) return;
+    // 64-spaxel segments with a line (batch_lines_thin wrote only those
+    // below the 63rd; the rest are zero)
+    const unsigned long long seg = segmask[(size_t)set * ny1 + y];
+    auto written = [&](int x) {
+        const int sg = x >> 6;
+        return sg >= 63 || ((seg >> sg) & 1ull);
+    };
     for (int i = threadIdx.x; i < TX; i += T) s_w[i] = wx[i];
     // lines in form 1 (batch_lines_thin): amplitude and window precomputed
     const float4* lrow = lines + set * lstride + (size_t)y * lpitch;
@@ -745,7 +766,7 @@ __global__ __launch_bounds__(256) void sep_xpass(
 #pragma unroll
     for (int u = 0; u < UX; ++u) {
         const int x = threadIdx.x + u * T;
-        V[u] = x < need_x ? lrow[x] : make_float4(0, 0, 0, 0);
+        V[u] = x < need_x && written(x) ? lrow[x] : make_float4(0, 0, 0, 0);
     }
     const int n4 = (zc * mp) >> 2;
     float* out = P1 + set * p1stride + (size_t)y * nxl;
@@ -774,6 +795,7 @@ __global__ __launch_bounds__(256) void sep_xpass(
         for (int u = 0; u < UX; ++u)
             if (V[u].x != 0.0f) deposit(V[u], threadIdx.x + u * T);
         for (int x = threadIdx.x + UX * T; x < need_x; x += T) {
+            if (!written(x)) continue;
             const float4 L = lrow[x];
             if (L.x != 0.0f) deposit(L, x);
         }
@@ -1054,6 +1076,7 @@ struct Sep {
     DevBuf wx, wy, wz;            // taps
     DevBuf P1, P2;                // work: per set (z, y, X) and (z, Y, X)
     DevBuf masks;                 // work: per set and row, live chunks
+    DevBuf segmask;               // work: per set and row, live segments
     DevBuf gflags;                // work: per set, channel, y-pass group
     DevBuf zero_part;             // z-pass partials of a zero model (data)
     int ngy = 0;                  // y-pass output groups per channel
@@ -1222,7 +1245,8 @@ static size_t work_bytes(const Batch& B)
         else
             n += mb * (size_t)B.sep.need[1] * B.sep.need[0] * sizeof(float4);
         n += mb * (size_t)(B.sep.p1stride + B.sep.p2stride) * sizeof(float);
-        n += mb * (size_t)B.sep.need[1] * sizeof(uint64_t);
+        n += mb * (size_t)B.sep.need[1] * sizeof(uint64_t) *
+             (B.sep.cube ? 1 : 2);
         n += mb * (size_t)B.sep.need[2] * B.sep.ngy;
     } else if (B.fused.on) {
         n += mb * (size_t)B.fused.sstride * sizeof(float2);
@@ -1258,6 +1282,9 @@ static void alloc_work(Batch& B)
         B.sep.P1.alloc(mb * (size_t)B.sep.p1stride * sizeof(float));
         B.sep.P2.alloc(mb * (size_t)B.sep.p2stride * sizeof(float));
         B.sep.masks.alloc(mb * (size_t)B.sep.need[1] * sizeof(uint64_t));
+        if (!B.sep.cube)
+            B.sep.segmask.alloc(mb * (size_t)B.sep.need[1] *
+                                sizeof(uint64_t));
         B.sep.gflags.alloc(mb * (size_t)B.sep.need[2] * B.sep.ngy);
     } else if (B.fused.on) {
         B.fused.S.alloc(mb * B.fused.sstride * sizeof(float2));
@@ -1289,6 +1316,7 @@ static void release_work(Batch& B)
     B.sep.P1.release();
     B.sep.P2.release();
     B.sep.masks.release();
+    B.sep.segmask.release();
     B.sep.gflags.release();
     B.hi.release();
     B.spec.release();
@@ -1523,7 +1551,7 @@ static void run_chunk_fused(Batch& b, const float* params, int set0,
         hipLaunchKernelGGL(batch_lines_thin, g0, dim3(64, 4), 0, c.stream,
                            b.tmpl, b.lay, P, D.pstride, D.grid, F.need[0],
                            F.need[1], F.nx, F.ny, lines, F.lstride, 0,
-                           nullptr, 1, 0);
+                           nullptr, 1, 0, nullptr);
         GBK_HIP(hipGetLastError());
         mark(b, ci, 1);
         dim3 grid(ceil_div(F.nx, 1 << F.fz.lshift), F.ny, nsets);
@@ -1844,7 +1872,7 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
         float* hi = b.hi.as<float>() + mb * b.n_hi;
         hipLaunchKernelGGL(sep_zero_masks, dim3(ceil_div(nsets * ny, 256)),
                            dim3(256), 0, c.stream, masks,
-                           (long long)nsets * ny);
+                           (long long)nsets * ny, nullptr);
         GBK_HIP(hipGetLastError());
         deposit_hi(b, par, set0, nsets, lane, hi);
         mark(b, ci, 1);
@@ -1856,14 +1884,16 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
                          nz, nxl, P.mp, P.zc, P1, P.p1stride, ny, masks);
         GBK_HIP(hipGetLastError());
     } else {
+        unsigned long long* segs = P.segmask.as<unsigned long long>() +
+                                   mb * (size_t)ny;
         hipLaunchKernelGGL(sep_zero_masks, dim3(ceil_div(nsets * ny, 256)),
                            dim3(256), 0, c.stream, masks,
-                           (long long)nsets * ny);
+                           (long long)nsets * ny, segs);
         GBK_HIP(hipGetLastError());
         dim3 g0(ceil_div(nx, 64), ceil_div(ny, 4), nsets);
         hipLaunchKernelGGL(batch_lines_thin, g0, dim3(64, 4), 0, c.stream,
                            b.tmpl, b.lay, par, D.pstride, D.grid, nx, ny, nx,
-                           ny, lines, lstride, 1, masks, P.zc, nz);
+                           ny, lines, lstride, 1, masks, P.zc, nz, segs);
         GBK_HIP(hipGetLastError());
         mark(b, ci, 1);
         dim3 grid(ny, 1, nsets);
@@ -1871,7 +1901,7 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
         GBK_SEP_DISPATCH(sep_xpass, D.scale[0], P.tx, grid, dim3(kSepThreads),
                          lds, c.stream, lines, lstride, nx, D.grid,
                          P.wx.as<float>(), nx, nz, nxl, P.mp, P.zc,
-                         P1, P.p1stride, ny, masks);
+                         P1, P.p1stride, ny, masks, segs);
         GBK_HIP(hipGetLastError());
     }
     mark(b, ci, 2);
