@@ -499,7 +499,8 @@ def _tune_key(dmodel, gmodel, mc):
 
 
 def autotune(driver, dmodel, gmodel, obs, sample_params, error=None,
-             mask=None, weight=1.0, max_batches=(16, 32, 64), reps=3,
+             mask=None, weight=1.0, max_batches=(16, 32, 64, 128, 256),
+             reps=3,
              budget_bytes=None, template_params=None, cache=True):
     """Chooses the batched pipeline (direct / fused / padded), the chunk size
     and the direct pipeline's tiles (lanes, ysplit) for one galaxy model by
