@@ -533,7 +533,7 @@ def measured_traffic(kernel):
     summaries (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel
     calibration): a line 'traffic_bytes_per_launch <kernel>: <bytes>' in
     profiles/*_traffic.txt, or None when none covers it."""
-    name = kernel.split(' ')[0]
+    name = kernel.split('<')[0].split(' ')[0]
     d = os.path.join(ROOT, 'profiles')
     if not os.path.isdir(d):
         return None
