@@ -60,6 +60,7 @@ N.register_signatures({
     'gbkhip_batch_info': ([_c.c_void_p, _c.POINTER(_c.c_int)], _c.c_int),
     'gbkhip_batch_set_tiles': ([_c.c_void_p, _c.c_int, _c.c_int], _c.c_int),
     'gbkhip_batch_reserve_clouds': ([_c.c_void_p, _c.c_longlong], _c.c_int),
+    'gbkhip_batch_check': ([_c.c_void_p, _c.POINTER(_c.c_int)], _c.c_int),
     'gbkhip_batch_tile_stats': ([_c.c_void_p, _c.POINTER(_c.c_longlong)],
                                 _c.c_int),
     'gbkhip_group_create': ([_c.c_void_p, _c.POINTER(_c.c_void_p), _c.c_int,
@@ -382,11 +383,25 @@ class BatchEvaluator:
                self._params_dev.data_ptr(), int(nsets),
                self._chi2_dev.data_ptr(), cubes, int(bool(use_graph)))
 
+    def check(self):
+        """Raises when an evaluation since the last check had an mcdisk set
+        with more clouds than the reserved capacity (waits for the batch's
+        stream)."""
+        v = _c.c_int()
+        N.call('gbkhip_batch_check', self.handle, _c.byref(v))
+        if v.value:
+            raise N.GbkHipError(
+                1, 'gbkhip_batch_check', "an mcdisk cloud budget exceeded "
+                "the reserved capacity (reserve_clouds); the results of the "
+                "evaluations since the last check are not valid")
+
     def evaluate(self, params_list, want_cubes=False, use_graph=True):
         """chi^2 per set (numpy FP64), and the model cubes if asked."""
         n = self.upload(params_list)
         self.run(n, want_cubes, use_graph)
         chi2 = self._chi2_dev.cpu().numpy().copy()
+        if self.mc:
+            self.check()
         if want_cubes:
             return chi2, self._cubes_dev.cpu().numpy().copy()
         return chi2
@@ -638,6 +653,9 @@ class RaggedGroup:
                K if want_cubes else None, int(bool(use_graph)))
         chi2 = [ev._chi2_dev.cpu().numpy().copy() if S[i] else
                 np.zeros(0) for i, ev in enumerate(self.members)]
+        for ev in self.members:
+            if ev.mc:
+                ev.check()
         if want_cubes:
             cubes = [ev._cubes_dev.cpu().numpy().copy()
                      if S[i] and download_cubes else None

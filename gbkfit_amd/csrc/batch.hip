@@ -610,7 +610,7 @@ __global__ __launch_bounds__(256) void batch_mcdisk_bin(
     gbkhip_disk tmpl, SetLayout lay, const float* __restrict__ params,
     int pstride, gbkhip_grid g, float cflux, const int* __restrict__ csum,
     int csum_len, const unsigned char* hoi, uint32_t key0, uint32_t key1,
-    McBin bin, long long cap, int ntiles)
+    McBin bin, long long cap, int ntiles, int* __restrict__ overflow)
 {
     extern __shared__ int s_dyn[];            // mc_bin_lds layout
     const McBinLds L = mc_bin_lds(csum_len, tmpl.nrnodes, ntiles);
@@ -624,6 +624,10 @@ __global__ __launch_bounds__(256) void batch_mcdisk_bin(
     mp.csum = mc_stage_csum(csum + (size_t)set * csum_len, csum_len, s_dyn);
     mp.csum_len = csum_len;
     mp.nclouds = (int)min((long long)mp.csum[csum_len - 1], cap);
+    // a budget beyond the reserved capacity would lose clouds: flag it
+    // (the host reports it; the batch's results are not valid)
+    if (mp.csum[csum_len - 1] > cap && blockIdx.x == 0 && threadIdx.x == 0)
+        *overflow = 1;
     mp.hoi = hoi;
     mp.key0 = key0;
     mp.key1 = key1;
@@ -1133,6 +1137,8 @@ struct Batch {
     int mc_tw = 0, mc_ntx = 0, mc_ntiles = 0;
     long long mc_cap = 0;
     DevBuf mcbin;
+    DevBuf overflow;        // int: set when a set's clouds exceed mc_cap
+    int max_grid_z = 65535; // device limit on gridDim.z (and y)
     unsigned long long work_gen = 0;    // bumped whenever work is allocated
     const int* csum = nullptr;       // mcdisk inputs of the current call
     const double* qscale = nullptr;
@@ -1810,7 +1816,7 @@ static void deposit_hi(Batch& b, const float* P, int set0, int nsets,
                            c.stream, b.tmpl, b.lay, P, D.pstride, g, D.cflux,
                            b.csum + (size_t)set0 * D.csum_len, D.csum_len,
                            D.hoi, (uint32_t)D.seed, (uint32_t)(D.seed >> 32),
-                           bin, cap, nt);
+                           bin, cap, nt, b.overflow.as<int>());
         GBK_HIP(hipGetLastError());
         launch_mc_binned_tail(c, bin, bb, nsets, 0,
                               b.csum + (size_t)set0 * D.csum_len, D.csum_len,
@@ -1905,16 +1911,25 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
         GBK_HIP(hipGetLastError());
     }
     mark(b, ci, 2);
-    {
-        dim3 grid(ceil_div(nxl, kSepXW), ceil_div(nyl, P.yblk), nz * nsets);
+    // the y and z passes put (channel or channel block, set) on grid z:
+    // launched over slices of sets that keep it within the device limit
+    auto slices = [&](int per_set, auto&& launch) {
+        const int step = std::max(1, b.max_grid_z / per_set);
+        for (int s0 = 0; s0 < nsets; s0 += step)
+            launch(s0, std::min(step, nsets - s0));
+    };
+    slices(nz, [&](int s0, int ns) {
+        dim3 grid(ceil_div(nxl, kSepXW), ceil_div(nyl, P.yblk), nz * ns);
         const int rows = D.scale[1] * (P.yblk - 1) + P.ty;
         const size_t lds = (size_t)rows * kSepYPitch * sizeof(float);
         GBK_SEP_DISPATCH(sep_ypass, D.scale[1], P.ty, grid, dim3(kSepThreads),
-                         lds, c.stream, P1, P.p1stride,
+                         lds, c.stream, P1 + s0 * P.p1stride, P.p1stride,
                          P.wy.as<float>(), nxl, nyl, ny, nz, P.yblk,
-                         P2, P.p2stride, masks, P.zc, gflags, P.ngy);
+                         P2 + s0 * P.p2stride, P.p2stride,
+                         masks + (size_t)s0 * ny, P.zc,
+                         gflags + (size_t)s0 * nz * P.ngy, P.ngy);
         GBK_HIP(hipGetLastError());
-    }
+    });
     mark(b, ci, 3);
     DataArgs da;
     da.obs = D.obs;
@@ -1922,17 +1937,19 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
     da.mask = D.mask;
     da.err_scalar = D.err_scalar;
     da.weight = D.weight;
-    {
+    slices(P.zblocks, [&](int s0, int ns) {
         dim3 grid(ceil_div(nxl, 64), ceil_div(nyl, kSepThreads / 64),
-                  P.zblocks * nsets);
+                  P.zblocks * ns);
         GBK_SEP_DISPATCH(sep_zpass, D.scale[2], P.tz, grid, dim3(kSepThreads),
-                         0, c.stream, P2, P.p2stride,
+                         0, c.stream, P2 + s0 * P.p2stride, P.p2stride,
                          P.wz.as<float>(), nxl, nyl, nzl, nz, P.zblocks, da,
-                         cubes ? cubes + (size_t)set0 * b.n_lo : nullptr,
-                         b.n_lo, partials, b.nparts, gflags, P.ngy,
+                         cubes ? cubes + (size_t)(set0 + s0) * b.n_lo
+                               : nullptr,
+                         b.n_lo, partials + (size_t)s0 * b.nparts, b.nparts,
+                         gflags + (size_t)s0 * nz * P.ngy, P.ngy,
                          sep_jy(P.ty), P.zero_part.as<double>());
         GBK_HIP(hipGetLastError());
-    }
+    });
     mark(b, ci, 4);
     hipLaunchKernelGGL(batch_chi2_final, dim3(nsets), dim3(256), 0, c.stream,
                        partials, b.nparts, chi2, set0);
@@ -2117,12 +2134,23 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
                   D.grid.spat_size[1] == D.size_hi[1] &&
                   D.grid.spec_size == D.size_hi[2], GBKHIP_ERR_ARG,
                   "gmodel grid must cover the padded cube");
+        int gy = 0, gz = 0;
+        GBK_HIP(hipDeviceGetAttribute(&gy, hipDeviceAttributeMaxGridDimY,
+                                      c->device));
+        GBK_HIP(hipDeviceGetAttribute(&gz, hipDeviceAttributeMaxGridDimZ,
+                                      c->device));
+        // one block row per set on grid y or z of the per-set kernels
+        GBK_CHECK(D.max_batch <= std::min(gy, gz), GBKHIP_ERR_ARG,
+                  "max_batch exceeds the device's grid y/z limit");
         auto* b = new gbkhip_batch();
         Batch& B = b->impl;
         B.ctx = c;
         B.device = c->device;
         B.desc = D;
+        B.max_grid_z = gz;
         try {
+            B.overflow.alloc(sizeof(int));
+            GBK_HIP(hipMemset(B.overflow.ptr, 0, sizeof(int)));
             build_layout(B);
             const int nx = D.size_hi[0], ny = D.size_hi[1], nz = D.size_hi[2];
             B.n_hi = (long long)nx * ny * nz;
@@ -2254,8 +2282,10 @@ static void evaluate_impl(gbkhip_batch* b, const float* params,
         hipStream_t s;
         ~Restore() { c.stream = s; }
     } restore{c, ctx_stream};
-    alloc_work(B);
+    // work buffers (and the zero fill of a fresh int64 cube) on the
+    // evaluation's own stream, ahead of the launches that use them
     c.stream = target;
+    alloc_work(B);
     B.csum = csum;
     B.qscale = qscale;
     if (!use_graph || B.timing) {   // stage events need direct launches
@@ -2552,7 +2582,7 @@ int gbkhip_batch_tile_stats(gbkhip_batch* b, long long* counts)
     return guarded([&] {
         GBK_CHECK(b && counts, GBKHIP_ERR_ARG, "null argument");
         Batch& B = b->impl;
-        for (int i = 0; i < 4; ++i) counts[i] = 0;
+        for (int i = 0; i < 6; ++i) counts[i] = 0;
         if (!B.sep.on || !B.work || B.sep.last_sets == 0) return;
         GBK_HIP(hipSetDevice(B.device));
         const hipStream_t s = B.stream ? B.stream : B.ctx->stream;
@@ -2594,6 +2624,22 @@ int gbkhip_batch_tile_stats(gbkhip_batch* b, long long* counts)
                     counts[4] += live;
                     counts[5] += 1;
                 }
+    });
+}
+
+int gbkhip_batch_check(gbkhip_batch* b, int* overflow)
+{
+    return guarded([&] {
+        GBK_CHECK(b && overflow, GBKHIP_ERR_ARG, "null argument");
+        Batch& B = b->impl;
+        GBK_HIP(hipSetDevice(B.device));
+        const hipStream_t s = B.stream ? B.stream : B.ctx->stream;
+        GBK_HIP(hipStreamSynchronize(s));
+        int h = 0;
+        GBK_HIP(hipMemcpy(&h, B.overflow.ptr, sizeof(int),
+                          hipMemcpyDeviceToHost));
+        if (h) GBK_HIP(hipMemset(B.overflow.ptr, 0, sizeof(int)));
+        *overflow = h;
     });
 }
 

@@ -237,7 +237,9 @@ typedef struct {
 typedef struct gbkhip_batch gbkhip_batch;
 
 /* Builds the plan: copies the layout, computes the kernel spectrum and sizes
- * the chunk buffers (max_batch padded cubes and spectra). */
+ * the chunk buffers (max_batch padded cubes and spectra).  max_batch must
+ * not exceed the device's grid y/z limit (one grid row per set); passes
+ * whose grid holds several rows per set are launched over slices of sets. */
 int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
                         gbkhip_batch** out);
 int gbkhip_batch_destroy(gbkhip_batch* batch);
@@ -261,8 +263,14 @@ int gbkhip_batch_evaluate_clouds(gbkhip_batch* batch, const float* params,
 /* Cloud capacity per set of an mcdisk batch (its binned deposits keep one
  * record per cloud): at least the largest total cloud count of the sets
  * evaluated next.  Growing it frees the work buffers (re-allocated by the
- * next evaluation) and the cached graphs. */
+ * next evaluation) and the cached graphs.  A set whose total cloud count
+ * exceeds the capacity deposits only the first `capacity` clouds, and the
+ * evaluation raises the batch's overflow flag (gbkhip_batch_check). */
 int gbkhip_batch_reserve_clouds(gbkhip_batch* batch, long long clouds);
+/* Waits for the batch's stream, then returns in *overflow whether an
+ * evaluation since the last check had a set with more clouds than the
+ * reserved capacity (its results are then not valid), and clears the flag. */
+int gbkhip_batch_check(gbkhip_batch* batch, int* overflow);
 /* Fixed-point scale 2^S of the int64 cube of an mcdisk evaluation of nclouds
  * clouds of flux cflux on grid: the value gbkhip_mcdisk_evaluate uses. */
 int gbkhip_mcdisk_fixed_scale(float cflux, int nclouds,
