@@ -85,6 +85,10 @@ _SIGS = {
     'gbkhip_mcdisk_evaluate': ([_vp, _c.c_float, _c.c_int, _vp, _c.c_int, _vp,
                                 _c.POINTER(Disk), _c.POINTER(Grid),
                                 _c.POINTER(Outputs), _c.c_uint64], _c.c_int),
+    'gbkhip_mcdisk_evaluate_noalloc': ([_vp, _c.c_float, _c.c_int, _vp,
+                                        _c.c_int, _vp, _c.POINTER(Disk),
+                                        _c.POINTER(Grid), _c.POINTER(Outputs),
+                                        _c.c_uint64], _c.c_int),
     'gbkhip_wcube_evaluate': ([_vp, _ip, _c.c_int, _vp, _vp], _c.c_int),
     'gbkhip_convolve': ([_vp, _ip, _ip, _vp, _vp, _vp], _c.c_int),
     'gbkhip_residual': ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _c.c_longlong,

@@ -115,6 +115,9 @@ int gbkhip_ctx_create(int device, gbkhip_ctx** out)
         GBK_CHECK(device >= 0 && device < n, GBKHIP_ERR_ARG,
                   "device index out of range");
         GBK_HIP(hipSetDevice(device));
+        // the fixed scratch's address on this device (a failure is reported
+        // by the entry point that uses it, not here)
+        noalloc_scratch_setup(device);
         auto* c = new gbkhip_ctx();
         c->impl.device = device;
         c->impl.stream = nullptr;
@@ -351,6 +354,34 @@ int gbkhip_mcdisk_evaluate(gbkhip_ctx* ctx, float cflux, int nclouds,
         std::lock_guard<std::mutex> lk(c->mu);
         launch_mcdisk(*c, cflux, nclouds, ncloudscsum, ncloudscsum_len,
                       hasordint, normalise(*disk), *grid, *out, seed);
+    });
+}
+
+int gbkhip_mcdisk_evaluate_noalloc(gbkhip_ctx* ctx, float cflux,
+                                   int nclouds, const int* ncloudscsum,
+                                   int ncloudscsum_len,
+                                   const unsigned char* hasordint,
+                                   const gbkhip_disk* disk,
+                                   const gbkhip_grid* grid,
+                                   const gbkhip_outputs* out, uint64_t seed)
+{
+    return guarded([&] {
+        Context* c = ctx_of(ctx);
+        check_disk(disk, grid);
+        check_ptr(out, "outputs");
+        GBK_CHECK(nclouds >= 0, GBKHIP_ERR_ARG, "negative cloud count");
+        if (nclouds == 0) return;
+        check_ptr(ncloudscsum, "ncloudscsum");
+        check_ptr(hasordint, "hasordint");
+        GBK_CHECK(ncloudscsum_len > 0, GBKHIP_ERR_ARG,
+                  "empty cloud cumulative sum");
+        GBK_CHECK(disk->rp.n > 0 && disk->rh.n >= disk->rp.n, GBKHIP_ERR_ARG,
+                  "mcdisk needs one density height trait per polar trait");
+        char* fixed = noalloc_scratch(c->device);
+        std::lock_guard<std::mutex> lk(c->mu);
+        launch_mcdisk(*c, cflux, nclouds, ncloudscsum, ncloudscsum_len,
+                      hasordint, normalise(*disk), *grid, *out, seed, fixed,
+                      kNoAllocScratchBytes);
     });
 }
 

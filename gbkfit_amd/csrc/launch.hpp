@@ -10,11 +10,19 @@ struct FFTPlan;
 // smdisk.hip
 void launch_smdisk(Context& c, const gbkhip_disk& d, const gbkhip_grid& g,
                    const gbkhip_outputs& o);
-// mcdisk.hip
+// mcdisk.hip.  With fixed_scratch (fixed_bytes long) the binned deposit
+// buffers live there and nothing is allocated; calls that need more, or the
+// non-binned deposit, are rejected before anything is enqueued.
 void launch_mcdisk(Context& c, float cflux, int nclouds, const int* csum,
                    int csum_len, const unsigned char* hoi,
                    const gbkhip_disk& d, const gbkhip_grid& g,
-                   const gbkhip_outputs& o, uint64_t seed);
+                   const gbkhip_outputs& o, uint64_t seed,
+                   char* fixed_scratch = nullptr, size_t fixed_bytes = 0);
+// size of the fixed scratch of gbkhip_mcdisk_evaluate_noalloc (64 MiB)
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+// its device address on `device` (the current device), looked up once
+void noalloc_scratch_setup(int device);
+char* noalloc_scratch(int device);
 double mcdisk_fixed_scale(float cflux, int nclouds, const gbkhip_grid& g);
 // dmodel.hip
 void launch_downscale(Context& c, const int* scale, const int* offset,

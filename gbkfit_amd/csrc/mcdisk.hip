@@ -6,6 +6,7 @@
 // thread count; this stream depends only on (seed, cloud id).  The int64
 // cube is converted and added to the FP32 cube afterwards: integer addition
 // is associative, so reruns are bit-identical.
+#include "launch.hpp"
 #include "mcdisk.hpp"
 
 namespace gbkhip {
@@ -401,10 +402,53 @@ double mcdisk_fixed_scale(float cflux, int nclouds, const gbkhip_grid& g)
     return std::ldexp(1.0, std::min(e, 900));
 }
 
+// Fixed scratch of gbkhip_mcdisk_evaluate_noalloc: the binned deposit
+// buffers (records, ranks, sorted records, tile offsets, split-tile
+// accumulators) of one evaluation, for callers that cannot allocate.  Every
+// device holds its own copy until the process ends, at the address
+// hipGetSymbolAddress returns on that device; host code uses that address,
+// never the symbol.  Calls on one device share it.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::vector<char*> g_scratch_ptr;          // per device
+static std::vector<std::string> g_scratch_err;    // per device, on failure
+
+void noalloc_scratch_setup(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) {
+        g_scratch_ptr.resize(device + 1, nullptr);
+        g_scratch_err.resize(device + 1);
+    }
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch): ") +
+                                hipGetErrorString(e);
+        return;
+    }
+    g_scratch_ptr[device] = static_cast<char*>(p);
+}
+
+char* noalloc_scratch(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    GBK_CHECK(device >= 0 && device < (int)g_scratch_ptr.size(),
+              GBKHIP_ERR_ARG, "no context was created on this device");
+    GBK_CHECK(g_scratch_ptr[device] != nullptr, GBKHIP_ERR_HIP,
+              g_scratch_err[device].empty()
+                  ? std::string("fixed scratch not set up on this device")
+                  : g_scratch_err[device]);
+    return g_scratch_ptr[device];
+}
+
 void launch_mcdisk(Context& c, float cflux, int nclouds, const int* csum,
                    int csum_len, const unsigned char* hoi,
                    const gbkhip_disk& d, const gbkhip_grid& g,
-                   const gbkhip_outputs& o, uint64_t seed)
+                   const gbkhip_outputs& o, uint64_t seed,
+                   char* fixed_scratch, size_t fixed_bytes)
 {
     if (nclouds <= 0) return;
     GBK_CHECK(d.rp.uids && d.rh.uids, GBKHIP_ERR_ARG,
@@ -419,6 +463,22 @@ void launch_mcdisk(Context& c, float cflux, int nclouds, const int* csum,
         !o.wdata_cmp && !o.rdata && !o.rdata_cmp && !o.ordata &&
         !o.ordata_cmp && !o.vdata_cmp && !o.ddata_cmp;
     const bool binned = only_scube && ncube > 0 && tw > 0;
+    if (fixed_scratch) {
+        // checked before anything is enqueued: a rejected call does nothing
+        GBK_CHECK(binned, GBKHIP_ERR_UNSUPPORTED,
+                  "the no-alloc mcdisk evaluation needs the binned deposit: "
+                  "the spectral cube as the only output and at most 8192 "
+                  "channels");
+        const int ntiles = ceil_div(g.spat_size[0], tw) *
+                           ceil_div(g.spat_size[1], tw);
+        const size_t need = mc_bufs(nullptr, 1, nclouds, ntiles, tw, nz,
+                                    nullptr);
+        GBK_CHECK(need <= fixed_bytes, GBKHIP_ERR_ARG,
+                  "the no-alloc mcdisk evaluation needs " +
+                  std::to_string(need) + " bytes of scratch for " +
+                  std::to_string(nclouds) + " clouds, beyond its fixed " +
+                  std::to_string(fixed_bytes) + " bytes");
+    }
     unsigned long long* icube = nullptr;
     if (o.scube && ncube > 0 && !binned) {
         const size_t bytes = (size_t)ncube * sizeof(unsigned long long);
@@ -446,11 +506,15 @@ void launch_mcdisk(Context& c, float cflux, int nclouds, const int* csum,
         const int ntx = ceil_div(nx, tw), nty = ceil_div(ny, tw);
         const int ntiles = ntx * nty;
         const long long nc = nclouds;
-        const size_t need = mc_bufs(nullptr, 1, nc, ntiles, tw, nz, nullptr);
-        if (c.mc_scratch.bytes < need) c.mc_scratch.alloc(need);
+        char* base = fixed_scratch;
+        if (!base) {
+            const size_t need = mc_bufs(nullptr, 1, nc, ntiles, tw, nz,
+                                        nullptr);
+            if (c.mc_scratch.bytes < need) c.mc_scratch.alloc(need);
+            base = static_cast<char*>(c.mc_scratch.ptr);
+        }
         McBufs bb;
-        mc_bufs(static_cast<char*>(c.mc_scratch.ptr), 1, nc, ntiles, tw, nz,
-                &bb);
+        mc_bufs(base, 1, nc, ntiles, tw, nz, &bb);
         McBin bin;
         bin.rec = bb.rec;
         bin.tr = bb.tr;

@@ -157,6 +157,23 @@ int gbkhip_mcdisk_evaluate(gbkhip_ctx* ctx, float cflux, int nclouds,
                            const unsigned char* hasordint,
                            const gbkhip_disk* disk, const gbkhip_grid* grid,
                            const gbkhip_outputs* out, uint64_t seed);
+/* gbkhip_mcdisk_evaluate that allocates nothing.  It has no counterpart in
+ * the reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results, bit for bit; the binned deposit
+ * buffers live in a fixed 64 MiB device array that each GPU holds until the
+ * process ends (about 1.4 M clouds on a 256x256x128 cube).  Requests that
+ * need more scratch, or outputs other than the spectral cube, or more than
+ * 8192 channels, are rejected (GBKHIP_ERR_ARG / GBKHIP_ERR_UNSUPPORTED)
+ * before anything is enqueued.  Calls that overlap on one GPU (from several
+ * contexts, streams or threads) share the array and race. */
+int gbkhip_mcdisk_evaluate_noalloc(gbkhip_ctx* ctx, float cflux, int nclouds,
+                                   const int* ncloudscsum,
+                                   int ncloudscsum_len,
+                                   const unsigned char* hasordint,
+                                   const gbkhip_disk* disk,
+                                   const gbkhip_grid* grid,
+                                   const gbkhip_outputs* out, uint64_t seed);
 /* Spectral weight cube from a spatial weight cube (gmodels.hpp:52-82). */
 int gbkhip_wcube_evaluate(gbkhip_ctx* ctx, const int spat_size[3],
                           int spec_size, const float* spat_data,
