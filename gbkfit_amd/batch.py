@@ -63,6 +63,8 @@ N.register_signatures({
     'gbkhip_batch_check': ([_c.c_void_p, _c.POINTER(_c.c_int)], _c.c_int),
     'gbkhip_batch_tile_stats': ([_c.c_void_p, _c.POINTER(_c.c_longlong)],
                                 _c.c_int),
+    'gbkhip_batch_line_stats': ([_c.c_void_p, _c.POINTER(_c.c_longlong)],
+                                _c.c_int),
     'gbkhip_group_create': ([_c.c_void_p, _c.POINTER(_c.c_void_p), _c.c_int,
                              _c.POINTER(_c.c_void_p)], _c.c_int),
     'gbkhip_group_destroy': ([_c.c_void_p], _c.c_int),
@@ -428,6 +430,14 @@ class BatchEvaluator:
         v = (_c.c_longlong * 6)()
         N.call('gbkhip_batch_tile_stats', self.handle, v)
         return (v[0], v[1]), (v[2], v[3]), (v[4], v[5])
+
+    def line_stats(self):
+        """Direct pipeline of a thin disk, last chunk on the first lane:
+        (live, total) hi-res spaxels in the 64-spaxel row segments some line
+        reaches (the only line parameters written and read)."""
+        v = (_c.c_longlong * 2)()
+        N.call('gbkhip_batch_line_stats', self.handle, v)
+        return v[0], v[1]
 
     def set_timing(self, enable):
         N.call('gbkhip_batch_set_timing', self.handle, int(bool(enable)))

@@ -18,6 +18,7 @@
 #include "launch.hpp"
 #include "disk.hpp"
 #include "mcdisk.hpp"
+#include "thick.hpp"
 #include "sepfilter.hpp"
 
 #include <climits>
@@ -90,57 +91,26 @@ __global__ __launch_bounds__(256) void batch_smdisk_thin(
         col[k * plane] = (k >= k0 && k <= k1) ? e.at(k) : 0.0f;
 }
 
-// Thick disk: 4 waves split the line of sight of 64 columns, spectrum tile
-// in LDS (chunked over channels), tile written in full.  Only columns
-// x < nxe and channels k < nze are evaluated and written, on the rows the
+// Thick disk: 4 waves split the line of sight of 64 columns, every sum in
+// a fixed order (thick.hpp).  Only columns x < nxe and channels k < nze
+// are evaluated and written (zero where no line lands), on the rows the
 // grid launches (the direct pipeline reads nothing beyond its inputs; the
 // padded-cube pipeline passes the whole grid).
+template <int J>
 __global__ __launch_bounds__(256) void batch_smdisk_thick(
     gbkhip_disk tmpl, SetLayout lay, const float* __restrict__ params,
     int pstride, gbkhip_grid g, float* __restrict__ cubes, long long cstride,
-    int chunk, int nxe, int nze)
+    int nxe, int nze)
 {
-    extern __shared__ float tile[];
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int x = blockIdx.x * 64 + lane;
-    const int y = blockIdx.y;
+    __shared__ ThickLds S;
     const int set = blockIdx.z;
-    const int nx = g.spat_size[0], ny = g.spat_size[1], nz = g.spat_size[2];
-    const size_t plane = (size_t)nx * ny;
     const gbkhip_disk d = set_disk(tmpl, lay, params + (size_t)set * pstride);
     const bool need_theta = disk_needs_theta(d);
     const NodeGeom ng = node_geom(d, 0);
-    float* cube = cubes + set * cstride;
-    for (int c0 = 0; c0 < nze; c0 += chunk) {
-        const int c1 = min(c0 + chunk, nze);
-        for (int i = threadIdx.x; i < chunk * 64; i += blockDim.x)
-            tile[i] = 0.0f;
-        __syncthreads();
-        if (x < nxe) {
-            for (int z = wave; z < nz; z += 4) {
-                const VoxelLine L = smdisk_voxel(d, g, ng, x, y, z,
-                                                 need_theta, false);
-                if (!L.on) continue;
-                int k0, k1;
-                line_window(g, L.v, L.s, k0, k1);
-                k0 = max(k0, c0);
-                k1 = min(k1, c1 - 1);
-                if (k0 > k1) continue;
-                const LineEval e = line_eval(g, L.orvalue, L.v, L.s);
-                for (int k = k0; k <= k1; ++k)
-                    atomicAdd(&tile[(k - c0) * 64 + lane], e.at(k));
-            }
-        }
-        __syncthreads();
-        for (int i = threadIdx.x; i < (c1 - c0) * 64; i += blockDim.x) {
-            const int k = c0 + i / 64, l = i % 64;
-            const int xx = blockIdx.x * 64 + l;
-            if (xx < nxe)
-                cube[k * plane + (size_t)y * nx + xx] = tile[i];
-        }
-        __syncthreads();
-    }
+    gbkhip_outputs o;
+    memset(&o, 0, sizeof(o));
+    thick_row_segment<J, false>(d, g, ng, need_theta, blockIdx.y, nxe, nze,
+                                cubes + set * cstride, true, o, S);
 }
 
 struct DataArgs {
@@ -1841,12 +1811,20 @@ static void deposit_hi(Batch& b, const float* P, int set0, int nsets,
         const int nxe = part ? b.sep.need[0] : g.spat_size[0];
         const int nye = part ? b.sep.need[1] : g.spat_size[1];
         const int nze = part ? b.sep.need[2] : g.spec_size;
-        const int chunk = std::max(1, std::min(nze, 240));
         dim3 grid(ceil_div(nxe, 64), nye, nsets);
-        hipLaunchKernelGGL(batch_smdisk_thick, grid, dim3(256),
-                           (size_t)chunk * 64 * sizeof(float), c.stream,
-                           b.tmpl, b.lay, P, D.pstride, g, hi, b.n_hi, chunk,
-                           nxe, nze);
+        const int nz = g.spat_size[2];
+        if (nz <= 16)
+            hipLaunchKernelGGL(batch_smdisk_thick<4>, grid, dim3(256), 0,
+                               c.stream, b.tmpl, b.lay, P, D.pstride, g, hi,
+                               b.n_hi, nxe, nze);
+        else if (nz <= 32)
+            hipLaunchKernelGGL(batch_smdisk_thick<8>, grid, dim3(256), 0,
+                               c.stream, b.tmpl, b.lay, P, D.pstride, g, hi,
+                               b.n_hi, nxe, nze);
+        else
+            hipLaunchKernelGGL(batch_smdisk_thick<16>, grid, dim3(256), 0,
+                               c.stream, b.tmpl, b.lay, P, D.pstride, g, hi,
+                               b.n_hi, nxe, nze);
     }
     GBK_HIP(hipGetLastError());
 }
@@ -2130,6 +2108,10 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
                       D.scale[i] <= D.size_hi[i], GBKHIP_ERR_ARG,
                       "downscale window exceeds the padded cube");
         }
+        GBK_CHECK(D.mcdisk || D.traits[1].n == 0 ||
+                  D.grid.spec_size <= 65535,
+                  GBKHIP_ERR_UNSUPPORTED,
+                  "thick disks: at most 65535 channels");
         GBK_CHECK(D.grid.spat_size[0] == D.size_hi[0] &&
                   D.grid.spat_size[1] == D.size_hi[1] &&
                   D.grid.spec_size == D.size_hi[2], GBKHIP_ERR_ARG,
@@ -2197,11 +2179,6 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
                 // cube per lane costs memory, so those start with one
                 B.lanes = B.sep.cube ? 1 : 2;
                 if (B.lanes == 2) ensure_lane_streams(B);
-                if (!B.thin && !D.mcdisk)
-                    (void)hipFuncSetAttribute(
-                        (const void*)batch_smdisk_thick,
-                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                        96 * 1024);
                 alloc_work(B);
                 compute_zero_partials(B);
                 *out = b;
@@ -2234,11 +2211,6 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
             }
             alloc_work(B);
             fft_prepare_attributes();
-            if (!B.thin && !D.mcdisk) {
-                (void)hipFuncSetAttribute(
-                    (const void*)batch_smdisk_thick,
-                    hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
-            }
             GBK_HIP(hipStreamSynchronize(c->stream));
         } catch (...) {
             delete b;
@@ -2624,6 +2596,31 @@ int gbkhip_batch_tile_stats(gbkhip_batch* b, long long* counts)
                     counts[4] += live;
                     counts[5] += 1;
                 }
+    });
+}
+
+int gbkhip_batch_line_stats(gbkhip_batch* b, long long* counts)
+{
+    return guarded([&] {
+        GBK_CHECK(b && counts, GBKHIP_ERR_ARG, "null argument");
+        Batch& B = b->impl;
+        counts[0] = counts[1] = 0;
+        if (!B.sep.on || B.sep.cube || !B.work || B.sep.last_sets == 0)
+            return;
+        GBK_HIP(hipSetDevice(B.device));
+        const hipStream_t s = B.stream ? B.stream : B.ctx->stream;
+        GBK_HIP(hipStreamSynchronize(s));
+        const int nx = B.sep.need[0];
+        const size_t rows = (size_t)B.sep.last_sets * B.sep.need[1];
+        std::vector<unsigned long long> h(rows);
+        GBK_HIP(hipMemcpy(h.data(), B.sep.segmask.ptr,
+                          rows * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        const int nseg = ceil_div(nx, 64);
+        for (unsigned long long m : h)
+            for (int sg = 0; sg < nseg; ++sg)
+                if (sg >= 63 || ((m >> sg) & 1ull))
+                    counts[0] += std::min(64, nx - 64 * sg);
+        counts[1] = (long long)rows * nx;
     });
 }
 

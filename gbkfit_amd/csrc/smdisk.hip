@@ -7,11 +7,13 @@
 //     spaxel; the wave walks the union of its lanes' channel windows so the
 //     read-modify-write of each channel plane is one coalesced 256-byte row.
 //   * thick disks: a 256-thread block owns a row segment of 64 columns; its
-//     four waves split the line of sight, add their lines into a
-//     (channels x 64) spectrum tile in LDS, and the block then adds the tile
-//     to the cube with coalesced rows.  No global atomics on the cube.
+//     four waves split the line of sight, add their lines into per-wave
+//     LDS partial spectra (each lane owns its column) and the block adds
+//     their fixed-order sum to the cube with coalesced rows (thick.hpp).
+//     No atomics: results are bit-identical from run to run.
 #include "common.hpp"
 #include "disk.hpp"
+#include "thick.hpp"
 
 #include <climits>
 
@@ -79,79 +81,19 @@ __global__ __launch_bounds__(256) void smdisk_thin_kernel(
         if (k >= k0 && k <= k1) o.scube[k * plane + col] += e.at(k);
 }
 
-// Thick disk: block of 4 waves on the row segment [x0, x0 + 64) of row y;
-// wave w evaluates z = w, w + 4, ...  Channels are processed in chunks of
-// `chunk` (the LDS tile height) when the spectrum is taller than the tile.
+// Thick disk: block of 4 waves on the row segment [x0, x0 + 64) of row y,
+// the line of sight split over the waves, every sum in a fixed order
+// (thick.hpp); lines added to the cube (+=).
+template <int J>
 __global__ __launch_bounds__(256) void smdisk_thick_kernel(
-    gbkhip_disk d, gbkhip_grid g, gbkhip_outputs o, int chunk)
+    gbkhip_disk d, gbkhip_grid g, gbkhip_outputs o)
 {
-    extern __shared__ float tile[];        // [chunk][64]
-    __shared__ int s_kmin, s_kmax;
-    __shared__ float s_img[kWave];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
-    const int x = blockIdx.x * kWave + lane;
-    const int y = blockIdx.y;
-    const int nx = g.spat_size[0], ny = g.spat_size[1], nz = g.spat_size[2];
-    const size_t plane = (size_t)nx * ny;
+    __shared__ ThickLds S;
     const bool need_theta = disk_needs_theta(d);
     const NodeGeom ng = node_geom(d, 0);
-    const int nspec = g.spec_size;
-
-    for (int c0 = 0; c0 < (o.scube ? nspec : 1); c0 += chunk) {
-        const bool first = c0 == 0;
-        const int c1 = min(c0 + chunk, nspec);   // exclusive
-        for (int i = threadIdx.x; i < chunk * kWave; i += blockDim.x)
-            tile[i] = 0.0f;
-        if (threadIdx.x == 0) { s_kmin = INT_MAX; s_kmax = INT_MIN; }
-        if (threadIdx.x < kWave) s_img[threadIdx.x] = 0.0f;
-        __syncthreads();
-        int kmin = INT_MAX, kmax = INT_MIN;
-        if (x < nx) {
-            for (int z = wave; z < nz; z += 4) {
-                const VoxelLine L = smdisk_voxel(d, g, ng, x, y, z,
-                                                 need_theta, o.wdata);
-                if (!L.on) continue;
-                if (first) {
-                    if (o.image) atomicAdd(&s_img[lane], L.orvalue);
-                    side_outputs(o, (size_t)z * plane + (size_t)y * nx + x, L);
-                }
-                if (!o.scube) continue;
-                int k0, k1;
-                line_window(g, L.v, L.s, k0, k1);
-                k0 = max(k0, c0);
-                k1 = min(k1, c1 - 1);
-                if (k0 > k1) continue;
-                const LineEval e = line_eval(g, L.orvalue, L.v, L.s);
-                for (int k = k0; k <= k1; ++k)
-                    atomicAdd(&tile[(k - c0) * kWave + lane], e.at(k));
-                kmin = min(kmin, k0);
-                kmax = max(kmax, k1);
-            }
-        }
-        kmin = wave_min(kmin);
-        kmax = wave_max(kmax);
-        if (lane == 0 && kmin <= kmax) {
-            atomicMin(&s_kmin, kmin);
-            atomicMax(&s_kmax, kmax);
-        }
-        __syncthreads();
-        if (first && o.image && threadIdx.x < kWave && x < nx &&
-            s_img[lane] != 0.0f)
-            o.image[(size_t)y * nx + x] += s_img[lane];
-        if (o.scube && s_kmin <= s_kmax) {
-            const int rows = s_kmax - s_kmin + 1;
-            for (int i = threadIdx.x; i < rows * kWave; i += blockDim.x) {
-                const int k = s_kmin + i / kWave;
-                const int l = i % kWave;
-                const int xx = blockIdx.x * kWave + l;
-                if (xx < nx)
-                    o.scube[k * plane + (size_t)y * nx + xx] +=
-                        tile[(k - c0) * kWave + l];
-            }
-        }
-        __syncthreads();
-    }
+    thick_row_segment<J, true>(d, g, ng, need_theta, blockIdx.y,
+                               g.spat_size[0], g.spec_size, o.scube, false,
+                               o, S);
 }
 
 void launch_smdisk(Context& c, const gbkhip_disk& d, const gbkhip_grid& g,
@@ -166,18 +108,19 @@ void launch_smdisk(Context& c, const gbkhip_disk& d, const gbkhip_grid& g,
         hipLaunchKernelGGL(smdisk_thin_kernel, grid, block, 0, c.stream, d, g,
                            o);
     } else {
-        static std::once_flag once;
-        std::call_once(once, [] {
-            (void)hipFuncSetAttribute(
-                (const void*)smdisk_thick_kernel,
-                hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
-        });
-        const int chunk = o.scube ? std::max(1, std::min(g.spec_size, 256))
-                                  : 1;
-        const size_t lds = (size_t)chunk * kWave * sizeof(float);
+        GBK_CHECK(g.spec_size <= 65535, GBKHIP_ERR_UNSUPPORTED,
+                  "thick smdisk: at most 65535 channels");
         dim3 grid(ceil_div(nx, kWave), ny);
-        hipLaunchKernelGGL(smdisk_thick_kernel, grid, dim3(256), lds,
-                           c.stream, d, g, o, chunk);
+        const int nz = g.spat_size[2];
+        if (nz <= 16)
+            hipLaunchKernelGGL(smdisk_thick_kernel<4>, grid, dim3(256), 0,
+                               c.stream, d, g, o);
+        else if (nz <= 32)
+            hipLaunchKernelGGL(smdisk_thick_kernel<8>, grid, dim3(256), 0,
+                               c.stream, d, g, o);
+        else
+            hipLaunchKernelGGL(smdisk_thick_kernel<16>, grid, dim3(256), 0,
+                               c.stream, d, g, o);
     }
     GBK_HIP(hipGetLastError());
 }

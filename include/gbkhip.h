@@ -326,6 +326,12 @@ int gbkhip_group_evaluate(gbkhip_group* group, const float* const* params,
  * z-pass block takes its precomputed data-only partial sum).  Waits for
  * the batch's stream.  Zeros for other pipelines. */
 int gbkhip_batch_tile_stats(gbkhip_batch* batch, long long* counts);
+/* Direct pipeline from line parameters (thin disks), last chunk evaluated
+ * on the first lane: counts[0] of counts[1] hi-res spaxels lie in 64-spaxel
+ * row segments that some line reaches, the only line parameters
+ * batch_lines_thin writes and the x pass reads.  Waits for the batch's
+ * stream.  Zeros for other pipelines. */
+int gbkhip_batch_line_stats(gbkhip_batch* batch, long long* counts);
 /* Stage timing: with timing on, evaluate() launches directly (no graph) and
  * records events around the smdisk, convolution and downscale+chi^2 stages
  * of every chunk; stage_times() waits for them and returns the summed stage

@@ -171,8 +171,8 @@ def test_direct_pipeline_other_shapes(scale, size, psf, lsf):
 def test_thick_disk_batch_matches_plain_path(path):
     """A thick disk runs the direct pipeline from its deposited hi-res cube
     (automatic choice) or the padded-cube one; chi^2 equals the plain path,
-    and two lanes of chunks give the chi^2 of one (within 1e-7: the thick
-    deposit adds into LDS with float atomics, so its sums vary with order)."""
+    and two lanes of chunks give the chi^2 of one, bit for bit (the thick
+    deposit sums every line of sight in a fixed order), as do reruns."""
     from gbkfit_amd.batch import BatchEvaluator
     from gbkfit_amd.objective import Data, Objective
     drv = H.driver()
@@ -194,10 +194,12 @@ def test_thick_disk_batch_matches_plain_path(path):
     for i, s in enumerate(sets):
         plain = -2 * obj.log_likelihood(s)[0]
         assert abs(plain - chi2[i]) <= 1e-5 * chi2[i], (i, plain, chi2[i])
+    again = ev.evaluate(sets, use_graph=True)
+    assert np.array_equal(again, chi2)
     if want == 'direct':
         ev.set_tiles(2, 2)
         two = ev.evaluate(sets, use_graph=True)
-        assert np.allclose(two, chi2, rtol=1e-7, atol=0), (two, chi2)
+        assert np.array_equal(two, chi2), (two, chi2)
     ev.close()
 
 
