@@ -700,6 +700,9 @@ constexpr int kSepJZ = 16;          // outputs per thread, S3
 constexpr int kSepXW = 32;          // X columns per S2 block
 constexpr int kSepYPitch = kSepXW + 1;
 constexpr int kSepMaxRows = (48 * 1024) / (kSepYPitch * 4);   // S2 LDS rows
+// channels of one x-pass chunk an S2 block filters in turn (the chunk's
+// zero rows and live groups are found once per block)
+constexpr int kSepYChannels = 16;
 // lines per thread an S1 row block keeps in registers (wider rows read the
 // rest again per chunk)
 constexpr int kSepXLines = 2;
@@ -830,124 +833,169 @@ __global__ __launch_bounds__(256) void sep_xpass_cube(
 }
 
 // S2: P2[z][Y][X] = sum_v Wy[v] P1[z][sy Y + v][X].  Block = (32 X
-// columns, Y outputs [Y0, Y0 + yblk), channel, set): the rows they read are
-// staged once in LDS (pitch 33); a thread filters J consecutive Y of one
-// column.
+// columns, Y outputs [Y0, Y0 + yblk), one x-pass channel chunk, set), in
+// groups of J consecutive Y.  The x pass leaves a row of a chunk unwritten
+// when no line reaches it, so every channel of the chunk has the same zero
+// rows: a group whose input rows are all zero is zero for the whole chunk,
+// neither computed nor written, and its flag tells the z pass to read
+// zeros.  The block lists the chunk's live groups once, then per channel
+// stages only the rows they read in LDS (pitch 33) and hands each live
+// group to one wave, in list order: lanes 0-31 filter the group's first
+// J / 2 outputs of their column, lanes 32-63 the last J / 2, so no lane of a
+// working wave idles on a dead group.
 template <int SY, int TY>
-__global__ __launch_bounds__(256) void sep_ypass(
-    const float* __restrict__ P1, long long p1stride, const float* wy,
-    int nxl, int nyl, int ny1, int need_z, int yblk,
-    float* __restrict__ P2, long long p2stride,
-    const unsigned long long* __restrict__ rowmask, int zc,
+__global__ __launch_bounds__(256, 4) void sep_ypass(
+    const float* __restrict__ P1, long long p1stride,
+    const float* __restrict__ wy, int nxl, int nyl, int ny1, int need_z,
+    int yblk, float* __restrict__ P2, long long p2stride,
+    const unsigned long long* __restrict__ rowmask, int zc, int zpb,
     unsigned char* __restrict__ gflag, int ngy)
 {
     constexpr int T = kSepThreads, J = sep_jy(TY), XW = kSepXW;
-    constexpr int PITCH = kSepYPitch, G = T / XW;
+    constexpr int PITCH = kSepYPitch;
     constexpr int NIN = SY * (J - 1) + TY;
+    static_assert(2 * XW == 64 && J % 2 == 0, "a wave is two column halves");
     extern __shared__ float tile[];       // [rows][PITCH]
-    __shared__ float s_w[TY];
+    __shared__ unsigned char s_flag[kSepMaxRows];
+    // one word per 64 rows, written by every wave of each 256-row pass
+    __shared__ unsigned long long s_bits[(kSepMaxRows + 255) / 256 * 4];
+    __shared__ unsigned char s_gl[T];     // live groups, in order
+    __shared__ int s_wcnt[T / 64];
     const int X0 = blockIdx.x * XW, Y0 = blockIdx.y * yblk;
-    const int z = blockIdx.z % need_z, set = blockIdx.z / need_z;
+    // blockIdx.z = (set, chunk, sub-block of zpb channels of the chunk)
+    const int nsub = (zc + zpb - 1) / zpb;
+    const int nzb = (need_z + zc - 1) / zc * nsub;
+    const int zb = blockIdx.z % nzb, set = blockIdx.z / nzb;
+    const int chunk = zb / nsub;
+    const int z0 = chunk * zc + (zb % nsub) * zpb;
+    const int z1 = min(min(z0 + zpb, chunk * zc + zc), need_z);
     const int ngrp = (min(yblk, nyl - Y0) + J - 1) / J;
     const int nrows = SY * (ngrp * J - 1) + TY;
-    __shared__ unsigned char s_flag[kSepMaxRows];
-    for (int i = threadIdx.x; i < TY; i += T) s_w[i] = wy[i];
-    const float* src = P1 + set * p1stride + (size_t)z * ny1 * nxl;
-    // rows whose x-pass tile no line reaches are zero (never written)
+    // rows whose x-pass tile no line reaches are zero (never written): a
+    // byte per row for the staging and a bit per row for the group test
     const unsigned long long* rm = rowmask + (size_t)set * ny1;
-    const int chunk = z / zc;
-    for (int r = threadIdx.x; r < nrows; r += T) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int r0 = 0; r0 < nrows; r0 += T) {          // uniform trip count
+        const int r = r0 + threadIdx.x;
         const int y = SY * Y0 + r;
-        s_flag[r] = y < ny1 ? (unsigned char)((rm[y] >> chunk) & 1ull) : 0;
+        const bool f = r < nrows && y < ny1 && ((rm[y] >> chunk) & 1ull);
+        if (r < nrows) s_flag[r] = f;
+        const unsigned long long b = __ballot(f);
+        if (lane == 0) s_bits[(r0 >> 6) + wv] = b;
     }
     __syncthreads();
-    // an output group whose input rows are all zero is zero: not computed
-    // and not written; its flag tells the z pass to read zeros instead
-    __shared__ unsigned char s_glive[kSepMaxRows];
-    int any = 0;
-    for (int gi = threadIdx.x; gi < ngrp; gi += T) {
-        unsigned char live = 0;
-        for (int i = 0; i < NIN; ++i) live |= s_flag[SY * J * gi + i];
-        s_glive[gi] = live;
-        any |= live;
-        gflag[((size_t)set * need_z + z) * ngy + Y0 / J + gi] = live;
+    // group gi reads rows [SY J gi, SY J gi + NIN): live if any is live
+    const int gi = threadIdx.x;
+    bool live = false;
+    if (gi < ngrp) {
+        const int a = SY * J * gi, e = a + NIN;
+        for (int w = a >> 6; w <= (e - 1) >> 6; ++w) {
+            const int lo = max(a, w << 6) - (w << 6);
+            const int hi = min(e, (w << 6) + 64) - (w << 6);
+            const unsigned long long m =
+                (hi - lo == 64 ? ~0ull : ((1ull << (hi - lo)) - 1ull)) << lo;
+            live |= (s_bits[w] & m) != 0ull;
+        }
+        for (int z = z0; z < z1; ++z)
+            gflag[((size_t)set * need_z + z) * ngy + Y0 / J + gi] = live;
     }
-    if (!__syncthreads_or(any)) return;      // every group of the block is zero
-    // staging: U independent loads in flight per thread before the stores
-    constexpr int U = 8;
-    if ((nxl & 3) == 0 && X0 + XW <= nxl) {
-        // whole strip, rows 16-byte aligned: float4 loads
-        constexpr int Q = XW / 4;
-        const int n4 = nrows * Q;
-        for (int i0 = threadIdx.x; i0 < n4; i0 += T * U) {
-            float4 t[U];
+    const unsigned long long gm = __ballot(live);
+    if (lane == 0) s_wcnt[wv] = __popcll(gm);
+    __syncthreads();
+    int base = 0, nl = 0;
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int i = i0 + u * T, r = i / Q, q = i - r * Q;
-                const int y = SY * Y0 + r;
-                t[u] = (i < n4 && s_flag[r])
-                           ? *reinterpret_cast<const float4*>(
-                                 src + (size_t)y * nxl + X0 + 4 * q)
-                           : make_float4(0, 0, 0, 0);
+    for (int w = 0; w < T / 64; ++w) {
+        base += w < wv ? s_wcnt[w] : 0;
+        nl += s_wcnt[w];
+    }
+    if (nl == 0) return;                 // every group of the chunk is zero
+    if (live) s_gl[base + __popcll(gm & ((1ull << lane) - 1ull))] = gi;
+    __syncthreads();
+    // the rows the live groups read
+    const int rlo = SY * J * s_gl[0];
+    const int rhi = SY * J * s_gl[nl - 1] + NIN;
+    constexpr int JH = J / 2, NH = SY * (JH - 1) + TY;
+    const int c = lane & (XW - 1), h = lane / XW, X = X0 + c;
+    for (int z = z0; z < z1; ++z) {
+        const float* src = P1 + set * p1stride + (size_t)z * ny1 * nxl;
+        // staging: U independent loads in flight per thread before the
+        // stores
+        constexpr int U = 8;
+        if ((nxl & 3) == 0 && X0 + XW <= nxl) {
+            // whole strip, rows 16-byte aligned: float4 loads
+            constexpr int Q = XW / 4;
+            const int n4 = (rhi - rlo) * Q;
+            for (int i0 = threadIdx.x; i0 < n4; i0 += T * U) {
+                float4 t[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int i = i0 + u * T, r = rlo + i / Q, q = i % Q;
+                    t[u] = (i < n4 && s_flag[r])
+                               ? *reinterpret_cast<const float4*>(
+                                     src + (size_t)(SY * Y0 + r) * nxl + X0 +
+                                     4 * q)
+                               : make_float4(0, 0, 0, 0);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int i = i0 + u * T, r = rlo + i / Q, q = i % Q;
+                    if (i < n4) {
+                        float* d = tile + r * PITCH + 4 * q;
+                        d[0] = t[u].x;
+                        d[1] = t[u].y;
+                        d[2] = t[u].z;
+                        d[3] = t[u].w;
+                    }
+                }
             }
+        } else {
+            const int n1 = (rhi - rlo) * XW;
+            for (int i0 = threadIdx.x; i0 < n1; i0 += T * U) {
+                float t[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int i = i0 + u * T, r = i / Q, q = i - r * Q;
-                if (i < n4) {
-                    float* d = tile + r * PITCH + 4 * q;
-                    d[0] = t[u].x;
-                    d[1] = t[u].y;
-                    d[2] = t[u].z;
-                    d[3] = t[u].w;
+                for (int u = 0; u < U; ++u) {
+                    const int i = i0 + u * T, r = rlo + i / XW, c = i % XW;
+                    const int X = X0 + c;
+                    t[u] = (i < n1 && s_flag[r] && X < nxl)
+                               ? src[(size_t)(SY * Y0 + r) * nxl + X] : 0.0f;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int i = i0 + u * T, r = rlo + i / XW, c = i % XW;
+                    if (i < n1) tile[r * PITCH + c] = t[u];
                 }
             }
         }
-    } else {
-        const int n1 = nrows * XW;
-        for (int i0 = threadIdx.x; i0 < n1; i0 += T * U) {
-            float t[U];
+        __syncthreads();
+        float* dst = P2 + set * p2stride + (size_t)z * nyl * nxl + X;
+        for (int k = wv; k < nl; k += T / 64) {
+            const int g = s_gl[k];
+            const float* in = tile + SY * (J * g + JH * h) * PITCH + c;
+            float v[NH];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int i = i0 + u * T, r = i / XW, c = i - r * XW;
-                const int y = SY * Y0 + r, X = X0 + c;
-                t[u] = (i < n1 && s_flag[r] && X < nxl)
-                           ? src[(size_t)y * nxl + X] : 0.0f;
+            for (int i = 0; i < NH; ++i) v[i] = in[i * PITCH];
+            float o[JH] = {};
+#pragma unroll
+            for (int u = 0; u < TY; ++u) {
+                const float w = wy[u];        // wave-uniform: held in SGPRs
+#pragma unroll
+                for (int j = 0; j < JH; ++j)
+                    o[j] = fmaf(w, v[SY * j + u], o[j]);
             }
+            // full halves (all but the last rows' group) store without a
+            // bounds check per output
+            const int Yb = Y0 + g * J + JH * h;
+            float* d = dst + (size_t)Yb * nxl;
+            if (X < nxl && Yb + JH <= nyl) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int i = i0 + u * T, r = i / XW, c = i - r * XW;
-                if (i < n1) tile[r * PITCH + c] = t[u];
+                for (int j = 0; j < JH; ++j) d[(size_t)j * nxl] = o[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < JH; ++j)
+                    if (X < nxl && Yb + j < nyl) d[(size_t)j * nxl] = o[j];
             }
         }
-    }
-    __syncthreads();
-    const int c = threadIdx.x % XW, X = X0 + c;
-    float* dst = P2 + set * p2stride + (size_t)z * nyl * nxl + X;
-    for (int gi = threadIdx.x / XW; gi < ngrp; gi += G) {
-        if (!s_glive[gi]) continue;
-        const float* in = tile + SY * J * gi * PITCH + c;
-        float v[NIN];
-#pragma unroll
-        for (int i = 0; i < NIN; ++i) v[i] = in[i * PITCH];
-        float o[J] = {};
-#pragma unroll
-        for (int u = 0; u < TY; ++u) {
-            const float w = s_w[u];
-#pragma unroll
-            for (int j = 0; j < J; ++j) o[j] = fmaf(w, v[SY * j + u], o[j]);
-        }
-        // full groups (all but the last rows' group) store without a
-        // bounds check per output
-        const int Yb = Y0 + gi * J;
-        float* d = dst + (size_t)Yb * nxl;
-        if (X < nxl && Yb + J <= nyl) {
-#pragma unroll
-            for (int j = 0; j < J; ++j) d[(size_t)j * nxl] = o[j];
-        } else {
-#pragma unroll
-            for (int j = 0; j < J; ++j)
-                if (X < nxl && Yb + j < nyl) d[(size_t)j * nxl] = o[j];
-        }
+        __syncthreads();                      // the tile is free again
     }
 }
 
@@ -1896,15 +1944,19 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
         for (int s0 = 0; s0 < nsets; s0 += step)
             launch(s0, std::min(step, nsets - s0));
     };
-    slices(nz, [&](int s0, int ns) {
-        dim3 grid(ceil_div(nxl, kSepXW), ceil_div(nyl, P.yblk), nz * ns);
+    // y-pass blocks per set: per x-pass chunk, its channels kSepYChannels
+    // at a time
+    const int zpb = std::min(kSepYChannels, P.zc);
+    const int nzb = P.nchunks * ceil_div(P.zc, zpb);
+    slices(nzb, [&](int s0, int ns) {
+        dim3 grid(ceil_div(nxl, kSepXW), ceil_div(nyl, P.yblk), nzb * ns);
         const int rows = D.scale[1] * (P.yblk - 1) + P.ty;
         const size_t lds = (size_t)rows * kSepYPitch * sizeof(float);
         GBK_SEP_DISPATCH(sep_ypass, D.scale[1], P.ty, grid, dim3(kSepThreads),
                          lds, c.stream, P1 + s0 * P.p1stride, P.p1stride,
                          P.wy.as<float>(), nxl, nyl, ny, nz, P.yblk,
                          P2 + s0 * P.p2stride, P.p2stride,
-                         masks + (size_t)s0 * ny, P.zc,
+                         masks + (size_t)s0 * ny, P.zc, zpb,
                          gflags + (size_t)s0 * nz * P.ngy, P.ngy);
         GBK_HIP(hipGetLastError());
     });
