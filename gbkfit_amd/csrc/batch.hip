@@ -724,7 +724,6 @@ __global__ __launch_bounds__(256) void sep_xpass(
 {
     extern __shared__ float4 M4[];        // [zc][mp] floats
     float* M = reinterpret_cast<float*>(M4);
-    __shared__ float s_w[TX];
     constexpr int T = kSepThreads, UX = kSepXLines;
     const int y = blockIdx.x, set = blockIdx.z;
     unsigned long long live = rowmask[(size_t)set * ny1 + y];
@@ -736,7 +735,6 @@ __global__ __launch_bounds__(256) void sep_xpass(
         const int sg = x >> 6;
         return sg >= 63 || ((seg >> sg) & 1ull);
     };
-    for (int i = threadIdx.x; i < TX; i += T) s_w[i] = wx[i];
     // lines in form 1 (batch_lines_thin): amplitude and window precomputed
     const float4* lrow = lines + set * lstride + (size_t)y * lpitch;
     float4 V[UX];
@@ -777,7 +775,7 @@ __global__ __launch_bounds__(256) void sep_xpass(
             if (L.x != 0.0f) deposit(L, x);
         }
         __syncthreads();
-        sep_xfilter<SX, TX>(M, s_w, mp, zn, z0, nxl, ny1, out);
+        sep_xfilter<SX, TX>(M, wx, mp, zn, z0, nxl, ny1, out);
         __syncthreads();            // the tile is free for the next chunk
     }
 }

@@ -29,9 +29,10 @@ __host__ __device__ constexpr int sep_dxb(int stride)
 // Lane l of a wave takes channel l % 16; its four 16-lane groups take
 // output blocks DXB apart, whose inputs start 16 banks apart (SX J DXB = 16
 // mod 64), and the 16 channels of a group, mp (odd) apart, hit 16 distinct
-// banks: no bank conflicts with 32 or 64 banks.
+// banks: no bank conflicts with 32 or 64 banks.  The taps w are wave-uniform
+// (global memory: scalar loads into SGPRs; or an LDS copy).
 template <int SX, int TX>
-__device__ __forceinline__ void sep_xfilter(const float* M, const float* s_w,
+__device__ __forceinline__ void sep_xfilter(const float* M, const float* taps,
                                             int mp, int zn, int z0, int nxl,
                                             int ny1, float* __restrict__ out)
 {
@@ -52,7 +53,7 @@ __device__ __forceinline__ void sep_xfilter(const float* M, const float* s_w,
         float o[J] = {};
 #pragma unroll
         for (int u = 0; u < TX; ++u) {
-            const float w = s_w[u];
+            const float w = taps[u];
 #pragma unroll
             for (int j = 0; j < J; ++j) o[j] = fmaf(w, v[SX * j + u], o[j]);
         }
