@@ -97,6 +97,7 @@ def lib():
         L.or_residual_sum.restype = ctypes.c_double
         L.or_residual_sum.argtypes = [_dp, ctypes.c_longlong, ctypes.c_int]
         L.or_circular_convolve.argtypes = [_ip, _dp, _dp, _dp]
+        L.or_wcube.argtypes = [_ip, ctypes.c_int, _dp, _dp]
         _LIB = L
     return _LIB
 
@@ -140,8 +141,11 @@ class DiskSpec:
 
     def __init__(self, rnodes, traits, xpos=0.0, ypos=0.0, posa=0.0, incl=0.0,
                  vsys=0.0, rstep=None, loose=False, tilted=False,
-                 has_vsys=True):
+                 has_vsys=True, opacity=None):
         self.nodes = np.asarray(rnodes, dtype=np.float64)
+        # spatial opacity cube (nz, ny, nx) of the grid, or None
+        self.opacity = None if opacity is None else \
+            np.asarray(opacity, dtype=np.float64)
         self.sub, self.rstep = subrnodes(self.nodes, rstep)
         self.loose = bool(loose)
         self.tilted = bool(tilted)
@@ -194,6 +198,7 @@ class DiskSpec:
         c.sub = r(self.sub)
         c.vsys, c.xpos, c.ypos = r(self.vsys), r(self.xpos), r(self.ypos)
         c.posa, c.incl = r(self.posa), r(self.incl)
+        c.opacity = r(self.opacity)
         c.traits = {}
         for k, tr in self.traits.items():
             c.traits[k] = None if tr is None else \
@@ -228,7 +233,7 @@ class DiskSpec:
                     _p(pc, _ip)
                 t.cvalues, t.pvalues = _p(cv), _p(pv)
             setattr(d, k, t)
-        d.opacity = None
+        d.opacity = arr(self.opacity)
         return d, keep
 
 
@@ -347,6 +352,17 @@ def residual(obs_d, obs_e, obs_m, mdl_d, mdl_w, mdl_m, weight):
 def residual_sum(r, squared=True):
     r = np.ascontiguousarray(r, np.float64).ravel()
     return lib().or_residual_sum(_p(r), r.size, int(squared))
+
+
+def wcube(spat, spec_size):
+    """Spectral weight cube from a spatial (nz, ny, nx) one
+    (core/gmodel/gmodels.hpp:52-82)."""
+    spat = np.ascontiguousarray(spat, np.float64)
+    nz, ny, nx = spat.shape
+    out = np.zeros((spec_size, ny, nx))
+    lib().or_wcube(_p(_i3((nx, ny, nz)), _ip), int(spec_size), _p(spat),
+                   _p(out))
+    return out
 
 
 def circular_convolve(a, k):

@@ -940,6 +940,23 @@ void or_downscale(const int scale[3], const int offset[3],
     }
 }
 
+void or_wcube(const int spat_size[3], int spec_size, const double* spat,
+              double* spec)                             /* gmodels.hpp:52-82 */
+{
+    const int nx = spat_size[0], ny = spat_size[1], nz = spat_size[2];
+    const long long plane = (long long)nx * ny;
+    for (long long c = 0; c < plane; ++c) {
+        double sum = 0, mx = 0;
+        for (int z = 0; z < nz; ++z) {
+            const double v = spat[c + z * plane];
+            sum += v;
+            if (v > mx) mx = v;
+        }
+        const double w = mx > 0 ? (sum / nz) / mx : 0.0;
+        for (int z = 0; z < spec_size; ++z) spec[c + z * plane] = w;
+    }
+}
+
 void or_mask(double cutoff, int apply, const int size[3], double* dcube,
              double* mcube, double* wcube)              /* dmodels.hpp:48-74 */
 {

@@ -116,6 +116,12 @@ void or_residual(const double* obs_d, const double* obs_e,
 /* Neumaier-compensated sum of r^2 (squared) or |r|. */
 double or_residual_sum(const double* r, long long size, int squared);
 
+/* Spectral weight cube (spec_z, ny, nx) from a spatial one (nz, ny, nx):
+ * per spaxel mean / max of the spatial column, 0 where the max is not
+ * positive, on every channel (core/gmodel/gmodels.hpp:52-82). */
+void or_wcube(const int spat_size[3], int spec_size, const double* spat,
+              double* spec);
+
 /* Circular 3-D convolution by brute force (tiny cubes only): the closed form
  * the FFT convolution (gbkfit_host/include/gbkfit/host/fft.hpp:199-207)
  * computes.  out[x] = sum_j a[j] * k[(x - j) mod n]. */

@@ -229,6 +229,18 @@ static T residual_sum(const T* r, int size, int squared, int parallel)
     return sum;
 }
 
+// Spectral weight cube from a spatial one, per spaxel: the reference's
+// gmodel_wcube_pixel (gmodels.hpp:52-82) over every (x, y), as its host loop
+// does (gbkfit_host/include/gbkfit/host/kernels.hpp, gmodel_wcube_evaluate).
+template<typename T>
+static void wcube(const int* spat, int spec_z, const T* sw, T* out)
+{
+    for (int y = 0; y < spat[1]; ++y)
+    for (int x = 0; x < spat[0]; ++x)
+        gbkfit::gmodel_wcube_pixel<T>(x, y, spat[0], spat[1], spat[2], spec_z,
+                                      sw, out);
+}
+
 #define REF_EXPORTS(T, S)                                                      \
 extern "C" {                                                                   \
 void ref_smdisk_##S(const RefDisk<T>* d, const RefGrid<T>* g,                  \
@@ -255,6 +267,8 @@ void ref_residual_##S(const T* od, const T* oe, const T* om, const T* md,      \
     { residual<T>(od, oe, om, md, mw, mm, size, weight, res, par); }           \
 T ref_residual_sum_##S(const T* r, int size, int squared, int par)             \
     { return residual_sum<T>(r, size, squared, par); }                         \
+void ref_wcube_##S(const int* spat, int spec_z, const T* sw, T* out)           \
+    { wcube<T>(spat, spec_z, sw, out); }                                       \
 }
 
 REF_EXPORTS(float, f32)

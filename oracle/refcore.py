@@ -78,6 +78,7 @@ class _Ref:
                                  ip, fp, fp, fp, ctypes.c_int]
         f('residual').argtypes = [fp, fp, fp, fp, fp, fp, ctypes.c_int,
                                   self.ct, fp, ctypes.c_int]
+        f('wcube').argtypes = [ip, ctypes.c_int, fp, fp]
         f('residual_sum').restype = self.ct
         f('residual_sum').argtypes = [fp, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int]
@@ -118,7 +119,7 @@ class _Ref:
                 t.cvalues = self.arr(cv, keep)
                 t.pvalues = self.arr(pv, keep)
             setattr(d, k, t)
-        d.opacity = None
+        d.opacity = self.arr(spec.opacity, keep)
         return d
 
     def grid(self, g):
@@ -220,6 +221,14 @@ class _Ref:
         r = np.ascontiguousarray(r, self.np).ravel()
         return self.f('residual_sum')(self.p(r), r.size, int(squared),
                                       int(parallel))
+
+    def wcube(self, spat, spec_size):
+        spat = np.ascontiguousarray(spat, self.np)
+        nz, ny, nx = spat.shape
+        out = np.zeros((spec_size, ny, nx), self.np)
+        size = (ctypes.c_int * 3)(nx, ny, nz)
+        self.f('wcube')(size, int(spec_size), self.p(spat), self.p(out))
+        return out
 
 
 def ref(dtype=np.float32):

@@ -118,6 +118,47 @@ SMDISK_CASES = [
         posa=nw(100.0, 90.0), incl=nw(60.0, 50.0), vsys=nw(10.0, 10.0)),
 ]
 
+def opacity_cube(case):
+    """A seeded spatial opacity cube (nz, ny, nx) for a case's grid: values
+    in [0, 0.25) over the disk's extent, zero elsewhere."""
+    g = build_grid(case)
+    nx, ny, nz = g.spat_size
+    rng = np.random.default_rng(2024)
+    op = rng.uniform(0.0, 0.25, (nz, ny, nx))
+    op[:, :, :nx // 8] = 0.0
+    return op
+
+
+# Opacity (a spatial cube that dims every voxel by the voxels in front of
+# it, gmodels.hpp:903-914; thick disks only, thin ones have one voxel per
+# column) and weights (wp traits, wdata outputs, gmodels.hpp:52-82,
+# traits.hpp:1859-1885; the only weight trait, axis_range, is 1).
+EXTRA_CASES = [
+    _thick('thick_opacity', dict(
+        rp=[T(2, params=(1.0, 4.0))],
+        rh=[T(7, consts=(0, 0), params=(1.0,))],
+        vp=[T(2, params=(3.0, 200.0))],
+        dp=[T(1, params=(20.0,))]), posa=60, incl=55),
+    _thin('thin_weights', dict(
+        rp=[T(2, params=(1.0, 4.0))],
+        vp=[T(2, params=(3.0, 200.0))],
+        dp=[T(1, params=(20.0,))],
+        wp=[T(1, consts=(1, 30.0, 0.5))]), posa=30, incl=45),
+    _thick('thick_weights', dict(
+        rp=[T(2, params=(1.0, 4.0))],
+        rh=[T(7, consts=(0, 0), params=(1.0,))],
+        vp=[T(2, params=(3.0, 200.0))],
+        dp=[T(1, params=(20.0,))],
+        wp=[T(1, consts=(0, 10.0, 2.0))]), posa=120, incl=60),
+]
+EXTRA_WANT = {
+    'thick_opacity': ('scube', 'image', 'rdata', 'rdata_cmp', 'ordata',
+                      'ordata_cmp'),
+    'thin_weights': ('scube', 'wdata', 'wdata_cmp'),
+    'thick_weights': ('scube', 'image', 'wdata', 'wdata_cmp'),
+}
+
+
 # mcdisk cases: thick disks whose density traits are sampled per ring.
 MCDISK_CASES = [
     dict(name='mc_sech2', traits=dict(
@@ -139,6 +180,8 @@ MCDISK_CASES = [
 def build_spec(case):
     kw = dict(case['kw']) if 'kw' in case else {
         k: case[k] for k in ('posa', 'incl') if k in case}
+    if case.get('name') == 'thick_opacity':
+        kw['opacity'] = opacity_cube(case)
     return DiskSpec(RN, case['traits'], **kw)
 
 

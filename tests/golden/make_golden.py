@@ -44,6 +44,23 @@ def smdisk_fixtures():
     return out
 
 
+def extra_fixtures():
+    """Opacity and weight cases (FP32 core, every output they exercise) and
+    the spectral weight cube of the thick weight case's spatial weights."""
+    out = {}
+    for case in cases.EXTRA_CASES:
+        want = cases.EXTRA_WANT[case['name']]
+        spec = cases.build_spec(case).rounded_f32()
+        r32 = refcore.ref(np.float32).smdisk(spec, oracle.grid_f32(
+            cases.build_grid(case)), want)
+        for k in want:
+            out[f"{case['name']}/{k}/f32"] = r32[k]
+    w = out['thick_weights/wdata/f32']
+    for dt, sfx in ((np.float32, 'f32'), (np.float64, 'f64')):
+        out[f'wcube/spec/{sfx}'] = refcore.ref(dt).wcube(w, 24)
+    return out
+
+
 def mcdisk_fixtures():
     out = {}
     for case in cases.MCDISK_CASES:
@@ -117,6 +134,7 @@ def main():
         raise SystemExit("reference-core harness missing: make -C oracle")
     fx = {}
     fx.update(smdisk_fixtures())
+    fx.update(extra_fixtures())
     fx.update(mcdisk_fixtures())
     fx.update(dmodel_fixtures())
     meta = dict(

@@ -32,7 +32,7 @@ def disk_args(drv, spec):
             uids, cv, cc, pv, pc = tr
             args += [dev(drv, uids, np.int32), dev(drv, cv), dev(drv, cc,
                      np.int32), dev(drv, pv), dev(drv, pc, np.int32)]
-    args.append(None)   # opacity
+    args.append(dev(drv, getattr(spec, 'opacity', None)))   # opacity
     return args
 
 

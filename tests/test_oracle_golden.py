@@ -252,3 +252,54 @@ def test_padding_rules():
     g = oracle.DCubeGeom((128, 128, 96), (1, 1, 10), scale=(2, 2, 1),
                          psf_sigma=3 / 2.35482, lsf_sigma=20 / 2.35482)
     assert g.size_hi == (320, 320, 128) and g.edge_hi == (20, 20, 7)
+
+
+@pytest.mark.parametrize('case', cases.EXTRA_CASES, ids=lambda c: c['name'])
+def test_opacity_and_weights_vs_reference_core(case):
+    """Opacity (each voxel dimmed by the opacity of the voxels in front of
+    it, gmodels.hpp:903-914) and weight traits (wdata, traits.hpp:
+    1859-1885) against the reference core: every output the case writes,
+    to FP32 rounding beyond the ambiguity bound."""
+    spec = cases.build_spec(case).rounded_f32()
+    grid = oracle.grid_f32(cases.build_grid(case))
+    want = cases.EXTRA_WANT[case['name']]
+    out, _ = oracle.smdisk(spec, grid, want=want + ('amb',))
+    for k in want:
+        ref = FX[f"{case['name']}/{k}/f32"].astype(np.float64)
+        assert np.abs(ref).max() > 0, k
+        if k == 'scube':
+            bad = np.abs(out[k] - ref) > 2e-6 * np.abs(ref).max() + \
+                out['amb']
+            assert bad.sum() == 0, (case['name'], k)
+        else:
+            assert normwise(out[k], ref) < 2e-6, (case['name'], k)
+
+
+def test_planted_opacity_mistakes_are_caught():
+    """Ignoring the opacity, or dimming by the voxels behind instead of in
+    front, must fail the comparison with the reference core."""
+    case = [c for c in cases.EXTRA_CASES if c['name'] == 'thick_opacity'][0]
+    grid = oracle.grid_f32(cases.build_grid(case))
+    ref = FX['thick_opacity/scube/f32'].astype(np.float64)
+    spec = cases.build_spec(case).rounded_f32()
+    spec.opacity = None
+    out, _ = oracle.smdisk(spec, grid, want=('scube',))
+    assert normwise(out['scube'], ref) > 1e-2
+    spec = cases.build_spec(case).rounded_f32()
+    spec.opacity = spec.opacity[::-1].copy()
+    out, _ = oracle.smdisk(spec, grid, want=('scube',))
+    assert normwise(out['scube'], ref) > 1e-2
+
+
+def test_wcube_vs_reference_core():
+    """Spectral weight cube: per spaxel the mean over the spatial column
+    divided by its maximum (gmodels.hpp:52-82); a sum instead of a mean
+    fails."""
+    w = FX['thick_weights/wdata/f32'].astype(np.float64)
+    got = oracle.wcube(w, 24)
+    assert normwise(got, FX['wcube/spec/f64']) < 1e-14
+    assert normwise(got, FX['wcube/spec/f32']) < 1e-6
+    assert 0 < got.max() <= 1 and np.unique(got).size > 3
+    planted = np.where(w.max(0) > 0, w.sum(0) / np.where(w.max(0) > 0,
+                                                        w.max(0), 1), 0)
+    assert np.abs(planted - FX['wcube/spec/f64'][0]).max() > 0.5
