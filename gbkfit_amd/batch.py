@@ -199,6 +199,10 @@ class BatchEvaluator:
                 type(comps[0].disk).__name__ not in ('SMDisk', 'MCDisk'):
             raise RuntimeError(
                 "BatchEvaluator needs one smdisk or mcdisk component")
+        if getattr(gmodel, 'ocomponents', ()) or \
+                not hasattr(gmodel, 'evaluate_scube'):
+            raise RuntimeError("BatchEvaluator takes spectral gmodels "
+                               "without opacity components")
         disk = comps[0].disk
         self.mc = type(disk).__name__ == 'MCDisk'
         cube = dmodel.dcube()

@@ -338,3 +338,145 @@ class DModelMMaps(DModel):
         return {key: dict(d=self._mmaps_d[i], m=self._mmaps_m,
                           w=self._mmaps_w[i])
                 for i, key in enumerate(self.keys())}
+
+
+class DModelImage(DModel):
+    """Image data model (image.py:17-124): a one-channel cube (spectral
+    step 0) convolved with the PSF only, for the GModelIntensity2D/3D
+    gmodels (their evaluate_image)."""
+
+    @staticmethod
+    def type():
+        return 'image'
+
+    def __init__(self, size, step=(1, 1), rpix=None, rval=(0, 0), rota=0,
+                 scale=(1, 1), psf=None, mask_cutoff=None, mask_apply=False,
+                 dtype='float32', padding='reference'):
+        super().__init__()
+        size = tuple(int(v) for v in size)
+        if rpix is None:
+            rpix = tuple(s / 2 - 0.5 for s in size)
+        self._dcube = DCube(size + (1,), tuple(step) + (0,),
+                            tuple(rpix) + (0,), tuple(rval) + (0,), rota,
+                            tuple(scale) + (1,), psf, None, False,
+                            mask_cutoff, mask_apply, dtype, padding)
+
+    def keys(self):
+        return ['image']
+
+    def size(self):
+        return self._dcube.size()[:2]
+
+    def step(self):
+        return self._dcube.step()[:2]
+
+    def zero(self):
+        return self._dcube.zero()[:2]
+
+    def rota(self):
+        return self._dcube.rota()
+
+    def scale(self):
+        return self._dcube.scale()[:2]
+
+    def psf(self):
+        return self._dcube.psf()
+
+    def dtype(self):
+        return self._dcube.dtype()
+
+    def dcube(self):
+        return self._dcube
+
+    def _prepare_impl(self, gmodel):
+        self._dcube.prepare(self._driver, gmodel.has_weights())
+
+    def _evaluate_impl(self, params, out_dmodel_extra, out_gmodel_extra):
+        d, g, c = self._driver, self._gmodel, self._dcube
+        d.mem_fill(c.scratch_dcube(), 0)
+        g.evaluate_image(d, params, c.scratch_dcube(), c.scratch_wcube(),
+                         c.scratch_size()[:2], c.scratch_step()[:2],
+                         c.scratch_zero()[:2], c.rota(), c.dtype(),
+                         out_gmodel_extra)
+        c.evaluate(out_dmodel_extra)
+        m, w = c.mcube(), c.wcube()
+        return dict(image=dict(d=c.dcube()[0],
+                               m=None if m is None else m[0],
+                               w=None if w is None else w[0]))
+
+
+class DModelLSlit(DModel):
+    """Long-slit data model (lslit.py:17-123): a spectral cube one spatial
+    pixel high (the slit; its width is the y step), returned as the
+    (channel, position) plane.
+
+    The reference's evaluate returns dcube()[0][:, 0, :], the first channel
+    indexed as if it were a cube, and the mask as the weights; here the
+    plane is cube[:, 0, :] of the data, mask and weight cubes.  The default
+    reference pixel is the centre of the slit and of the spectrum, in the
+    middle of the single row."""
+
+    @staticmethod
+    def type():
+        return 'lslit'
+
+    def __init__(self, size, step=(1, 1, 1), rpix=None, rval=(0, 0, 0),
+                 rota=0, scale=(1, 1, 1), psf=None, lsf=None, weight=1,
+                 mask_cutoff=None, mask_create=False, mask_apply=False,
+                 dtype='float32', padding='reference'):
+        super().__init__()
+        if len(size) != 2:
+            raise RuntimeError("lslit size is (positions, channels)")
+        size = (int(size[0]), 1, int(size[1]))
+        if rpix is None:
+            rpix = (size[0] / 2 - 0.5, 0.0, size[2] / 2 - 0.5)
+        self._weight = weight
+        self._dcube = DCube(size, tuple(step), tuple(rpix), tuple(rval),
+                            rota, tuple(scale), psf, lsf, False,
+                            mask_cutoff if (mask_create or mask_apply)
+                            else None, mask_apply, dtype, padding)
+
+    def keys(self):
+        return ['lslit']
+
+    def size(self):
+        return self._dcube.size()
+
+    def step(self):
+        return self._dcube.step()
+
+    def zero(self):
+        return self._dcube.zero()
+
+    def rota(self):
+        return self._dcube.rota()
+
+    def scale(self):
+        return self._dcube.scale()
+
+    def psf(self):
+        return self._dcube.psf()
+
+    def lsf(self):
+        return self._dcube.lsf()
+
+    def dtype(self):
+        return self._dcube.dtype()
+
+    def dcube(self):
+        return self._dcube
+
+    def _prepare_impl(self, gmodel):
+        self._dcube.prepare(self._driver, gmodel.has_weights())
+
+    def _evaluate_impl(self, params, out_dmodel_extra, out_gmodel_extra):
+        d, g, c = self._driver, self._gmodel, self._dcube
+        d.mem_fill(c.scratch_dcube(), 0)
+        g.evaluate_scube(d, params, c.scratch_dcube(), c.scratch_wcube(),
+                         c.scratch_size(), c.scratch_step(), c.scratch_zero(),
+                         c.rota(), c.dtype(), out_gmodel_extra)
+        c.evaluate(out_dmodel_extra)
+        m, w = c.mcube(), c.wcube()
+        return dict(lslit=dict(d=c.dcube()[:, 0, :],
+                               m=None if m is None else m[:, 0, :],
+                               w=None if w is None else w[:, 0, :]))

@@ -7,10 +7,22 @@ Mirror of the reference callers of the GModel backend:
   * SMDisk / MCDisk (_smdisk.py:38-96, _mcdisk.py:60-189);
   * spectral components SpectralSMDisk2D/3D and SpectralMCDisk3D
     (spectral_smdisk_2d.py, spectral_smdisk_3d.py, spectral_mcdisk_3d.py);
-  * GModelKinematics2D/3D (kinematics_2d.py, kinematics_3d.py).
+  * brightness components BrightnessSMDisk2D/3D and BrightnessMCDisk3D
+    (brightness_smdisk_2d.py, brightness_smdisk_3d.py,
+    brightness_mcdisk_3d.py) and opacity components OpacitySMDisk3D /
+    OpacityMCDisk3D (opacity_smdisk_3d.py, opacity_mcdisk_3d.py);
+  * GModelKinematics2D/3D (kinematics_2d.py, kinematics_3d.py) and
+    GModelIntensity2D/3D (intensity_2d.py, intensity_3d.py).
 Parameter names follow the reference: 'xpos', 'ypos', 'posa', 'incl',
 'vsys' and '<kind>[i]_<name>' per trait (miscutils.py:70-95), with 'cmp<i>_'
-prefixed for components after the first.
+prefixed for components after the first and 'ocmp_', 'ocmp<i>_' for opacity
+components (_detail.py:339-342).
+
+The 3-D spatial weight and opacity cubes are zeroed before every evaluation:
+the reference allocates them uninitialised once and adds into them on every
+call (kinematics_3d.py:112-118, intensity_3d.py:111-117; the smdisk kernels
+add, gmodels.hpp:933-948), so its weights and opacities depend on what the
+memory held.
 """
 import itertools
 
@@ -393,61 +405,178 @@ class SpectralMCDisk3D(_Component):
                            spec_size, spec_step, spec_zero, dtype, out_extra)
 
 
-def _merge_params(components):
-    """Component parameters, 'cmp<i>_'-prefixed after the first
-    (gmodels/_detail.py:320-342)."""
+def _b3d_traits(bptraits, bhtraits, zptraits, sptraits, wptraits,
+                kinds=('bpt', 'bht')):
+    """parse_component_b3d_traits / parse_component_o3d_traits
+    (gmodels/_detail.py): density polar and height traits, one height trait
+    per polar one."""
+    rp, rh = kinds
+    tr = dict(rp=_traits_of(rp, bptraits), rh=_traits_of(rh, bhtraits),
+              zp=_traits_of('zpt', zptraits), sp=_traits_of('spt', sptraits),
+              wp=_traits_of('wpt', wptraits))
+    if not (tr['rp'] and tr['rh']):
+        raise RuntimeError(f"{rp}traits and {rh}traits are required")
+    if len(tr['rp']) != len(tr['rh']):
+        raise RuntimeError(f"{rp} and {rh} trait counts differ")
+    return tr
+
+
+class BrightnessSMDisk2D(_Component):
+    """Thin smooth disk of surface brightness (brightness_smdisk_2d.py:
+    50-106): density polar, selection and weight traits, no kinematics,
+    evaluated into an image (spectral size 1)."""
+
+    def __init__(self, loose, tilted, bptraits, sptraits=None, wptraits=None,
+                 rnmin=None, rnmax=None, rnsep=None, rnlen=None, rnodes=None,
+                 rstep=None, interp='linear'):
+        if interp != 'linear':
+            raise RuntimeError("only linear interpolation is supported")
+        rn, rs = parse_rnodes(rnmin, rnmax, rnsep, rnlen, rnodes, rstep)
+        tr = dict(rp=_traits_of('bpt', bptraits),
+                  sp=_traits_of('spt', sptraits),
+                  wp=_traits_of('wpt', wptraits))
+        if not tr['rp']:
+            raise RuntimeError("bptraits are required")
+        self.disk = SMDisk(loose, tilted, rn, rs, tr)
+        self.thin = True
+
+    def evaluate(self, driver, params, image, wdata, bdata, spat_size,
+                 spat_step, spat_zero, spat_rota, dtype, out_extra=None):
+        self.disk.evaluate(driver, params, None, image, None, wdata, bdata,
+                           None, tuple(spat_size) + (1,),
+                           tuple(spat_step) + (0,), tuple(spat_zero) + (0,),
+                           spat_rota, 1, 0, 0, dtype, out_extra)
+
+
+class BrightnessSMDisk3D(_Component):
+    """Thick smooth disk of brightness (brightness_smdisk_3d.py:62-123):
+    the image is the line-of-sight sum through the opacity cube."""
+
+    def __init__(self, loose, tilted, bptraits, bhtraits, zptraits=None,
+                 sptraits=None, wptraits=None, rnmin=None, rnmax=None,
+                 rnsep=None, rnlen=None, rnodes=None, rstep=None,
+                 interp='linear'):
+        if interp != 'linear':
+            raise RuntimeError("only linear interpolation is supported")
+        rn, rs = parse_rnodes(rnmin, rnmax, rnsep, rnlen, rnodes, rstep)
+        self.disk = SMDisk(loose, tilted, rn, rs, _b3d_traits(
+            bptraits, bhtraits, zptraits, sptraits, wptraits))
+        self.thin = False
+
+    def evaluate(self, driver, params, odata, image, wdata, bdata, obdata,
+                 spat_size, spat_step, spat_zero, spat_rota, dtype,
+                 out_extra=None):
+        self.disk.evaluate(driver, params, odata, image, None, wdata, bdata,
+                           obdata, spat_size, spat_step, spat_zero,
+                           spat_rota, 1, 0, 0, dtype, out_extra)
+
+
+class BrightnessMCDisk3D(_Component):
+    """Monte Carlo cloud disk of brightness (brightness_mcdisk_3d.py:
+    55-105)."""
+
+    def __init__(self, cflux, loose, tilted, bptraits, bhtraits,
+                 zptraits=None, sptraits=None, wptraits=None, rnmin=None,
+                 rnmax=None, rnsep=None, rnlen=None, rnodes=None, rstep=None,
+                 interp='linear'):
+        if interp != 'linear':
+            raise RuntimeError("only linear interpolation is supported")
+        rn, rs = parse_rnodes(rnmin, rnmax, rnsep, rnlen, rnodes, rstep)
+        self.disk = MCDisk(cflux, loose, tilted, rn, rs, _b3d_traits(
+            bptraits, bhtraits, zptraits, sptraits, wptraits))
+        self.thin = False
+
+    evaluate = BrightnessSMDisk3D.evaluate
+
+
+class OpacitySMDisk3D(_Component):
+    """Smooth disk of opacity (opacity_smdisk_3d.py:52-127): its density
+    (opacity polar and height traits) is added into the 3-D opacity cube,
+    which the brightness or spectral components then read."""
+
+    def __init__(self, loose, tilted, optraits, ohtraits, zptraits=None,
+                 sptraits=None, wptraits=None, rnmin=None, rnmax=None,
+                 rnsep=None, rnlen=None, rnodes=None, rstep=None,
+                 interp='linear'):
+        if interp != 'linear':
+            raise RuntimeError("only linear interpolation is supported")
+        rn, rs = parse_rnodes(rnmin, rnmax, rnsep, rnlen, rnodes, rstep)
+        self.disk = SMDisk(loose, tilted, rn, rs, _b3d_traits(
+            optraits, ohtraits, zptraits, sptraits, wptraits,
+            kinds=('opt', 'oht')))
+        self.thin = False
+
+    def evaluate(self, driver, params, odata, spat_size, spat_step,
+                 spat_zero, spat_rota, dtype, out_extra=None):
+        self.disk.evaluate(driver, params, None, None, None, None, odata,
+                           None, spat_size, spat_step, spat_zero, spat_rota,
+                           1, 0, 0, dtype, out_extra)
+
+
+class OpacityMCDisk3D(_Component):
+    """Monte Carlo cloud disk of opacity (opacity_mcdisk_3d.py:54-120)."""
+
+    def __init__(self, cflux, loose, tilted, optraits, ohtraits,
+                 zptraits=None, sptraits=None, wptraits=None, rnmin=None,
+                 rnmax=None, rnsep=None, rnlen=None, rnodes=None, rstep=None,
+                 interp='linear'):
+        if interp != 'linear':
+            raise RuntimeError("only linear interpolation is supported")
+        rn, rs = parse_rnodes(rnmin, rnmax, rnsep, rnlen, rnodes, rstep)
+        self.disk = MCDisk(cflux, loose, tilted, rn, rs, _b3d_traits(
+            optraits, ohtraits, zptraits, sptraits, wptraits,
+            kinds=('opt', 'oht')))
+        self.thin = False
+
+    evaluate = OpacitySMDisk3D.evaluate
+
+
+def _merge_params(components, prefix='cmp', zero_prefix=False):
+    """Component parameters: '<prefix><i>_'-prefixed after the first, and
+    '<prefix>_' on the first with zero_prefix (miscutils.py:70-95,
+    gmodels/_detail.py:320-342)."""
     merged, maps = {}, []
     for i, c in enumerate(components):
+        full_prefix = (prefix if i or zero_prefix else '') + \
+            (str(i) if i else '')
+        full_prefix += '_' if full_prefix else ''
         m = {}
         for n, size in c.pdescs().items():
-            full = f'cmp{i}_{n}' if i else n
-            m[n] = full
-            merged[full] = size
+            m[n] = full_prefix + n
+            merged[full_prefix + n] = size
         maps.append(m)
     return merged, maps
 
 
-class GModelKinematics2D:
-    """kinematics_2d.py: thin components on the data model's spatial grid."""
-
-    def __init__(self, components):
-        self.components = tuple(components) if isinstance(
-            components, (list, tuple)) else (components,)
-        self._pdescs, self._maps = _merge_params(self.components)
-        self._wdata = None
-
-    def pdescs(self):
-        return dict(self._pdescs)
-
-    def has_weights(self):
-        return any(c.has_weights() for c in self.components)
-
-    def evaluate_scube(self, driver, params, scube_d, scube_w, size, step,
-                       zero, rota, dtype, out_extra=None):
-        spat_size, spat_step, spat_zero = size[:2], step[:2], zero[:2]
-        if scube_w is not None and self._wdata is None:
-            self._wdata = driver.mem_alloc_d(tuple(spat_size)[::-1], dtype)
-        for c, m in zip(self.components, self._maps):
-            cp = {n: params[m[n]] for n in c.pdescs()}
-            c.evaluate(driver, cp, scube_d, self._wdata, None, spat_size,
-                       spat_step, spat_zero, rota, size[2], step[2], zero[2],
-                       dtype, out_extra)
-        if scube_w is not None:
-            driver.backends().gmodel(dtype).wcube_evaluate(
-                tuple(spat_size) + (1,), size[2], self._wdata, scube_w)
+def _tuplify(components):
+    if components is None:
+        return ()
+    return tuple(components) if isinstance(components, (list, tuple)) \
+        else (components,)
 
 
-class GModelKinematics3D:
-    """kinematics_3d.py:28-187: thick components on a 3-D spatial grid whose
-    z extent defaults to the larger spatial axis (:104-109)."""
+def _sub_params(params, c, m):
+    return {n: params[m[n]] for n in c.pdescs()}
 
-    def __init__(self, components, size_z=None, step_z=None, zero_z=None):
-        self.components = tuple(components) if isinstance(
-            components, (list, tuple)) else (components,)
+
+class _GModel:
+    """Components and opacity components, their merged parameters, and the
+    3-D spatial grid of the thick gmodels: z extent of the larger spatial
+    axis with zero in the middle unless given (kinematics_3d.py:97-109,
+    intensity_3d.py:95-107)."""
+
+    def __init__(self, components, opacity_components=None, size_z=None,
+                 step_z=None, zero_z=None):
+        self.components = _tuplify(components)
+        if not self.components:
+            raise RuntimeError("at least one component must be configured")
+        self.ocomponents = _tuplify(opacity_components)
         self._zspec = (size_z, step_z, zero_z)
-        self._pdescs, self._maps = _merge_params(self.components)
-        self._grid = None
-        self._wdata = None
+        p, self._maps = _merge_params(self.components)
+        op, self._omaps = _merge_params(self.ocomponents, 'ocmp', True)
+        self._pdescs = {**p, **op}
+        self._wdata = self._odata = None
+        self._shape = None
 
     def pdescs(self):
         return dict(self._pdescs)
@@ -467,16 +596,122 @@ class GModelKinematics3D:
         return ((int(size[0]), int(size[1]), int(sz)),
                 (step[0], step[1], stz), (zero[0], zero[1], zz))
 
+    def _spatial(self, driver, spat_size, dtype, want_w):
+        """Zeroed 3-D weight (when weighted) and opacity (when there are
+        opacity components) cubes of the spatial grid."""
+        shape = (tuple(spat_size)[::-1], dtype)
+        if shape != self._shape:
+            self._shape = shape
+            self._wdata = self._odata = None
+        if want_w:
+            if self._wdata is None:
+                self._wdata = driver.mem_alloc_d(*shape)
+            driver.mem_fill(self._wdata, 0)
+        if self.ocomponents:
+            if self._odata is None:
+                self._odata = driver.mem_alloc_d(*shape)
+            driver.mem_fill(self._odata, 0)
+        return (self._wdata if want_w else None), \
+            (self._odata if self.ocomponents else None)
+
+    def _opacity(self, driver, params, odata, spat_size, spat_step,
+                 spat_zero, rota, dtype, out_extra):
+        for c, m in zip(self.ocomponents, self._omaps):
+            c.evaluate(driver, _sub_params(params, c, m), odata, spat_size,
+                       spat_step, spat_zero, rota, dtype, out_extra)
+
+
+class GModelKinematics2D(_GModel):
+    """kinematics_2d.py: thin components on the data model's spatial grid."""
+
+    def __init__(self, components):
+        super().__init__(components)
+
     def evaluate_scube(self, driver, params, scube_d, scube_w, size, step,
                        zero, rota, dtype, out_extra=None):
-        spat_size, spat_step, spat_zero = self.spatial_grid(size, step, zero)
-        if scube_w is not None and self._wdata is None:
-            self._wdata = driver.mem_alloc_d(spat_size[::-1], dtype)
+        spat_size, spat_step, spat_zero = size[:2], step[:2], zero[:2]
+        wdata, _ = self._spatial(driver, tuple(spat_size) + (1,), dtype,
+                                 scube_w is not None)
         for c, m in zip(self.components, self._maps):
-            cp = {n: params[m[n]] for n in c.pdescs()}
-            c.evaluate(driver, cp, None, scube_d, self._wdata, None, None,
-                       spat_size, spat_step, spat_zero, rota, size[2],
+            c.evaluate(driver, _sub_params(params, c, m), scube_d, wdata,
+                       None, spat_size, spat_step, spat_zero, rota, size[2],
                        step[2], zero[2], dtype, out_extra)
         if scube_w is not None:
             driver.backends().gmodel(dtype).wcube_evaluate(
-                spat_size, size[2], self._wdata, scube_w)
+                tuple(spat_size) + (1,), size[2], wdata, scube_w)
+
+
+class GModelKinematics3D(_GModel):
+    """kinematics_3d.py:28-187: thick spectral components, and opacity
+    components evaluated first into the opacity cube they read, on a 3-D
+    spatial grid."""
+
+    def __init__(self, components, opacity_components=None, size_z=None,
+                 step_z=None, zero_z=None):
+        super().__init__(components, opacity_components, size_z, step_z,
+                         zero_z)
+
+    def evaluate_scube(self, driver, params, scube_d, scube_w, size, step,
+                       zero, rota, dtype, out_extra=None):
+        spat_size, spat_step, spat_zero = self.spatial_grid(size, step, zero)
+        wdata, odata = self._spatial(driver, spat_size, dtype,
+                                     scube_w is not None)
+        self._opacity(driver, params, odata, spat_size, spat_step, spat_zero,
+                      rota, dtype, out_extra)
+        for c, m in zip(self.components, self._maps):
+            c.evaluate(driver, _sub_params(params, c, m), odata, scube_d,
+                       wdata, None, None, spat_size, spat_step, spat_zero,
+                       rota, size[2], step[2], zero[2], dtype, out_extra)
+        if scube_w is not None:
+            driver.backends().gmodel(dtype).wcube_evaluate(
+                spat_size, size[2], wdata, scube_w)
+
+
+class GModelIntensity2D(_GModel):
+    """intensity_2d.py: thin brightness components summed into an image on
+    the data model's spatial grid; the weight image is the spaxel's weight
+    (wcube with one channel)."""
+
+    def __init__(self, components):
+        super().__init__(components)
+
+    def evaluate_image(self, driver, params, image_d, image_w, size, step,
+                       zero, rota, dtype, out_extra=None):
+        spat_size, spat_step, spat_zero = size[:2], step[:2], zero[:2]
+        wdata, _ = self._spatial(driver, tuple(spat_size) + (1,), dtype,
+                                 image_w is not None)
+        for c, m in zip(self.components, self._maps):
+            c.evaluate(driver, _sub_params(params, c, m), image_d, wdata,
+                       None, spat_size, spat_step, spat_zero, rota, dtype,
+                       out_extra)
+        if image_w is not None:
+            driver.backends().gmodel(dtype).wcube_evaluate(
+                tuple(spat_size) + (1,), 1, wdata, image_w)
+
+
+class GModelIntensity3D(_GModel):
+    """intensity_3d.py:31-199: thick brightness components (and opacity
+    components evaluated first) on a 3-D spatial grid, summed along the line
+    of sight into an image."""
+
+    def __init__(self, components, opacity_components=None, size_z=None,
+                 step_z=None, zero_z=None):
+        super().__init__(components, opacity_components, size_z, step_z,
+                         zero_z)
+
+    def evaluate_image(self, driver, params, image_d, image_w, size, step,
+                       zero, rota, dtype, out_extra=None):
+        spat_size, spat_step, spat_zero = self.spatial_grid(size[:2],
+                                                            step[:2],
+                                                            zero[:2])
+        wdata, odata = self._spatial(driver, spat_size, dtype,
+                                     image_w is not None)
+        self._opacity(driver, params, odata, spat_size, spat_step, spat_zero,
+                      rota, dtype, out_extra)
+        for c, m in zip(self.components, self._maps):
+            c.evaluate(driver, _sub_params(params, c, m), odata, image_d,
+                       wdata, None, None, spat_size, spat_step, spat_zero,
+                       rota, dtype, out_extra)
+        if image_w is not None:
+            driver.backends().gmodel(dtype).wcube_evaluate(
+                spat_size, 1, wdata, image_w)

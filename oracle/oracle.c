@@ -586,13 +586,17 @@ static void smdisk_voxel(const or_disk* d, const or_grid* g, const or_out* o,
 
     const long long plane = (long long)nx * ny;
     const long long col = x + (long long)y * nx;
+    /* with no spectral cube wanted, amb bounds the image (its first plane) */
+    const int image_amb = o->amb && o->image && !o->scube;
     if (!in) {  /* ambiguous miss: the line may appear in an FP32 run */
         if (o->scube && o->amb)
             deposit_line(g, o->scube, o->amb, col, plane, orvalue, vvalue,
                          dvalue, 1);
+        if (image_amb) o->amb[col] += fabs(orvalue);
         return;
     }
     if (o->image) o->image[col] += orvalue;
+    if (image_amb && amb) o->amb[col] += fabs(orvalue);
     if (o->scube) {
         const int nch = deposit_line(g, o->scube, o->amb, col, plane, orvalue,
                                      vvalue, dvalue, 0);
@@ -856,8 +860,9 @@ static void mcdisk_cloud(double cflux, int ci, const int* csum,
     }
 
     const long long plane = (long long)nx * ny;
-    /* ambiguous position: every candidate voxel gets the line in `amb` */
-    if (o->amb && o->scube && (pos_amb || amb)) {
+    /* ambiguous position: every candidate voxel gets the line in `amb`
+     * (its flux, when amb bounds the image) */
+    if (o->amb && (o->scube || o->image) && (pos_amb || amb)) {
         const int xs[2] = {(int)floor(ax), (int)floor(ax) + 1};
         const int ys[2] = {(int)floor(ay), (int)floor(ay) + 1};
         const int zs[2] = {(int)floor(az), (int)floor(az) + 1};
@@ -876,8 +881,11 @@ static void mcdisk_cloud(double cflux, int ci, const int* csum,
                 for (int oz = cz + 1; oz < nz; ++oz)
                     orv -= orv * d->opacity[cx + (long long)cy * nx +
                                             (long long)oz * plane];
-            deposit_line(g, o->scube, o->amb, cx + (long long)cy * nx, plane,
-                         orv, vvalue, dvalue, 1);
+            if (o->scube)
+                deposit_line(g, o->scube, o->amb, cx + (long long)cy * nx,
+                             plane, orv, vvalue, dvalue, 1);
+            else
+                o->amb[cx + (long long)cy * nx] += fabs(orv);
         }
     }
     if (!inside || !in) return;

@@ -69,7 +69,8 @@ typedef struct {
     double* ordata_cmp;
     double* vdata_cmp;
     double* ddata_cmp;
-    double* amb;        /* optional ambiguity bound on scube */
+    double* amb;        /* optional ambiguity bound on scube (on image when
+                           scube is not wanted) */
 } or_out;
 
 /* Per-call work counters (algorithmic work the roofline is priced on). */

@@ -27,8 +27,9 @@ def _thin(name, traits, **kw):
 
 def _thick(name, traits, size_z=16, **kw):
     t = dict(traits)
-    t.setdefault('vh', [ONE] * len(t['vp']))
-    t.setdefault('dh', [ONE] * len(t['dp']))
+    for p, h in (('vp', 'vh'), ('dp', 'dh')):
+        if t.get(p):
+            t.setdefault(h, [ONE] * len(t[p]))
     return dict(name=name, thin=False, traits=t, kw=kw, size_z=size_z)
 
 
@@ -150,12 +151,32 @@ EXTRA_CASES = [
         vp=[T(2, params=(3.0, 200.0))],
         dp=[T(1, params=(20.0,))],
         wp=[T(1, consts=(0, 10.0, 2.0))]), posa=120, incl=60),
+    # an opacity disk (opacity polar traits uid 1000 + n, opacity height
+    # traits amplitude x shape, traits.hpp:1437-1514), its density into
+    # rdata (opacity_smdisk_3d.py:116-127); the second height trait is
+    # truncated at 2 scales
+    _thick('opacity_disk', dict(
+        rp=[T(1002, params=(0.1, 4.0)), T(1003, params=(0.05, 3.0))],
+        rh=[T(1007, consts=(0, 0), params=(1.0, 1.0)),
+            T(1003, consts=(2.0, 0), params=(0.5, 0.8))]),
+        posa=75, incl=65),
+    # brightness-only disks (no kinematic traits) into an image
+    # (brightness_smdisk_2d.py / _3d.py: spectral size 1)
+    _thin('bright_thin', dict(
+        rp=[T(2, params=(1.0, 4.0)), T(3, params=(0.5, 2.0))]),
+        posa=210, incl=40),
+    _thick('bright_thick', dict(
+        rp=[T(3, params=(1.0, 3.0))],
+        rh=[T(2, consts=(0, 0), params=(0.7,))]), posa=300, incl=70),
 ]
 EXTRA_WANT = {
     'thick_opacity': ('scube', 'image', 'rdata', 'rdata_cmp', 'ordata',
                       'ordata_cmp'),
     'thin_weights': ('scube', 'wdata', 'wdata_cmp'),
     'thick_weights': ('scube', 'image', 'wdata', 'wdata_cmp'),
+    'opacity_disk': ('rdata', 'rdata_cmp'),
+    'bright_thin': ('image',),
+    'bright_thick': ('image', 'rdata', 'ordata'),
 }
 
 

@@ -291,6 +291,25 @@ def test_planted_opacity_mistakes_are_caught():
     assert normwise(out['scube'], ref) > 1e-2
 
 
+def test_planted_opacity_trait_mistakes_are_caught():
+    """The opacity disk's height traits read as the brightness pdfs (uid
+    7 / 3 instead of 1007 / 1003: normalised by the scale instead of an
+    amplitude), or without the second trait's truncation, must fail the
+    comparison with the reference core."""
+    case = [c for c in cases.EXTRA_CASES if c['name'] == 'opacity_disk'][0]
+    grid = oracle.grid_f32(cases.build_grid(case))
+    ref = FX['opacity_disk/rdata/f32'].astype(np.float64)
+    T = oracle.Trait
+    good = case['traits']['rh']
+    for rh in ([T(7, consts=(0, 0), params=(1.0,)), good[1]],
+               [good[0], T(3, consts=(2.0, 0), params=(0.8,))],
+               [good[0], T(1003, consts=(0, 0), params=(0.5, 0.8))]):
+        c = dict(case, traits=dict(case['traits'], rh=rh))
+        spec = cases.build_spec(c).rounded_f32()
+        out, _ = oracle.smdisk(spec, grid, want=('rdata',))
+        assert normwise(out['rdata'], ref) > 1e-3
+
+
 def test_wcube_vs_reference_core():
     """Spectral weight cube: per spaxel the mean over the spatial column
     divided by its maximum (gmodels.hpp:52-82); a sum instead of a mean
