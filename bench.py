@@ -27,6 +27,12 @@ The plain path is the drop-in call sequence of the reference
 (Objective.log_likelihood per set: pack + upload trait arrays, smdisk,
 FFT convolution, downscale, residual, residual sum, scalar download).
 
+A thick leg (skip with --no-thick) times the same workload with a thick
+disk (sech^2 height of 2 px on a spatial z axis of 64): the deterministic
+thick deposit into a hi-res cube, then the direct pipeline's filters; its
+kernel times, roofline and the deposit's op count (W = 60 V + 12 sum n_ch
+on the 3-D grid) go into the `thick_variant` object.
+
 A serving leg (skip with --no-serve) replays a Poisson stream of
 single-set requests (default 1000/s for 60 s) over 64 resident galaxies with
 cube sizes log-uniform from 64x64x32 to 256x256x128, half smdisk, half mcdisk
@@ -540,6 +546,49 @@ def smdisk_op_count(drv, model, sets):
                 sets=len(sets))
 
 
+def thick_op_count(drv, model, sets):
+    """Voxels on the disk V and channel deposits sum n_ch of the thick
+    smdisk per parameter set on its 3-D grid: V counted from the per-voxel
+    velocity output (written for on-disk voxels only), n_ch from each
+    voxel's rint channel window (gmodels.hpp:100-105, half-width 5 sigma,
+    FP32 as on the device); W = 60 V + 12 sum n_ch (means over `sets`)."""
+    import torch
+    dm, g = model.dmodel(), model.gmodel()
+    c = dm.dcube()
+    model.model_d(sets[0])                # prepares the scratch cube
+    spat_size, spat_step, spat_zero = g.spatial_grid(
+        c.scratch_size(), c.scratch_step(), c.scratch_zero())
+    nz_s, st_s, z0_s = c.scratch_size()[2], c.scratch_step()[2], \
+        c.scratch_zero()[2]
+    disk = g.components[0].disk
+    be = drv.backends().gmodel(np.float32)
+    shape = tuple(spat_size)[::-1]
+    vd = drv.mem_alloc_d(shape, np.float32)
+    dd = drv.mem_alloc_d(shape, np.float32)
+    vs, ns = [], []
+    for p in sets:
+        drv.mem_fill(vd, np.nan)
+        drv.mem_fill(dd, np.nan)
+        be.smdisk_evaluate(*disk.native_args(drv, p), None, spat_size,
+                           spat_step, spat_zero, c.rota(), nz_s, st_s, z0_s,
+                           None, None, None, None, None, None, None, None,
+                           vd, dd)
+        v = vd.cpu().numpy()
+        sd = dd.cpu().numpy()
+        on = np.isfinite(v)
+        v, sd = v[on], sd[on]
+        f32 = np.float32
+        k0 = np.maximum(np.rint((v - sd * f32(5) - f32(z0_s)) / f32(st_s)), 0)
+        k1 = np.minimum(np.rint((v + sd * f32(5) - f32(z0_s)) / f32(st_s)),
+                        nz_s - 1)
+        vs.append(int(on.sum()))
+        ns.append(int(np.maximum(k1 - k0 + 1, 0).sum()))
+    torch.cuda.synchronize()
+    v, n = float(np.mean(vs)), float(np.mean(ns))
+    return dict(v_disk=v, sum_nch=n, ops=60.0 * v + 12.0 * n,
+                sets=len(sets), grid=list(spat_size))
+
+
 def algorithmic_bytes(ev, info, live=1.0, glive=1.0, seg=1.0):
     """Compulsory HBM bytes of each kernel (or stage): (per parameter set,
     shared per launch).  The observed cube is one array read by every set
@@ -626,6 +675,204 @@ def measured_traffic(kernel):
     return None
 
 
+def batched_leg(drv, args, thick, nsets, steps, warmup, want_check):
+    """The batched workload (a thin or thick smdisk on the 128x128x96 cube):
+    `warmup` untimed steps, `steps` timed steps of `nsets` sets (graph
+    replays), then the same steps launched directly with HIP events around
+    every kernel for the per-kernel averages, their algorithmic bytes and
+    flops and the roofline of the dominant kernel; with `want_check`, the
+    GPU cubes and chi^2 of a seeded sample of sets for the per-set oracle
+    errors (computed in the CPU leg)."""
+    import torch
+    from gbkfit_amd.batch import BatchEvaluator
+    model = build_model(drv, thick)
+    arrays = workload(nsets, thick=thick)
+    # mock data: the model of the first set plus seeded noise
+    truth = param_dicts(arrays, 1)[0]
+    cube0 = model.model_h(truth)['scube']['d'].astype(np.float32)
+    rng = np.random.default_rng(2024)
+    mock = (cube0 + rng.normal(0, 0.01, cube0.shape)).astype(np.float32)
+
+    ev = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock, error=0.01,
+                        max_batch=args.chunk, template_params=truth)
+    if args.tiles and not thick:
+        ev.set_tiles(*(int(v) for v in args.tiles.split(',')))
+    use_graph = not args.no_graph
+
+    def step():
+        host = ev.packer.pack_arrays(arrays, nsets)
+        ev.upload_packed(host)
+        ev.run(nsets, use_graph=use_graph)
+        return ev._chi2_dev.cpu().numpy()
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        chi2 = step()
+        lat.append(time.perf_counter() - ts)
+    torch.cuda.synchronize()
+    t_batched = time.perf_counter() - t0
+    evals = nsets * steps
+    value = evals / t_batched
+
+    # second timed region: the same steps launched directly with HIP events
+    # around each stage (events on the kernels' own stream)
+    ev.set_timing(True)
+    step()
+    torch.cuda.synchronize()
+    stage_ms = np.zeros(3)
+    kern_ms = {}
+    chunks = 0
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        chi2_t = step()
+        ms, chunks = ev.stage_times()
+        stage_ms += ms
+        for k, v in ev.kernel_times().items():
+            kern_ms[k] = kern_ms.get(k, 0.0) + v
+    torch.cuda.synchronize()
+    t_timed = time.perf_counter() - t1
+    ev.set_timing(False)
+    # the two regions give the same chi^2, bit for bit
+    assert np.array_equal(chi2_t, chi2)
+    info = ev.info()
+
+    # roofline of the dominant kernel: its algorithmic bytes per launch (the
+    # compulsory reads and writes of its inputs and outputs for the sets of
+    # one chunk) over its average launch time from the HIP events above
+    launches = steps * max(chunks, 1)
+    per_chunk = nsets / max(chunks, 1)
+    # the last timed step ran one lane: its tile statistics are the last
+    # chunk's (a chunk of sets of the workload)
+    live_tiles = live_groups = live_blocks = None
+    if info['pipeline'] == 'direct':
+        (nzt, tot), (gz, gt), (bz, bt) = ev.tile_stats()
+        live_tiles = nzt / tot if tot else None
+        live_groups = gz / gt if gt else None
+        live_blocks = bz / bt if bt else None
+    live = live_tiles if live_tiles is not None else 1.0
+    glive = live_groups if live_groups is not None else 1.0
+    blive = live_blocks if live_blocks is not None else 1.0
+    seg = None
+    if info['pipeline'] == 'direct' and not ev.cube_source:
+        sl, st = ev.line_stats()
+        seg = sl / st if st else None
+    algo, shared = algorithmic_bytes(ev, info, live, glive,
+                                     seg if seg is not None else 1.0)
+    aflops = algorithmic_flops(ev, info, live, glive)
+    algo_dense, _ = algorithmic_bytes(ev, info)
+    flops_dense = algorithmic_flops(ev, info)
+    kernels = {}
+    for k, t in kern_ms.items():
+        avg_ms = t / launches
+        b = algo.get(k)
+        kernels[k] = dict(avg_ms_per_launch=avg_ms)
+        if b is not None and avg_ms > 0:
+            per_launch = b * per_chunk + shared.get(k, 0)
+            gbs = per_launch / (avg_ms * 1e-3) / 1e9
+            kernels[k].update(bytes_per_set=b, bytes_per_launch=per_launch,
+                              achieved_gbs=gbs, frac=gbs / HBM_PEAK_GBS)
+            if k in shared:
+                kernels[k]['shared_bytes_per_launch'] = shared[k]
+        if k in algo_dense and algo_dense[k] != b:
+            kernels[k]['dense_bytes_per_set'] = algo_dense[k]
+        if k in flops_dense and flops_dense[k] != aflops.get(k):
+            kernels[k]['dense_flops_per_set'] = flops_dense[k]
+        f = aflops.get(k)
+        if f is not None and avg_ms > 0:
+            tfs = f * per_chunk / (avg_ms * 1e-3) / 1e12
+            kernels[k].update(flops_per_set=f, achieved_tflops=tfs,
+                              frac_fp32=tfs / FP32_PEAK_TFLOPS)
+    dom = max(kern_ms, key=kern_ms.get)
+    dk = kernels[dom]
+    traffic = measured_traffic(dom)
+    # op roofline of the smdisk work: per-spaxel geometry and traits in
+    # batch_lines_thin, the channel deposits in the x pass (which also runs
+    # the x filter, so this understates the deposit's own rate)
+    smdisk = None
+    if info['pipeline'] == 'direct' and thick:
+        smdisk = thick_op_count(drv, model, param_dicts(arrays, 4))
+        t = kernels.get('model_deposit', {}).get('avg_ms_per_launch', 0)
+        if t > 0:
+            tfs = smdisk['ops'] * per_chunk / (t * 1e-3) / 1e12
+            smdisk.update(kernels='batch_smdisk_thick (model_deposit)',
+                          achieved_tflops=tfs, frac=tfs / FP32_PEAK_TFLOPS,
+                          note='W = 60 V + 12 sum n_ch FP32 ops per set (a '
+                               'transcendental counts 4), V the voxels of '
+                               'the 3-D grid on the disk and n_ch their '
+                               'channel windows, counted on 4 sets of the '
+                               'workload; the time is the thick deposit '
+                               'kernel per launch')
+    if info['pipeline'] == 'direct' and not thick:
+        smdisk = smdisk_op_count(drv, model, param_dicts(arrays, 16))
+        t_ms = sum(kernels[k]['avg_ms_per_launch']
+                   for k in ('batch_lines_thin', 'sep_xpass') if k in kernels)
+        if t_ms > 0:
+            tfs = smdisk['ops'] * per_chunk / (t_ms * 1e-3) / 1e12
+            xf = aflops.get('sep_xpass', 0.0)
+            both = (smdisk['ops'] + xf) * per_chunk / (t_ms * 1e-3) / 1e12
+            smdisk.update(
+                kernels='batch_lines_thin + sep_xpass',
+                achieved_tflops=tfs, frac=tfs / FP32_PEAK_TFLOPS,
+                with_x_filter_tflops=both,
+                with_x_filter_frac=both / FP32_PEAK_TFLOPS,
+                note='W = 60 V + 12 sum n_ch FP32 ops per set (a '
+                     'transcendental counts 4), V and n_ch counted on the '
+                     'hi-res cube of 16 sets of the workload; the time is '
+                     'both kernels per launch, the x pass also '
+                     'filtering (with_x_filter adds its useful flops)')
+    # the binding roof of the dominant kernel: FP32 compute when its flop
+    # rate is the larger fraction of peak, else HBM bandwidth
+    if dk.get('frac_fp32', 0) > dk.get('frac', 0):
+        roofline = dict(bound='mfma', achieved=dk['achieved_tflops'],
+                        peak=FP32_PEAK_TFLOPS, unit='TFLOP/s',
+                        frac=dk['frac_fp32'], traffic=traffic, kernel=dom,
+                        note='FP32 multiply-adds on the vector ALUs '
+                             '(packed FP32 disabled: at most half of the '
+                             'FP32 peak is reachable); HBM: %.0f GB/s'
+                             % dk['achieved_gbs'])
+    else:
+        achieved = dk.get('achieved_gbs')
+        roofline = dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS,
+                        unit='GB/s',
+                        frac=(achieved / HBM_PEAK_GBS) if achieved else None,
+                        traffic=traffic, kernel=dom)
+    if live_tiles is not None:
+        roofline['live_tile_fraction'] = live_tiles
+        roofline['live_group_fraction'] = live_groups
+        roofline['live_zblock_fraction'] = live_blocks
+        roofline['live_line_fraction'] = seg
+        roofline['note'] = (roofline.get('note', '') + '; ' if
+                            'note' in roofline else '') + (
+            'achieved counts the work actually done: the x-pass tiles some '
+            'line reaches (live_tile_fraction), the y-pass output groups '
+            'and z-pass blocks with a nonzero input (live_group_fraction, '
+            'live_zblock_fraction), the line parameters of row segments '
+            'some line reaches (live_line_fraction); the rest is zero and '
+            'skipped; the observed cube counts once per launch')
+
+    check = None
+    if want_check:
+        pick = np.sort(np.random.default_rng(7).choice(
+            nsets, size=min(4, nsets), replace=False))
+        csets = [param_dicts(arrays, nsets)[i] for i in pick]
+        cc, cb = ev.evaluate(csets, want_cubes=True, use_graph=False)
+        check = dict(index=pick, sets=csets, cubes=cb, chi2=cc, mock=mock,
+                     same_as_step=bool(np.array_equal(cc, chi2[pick])))
+    size_hi = list(ev.size_hi)
+    ev.close()
+    return dict(model=model, arrays=arrays, mock=mock, value=value,
+                t_batched=t_batched, lat=lat, chi2=chi2, evals=evals,
+                t_timed=t_timed, stage_ms=stage_ms, kernels=kernels,
+                smdisk=smdisk, roofline=roofline, per_chunk=per_chunk,
+                info=info, use_graph=use_graph, check=check,
+                size_hi=size_hi)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -637,7 +884,10 @@ def main():
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-graph', action='store_true')
-    ap.add_argument('--thick', action='store_true')
+    ap.add_argument('--thick', action='store_true',
+                    help='the thick disk as the headline workload')
+    ap.add_argument('--no-thick', action='store_true',
+                    help='skip the thick variant leg')
     ap.add_argument('--serve-galaxies', type=int, default=64)
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument('--serve-rate', type=float, default=1000.0)
@@ -692,66 +942,15 @@ def main():
         sys.exit(2)
 
     import torch
-    from gbkfit_amd.batch import BatchEvaluator
     from gbkfit_amd.driver.hip import DriverHip
     from gbkfit_amd.objective import Data, Objective
 
     drv = DriverHip()
-    model = build_model(drv, args.thick)
-    arrays = workload(args.nsets, thick=args.thick)
-    # mock data: the model of the first set plus seeded noise
-    truth = param_dicts(arrays, 1)[0]
-    cube0 = model.model_h(truth)['scube']['d'].astype(np.float32)
-    rng = np.random.default_rng(2024)
-    mock = (cube0 + rng.normal(0, 0.01, cube0.shape)).astype(np.float32)
-
-    ev = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock, error=0.01,
-                        max_batch=args.chunk, template_params=truth)
-    if args.tiles:
-        ev.set_tiles(*(int(v) for v in args.tiles.split(',')))
-    use_graph = not args.no_graph
-
-    def step():
-        host = ev.packer.pack_arrays(arrays, args.nsets)
-        ev.upload_packed(host)
-        ev.run(args.nsets, use_graph=use_graph)
-        return ev._chi2_dev.cpu().numpy()
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    lat = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        chi2 = step()
-        lat.append(time.perf_counter() - ts)
-    torch.cuda.synchronize()
-    t_batched = time.perf_counter() - t0
-    evals = args.nsets * args.steps
-    value = evals / t_batched
-
-    # second timed region: the same steps launched directly with HIP events
-    # around each stage (events on the kernels' own stream)
-    ev.set_timing(True)
-    step()
-    torch.cuda.synchronize()
-    stage_ms = np.zeros(3)
-    kern_ms = {}
-    chunks = 0
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        chi2_t = step()
-        ms, chunks = ev.stage_times()
-        stage_ms += ms
-        for k, v in ev.kernel_times().items():
-            kern_ms[k] = kern_ms.get(k, 0.0) + v
-    torch.cuda.synchronize()
-    t_timed = time.perf_counter() - t1
-    ev.set_timing(False)
-    # the two regions give the same chi^2, bit for bit
-    assert np.array_equal(chi2_t, chi2)
-    info = ev.info()
+    head = batched_leg(drv, args, args.thick, args.nsets, args.steps,
+                       args.warmup, not args.no_cpu)
+    model, arrays, mock = head['model'], head['arrays'], head['mock']
+    value, t_batched, chi2 = head['value'], head['t_batched'], head['chi2']
+    evals, info = head['evals'], head['info']
 
     # plain drop-in path, one parameter set per call
     obj = Objective([dict(scube=Data(mock, error=np.full(mock.shape, 0.01,
@@ -771,120 +970,30 @@ def main():
     t_plain = time.perf_counter() - tp
     plain_value = len(plain_sets) / t_plain
 
-    # roofline of the dominant kernel: its algorithmic bytes per launch (the
-    # compulsory reads and writes of its inputs and outputs for the sets of
-    # one chunk) over its average launch time from the HIP events above
-    launches = args.steps * max(chunks, 1)
-    per_chunk = args.nsets / max(chunks, 1)
-    # the last timed step ran one lane: its tile statistics are the last
-    # chunk's (a chunk of sets of the workload)
-    live_tiles = live_groups = live_blocks = None
-    if info['pipeline'] == 'direct':
-        (nzt, tot), (gz, gt), (bz, bt) = ev.tile_stats()
-        live_tiles = nzt / tot if tot else None
-        live_groups = gz / gt if gt else None
-        live_blocks = bz / bt if bt else None
-    live = live_tiles if live_tiles is not None else 1.0
-    glive = live_groups if live_groups is not None else 1.0
-    blive = live_blocks if live_blocks is not None else 1.0
-    seg = None
-    if info['pipeline'] == 'direct' and not ev.cube_source:
-        sl, st = ev.line_stats()
-        seg = sl / st if st else None
-    algo, shared = algorithmic_bytes(ev, info, live, glive,
-                                     seg if seg is not None else 1.0)
-    aflops = algorithmic_flops(ev, info, live, glive)
-    algo_dense, _ = algorithmic_bytes(ev, info)
-    flops_dense = algorithmic_flops(ev, info)
-    kernels = {}
-    for k, t in kern_ms.items():
-        avg_ms = t / launches
-        b = algo.get(k)
-        kernels[k] = dict(avg_ms_per_launch=avg_ms)
-        if b is not None and avg_ms > 0:
-            per_launch = b * per_chunk + shared.get(k, 0)
-            gbs = per_launch / (avg_ms * 1e-3) / 1e9
-            kernels[k].update(bytes_per_set=b, bytes_per_launch=per_launch,
-                              achieved_gbs=gbs, frac=gbs / HBM_PEAK_GBS)
-            if k in shared:
-                kernels[k]['shared_bytes_per_launch'] = shared[k]
-        if k in algo_dense and algo_dense[k] != b:
-            kernels[k]['dense_bytes_per_set'] = algo_dense[k]
-        if k in flops_dense and flops_dense[k] != aflops.get(k):
-            kernels[k]['dense_flops_per_set'] = flops_dense[k]
-        f = aflops.get(k)
-        if f is not None and avg_ms > 0:
-            tfs = f * per_chunk / (avg_ms * 1e-3) / 1e12
-            kernels[k].update(flops_per_set=f, achieved_tflops=tfs,
-                              frac_fp32=tfs / FP32_PEAK_TFLOPS)
-    dom = max(kern_ms, key=kern_ms.get)
-    dk = kernels[dom]
-    traffic = measured_traffic(dom)
-    # op roofline of the smdisk work: per-spaxel geometry and traits in
-    # batch_lines_thin, the channel deposits in the x pass (which also runs
-    # the x filter, so this understates the deposit's own rate)
-    smdisk = None
-    if info['pipeline'] == 'direct' and not args.thick:
-        smdisk = smdisk_op_count(drv, model, param_dicts(arrays, 16))
-        t_ms = sum(kernels[k]['avg_ms_per_launch']
-                   for k in ('batch_lines_thin', 'sep_xpass') if k in kernels)
-        if t_ms > 0:
-            tfs = smdisk['ops'] * per_chunk / (t_ms * 1e-3) / 1e12
-            xf = aflops.get('sep_xpass', 0.0)
-            both = (smdisk['ops'] + xf) * per_chunk / (t_ms * 1e-3) / 1e12
-            smdisk.update(
-                kernels='batch_lines_thin + sep_xpass',
-                achieved_tflops=tfs, frac=tfs / FP32_PEAK_TFLOPS,
-                with_x_filter_tflops=both,
-                with_x_filter_frac=both / FP32_PEAK_TFLOPS,
-                note='W = 60 V + 12 sum n_ch FP32 ops per set (a '
-                     'transcendental counts 4), V and n_ch counted on the '
-                     'hi-res cube of 16 sets of the workload; the time is '
-                     'both kernels per launch, the x pass also '
-                     'filtering (with_x_filter adds its useful flops)')
-    # the binding roof of the dominant kernel: FP32 compute when its flop
-    # rate is the larger fraction of peak, else HBM bandwidth
-    if dk.get('frac_fp32', 0) > dk.get('frac', 0):
-        roofline = dict(bound='mfma', achieved=dk['achieved_tflops'],
-                        peak=FP32_PEAK_TFLOPS, unit='TFLOP/s',
-                        frac=dk['frac_fp32'], traffic=traffic, kernel=dom,
-                        note='FP32 multiply-adds on the vector ALUs '
-                             '(packed FP32 disabled: at most half of the '
-                             'FP32 peak is reachable); HBM: %.0f GB/s'
-                             % dk['achieved_gbs'])
-    else:
-        achieved = dk.get('achieved_gbs')
-        roofline = dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS,
-                        unit='GB/s',
-                        frac=(achieved / HBM_PEAK_GBS) if achieved else None,
-                        traffic=traffic, kernel=dom)
-    if live_tiles is not None:
-        roofline['live_tile_fraction'] = live_tiles
-        roofline['live_group_fraction'] = live_groups
-        roofline['live_zblock_fraction'] = live_blocks
-        roofline['live_line_fraction'] = seg
-        roofline['note'] = (roofline.get('note', '') + '; ' if
-                            'note' in roofline else '') + (
-            'achieved counts the work actually done: the x-pass tiles some '
-            'line reaches (live_tile_fraction), the y-pass output groups '
-            'and z-pass blocks with a nonzero input (live_group_fraction, '
-            'live_zblock_fraction), the line parameters of row segments '
-            'some line reaches (live_line_fraction); the rest is zero and '
-            'skipped; the observed cube counts once per launch')
 
-    # a seeded sample of the step's sets for the per-set oracle errors
-    # (checked in the CPU-baseline leg)
-    check = None
-    if not args.no_cpu:
-        pick = np.sort(np.random.default_rng(7).choice(
-            args.nsets, size=min(4, args.nsets), replace=False))
-        csets = [param_dicts(arrays, args.nsets)[i] for i in pick]
-        cc, cb = ev.evaluate(csets, want_cubes=True, use_graph=False)
-        check = dict(index=pick, sets=csets, cubes=cb, chi2=cc, mock=mock,
-                     same_as_step=bool(np.array_equal(cc, chi2[pick])))
+    # the thick variant of the same workload (sech^2 height, spatial z 64):
+    # a secondary object with its own kernel times, roofline and op count
+    thick_variant = None
+    if not args.thick and not args.no_thick:
+        tsteps = max(2, args.steps // 2)
+        tv = batched_leg(drv, args, True, args.nsets, tsteps, 1, False)
+        thick_variant = dict(
+            value=tv['value'], unit='evals/s', steps=tsteps,
+            ms_per_step=1e3 * tv['t_batched'] / tsteps,
+            workload='smdisk thick (sech^2 height 2 px, spatial z 64) '
+                     '128x128x96, 2x spatial supersampling, PSF FWHM 3 px + '
+                     'LSF FWHM 2 ch, chi^2 vs seeded mock; '
+                     f'{args.nsets} parameter sets per step',
+            stages_ms_per_step=dict(
+                model=tv['stage_ms'][0] / tsteps,
+                convolution=tv['stage_ms'][1] / tsteps,
+                downscale_chi2=tv['stage_ms'][2] / tsteps),
+            kernels=tv['kernels'], smdisk_ops=tv['smdisk'],
+            roofline=dict(tv['roofline'], sets_per_launch=tv['per_chunk']))
+        print(f"thick leg done: {tv['value']:.0f} evals/s", file=sys.stderr,
+              flush=True)
 
     serving = None
-    ev.close()
     if not args.no_serve:
         serving = serve_benchmark(drv, args.serve_galaxies,
                                   args.serve_seconds, args.serve_rate)
@@ -910,6 +1019,7 @@ def main():
         except Exception as e:  # the harness may be absent
             cpu = dict(error=str(e))
         try:
+            check = head['check']
             oracle_err = dict(sets=oracle_errors(check, args.thick),
                               chi2_equal_to_the_step=check['same_as_step'])
         except Exception as e:  # the oracle library may be absent
@@ -924,8 +1034,8 @@ def main():
             f"smdisk {'thick' if args.thick else 'thin'} 128x128x96, 2x "
             f"spatial supersampling, PSF FWHM 3 px + LSF FWHM 2 ch, chi^2 vs "
             f"seeded mock; {args.nsets} parameter sets per step"),
-            nsets=args.nsets, chunk=args.chunk, graph=use_graph,
-            padding='reference', dcube_size_hi=list(ev.size_hi),
+            nsets=args.nsets, chunk=args.chunk, graph=head['use_graph'],
+            padding='reference', dcube_size_hi=head['size_hi'],
             pipeline=dict(direct='direct separable convolution',
                           fused='spectral-first fused FFT',
                           padded='padded-cube FFT')[info['pipeline']],
@@ -933,15 +1043,17 @@ def main():
         plain=dict(value=plain_value, unit='evals/s',
                    sets=len(plain_sets), **percentiles(plat)),
         batched_over_plain=value / plain_value,
-        batch_latency=percentiles(lat),
+        batch_latency=percentiles(head['lat']),
         per_eval_latency_batched_ms=1e3 * t_batched / evals,
-        direct_launch_evals_per_s=evals / t_timed,
-        stages_ms_per_step=dict(model=stage_ms[0] / args.steps,
-                                convolution=stage_ms[1] / args.steps,
-                                downscale_chi2=stage_ms[2] / args.steps),
-        kernels=kernels,
-        smdisk_ops=smdisk,
-        roofline=dict(roofline, sets_per_launch=per_chunk),
+        direct_launch_evals_per_s=evals / head['t_timed'],
+        stages_ms_per_step=dict(
+            model=head['stage_ms'][0] / args.steps,
+            convolution=head['stage_ms'][1] / args.steps,
+            downscale_chi2=head['stage_ms'][2] / args.steps),
+        kernels=head['kernels'],
+        smdisk_ops=head['smdisk'],
+        roofline=dict(head['roofline'], sets_per_launch=head['per_chunk']),
+        thick_variant=thick_variant,
         serving=serving,
         ragged=ragged,
         mcdisk_plain=mcdisk,
