@@ -298,6 +298,11 @@ int mc_tile_width(int nz)
 size_t mc_bufs(char* base, int nsets, long long cap, int ntiles, int tw,
                int nz, McBufs* out)
 {
+    // tile numbers are the wave ranks' sort keys
+    GBK_CHECK(ntiles < kMcNoKey, GBKHIP_ERR_UNSUPPORTED,
+              "binned mcdisk deposit: " + std::to_string(ntiles) +
+              " spatial tiles, beyond its limit of " +
+              std::to_string(kMcNoKey - 1));
     McBufs b;
     b.cap = cap;
     b.ntiles = ntiles;

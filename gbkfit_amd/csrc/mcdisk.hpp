@@ -131,37 +131,38 @@ struct McBin {
 // of the wave must call it (inactive lanes pass key = -1).  Same-address
 // atomics serialise; this takes one per distinct key and wave, all in one
 // instruction.
+// Keys must be below kMcNoKey (the host checks the tile count): a key and
+// its lane are packed into one int, key * 64 + lane, so the sort moves one
+// value per compare-exchange and every packed value is distinct.
+constexpr int kMcNoKey = (1 << 25) - 1;
+
 __device__ __forceinline__ int wave_rank(int key, int* counters, int* wscr)
 {
     const int lane = __lane_id();
-    int k = key < 0 ? 0x7fffffff : key, idx = lane;
+    int kk = ((key < 0 ? kMcNoKey : key) << 6) | lane;
 #pragma unroll
     for (int size = 2; size <= 64; size <<= 1) {
 #pragma unroll
         for (int j = size >> 1; j > 0; j >>= 1) {
-            const int pk = __shfl_xor(k, j, 64), pi = __shfl_xor(idx, j, 64);
+            const int pk = __shfl_xor(kk, j, 64);
             const bool up = (lane & size) == 0, lower = (lane & j) == 0;
-            const bool pless = pk < k || (pk == k && pi < idx);
-            if (lower == up ? pless : !pless) {
-                k = pk;
-                idx = pi;
-            }
+            kk = lower == up ? min(kk, pk) : max(kk, pk);
         }
     }
-    // runs of equal keys: start lane by an inclusive max-scan of head lanes
+    const int k = kk >> 6, idx = kk & 63;
+    // runs of equal keys
     const int prev = __shfl_up(k, 1, 64), next = __shfl_down(k, 1, 64);
     const bool head = lane == 0 || prev != k;
     const bool tail = lane == 63 || next != k;
-    int start = head ? lane : 0;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int v = __shfl_up(start, o, 64);
-        if (lane >= o) start = max(start, v);
-    }
-    if (tail && k != 0x7fffffff)
+    // the run's head: the highest head lane at or below this one
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = lane == 63 ? ~0ull
+                                               : (2ull << lane) - 1ull;
+    const int start = 63 - __clzll((long long)(heads & upto));
+    if (tail && k != kMcNoKey)
         wscr[start] = atomicAdd(&counters[k], lane - start + 1);
     __builtin_amdgcn_wave_barrier();
-    const int rank = k != 0x7fffffff ? wscr[start] + lane - start : 0;
+    const int rank = k != kMcNoKey ? wscr[start] + lane - start : 0;
     __builtin_amdgcn_wave_barrier();
     wscr[64 + idx] = rank;
     __builtin_amdgcn_wave_barrier();
@@ -284,9 +285,19 @@ __device__ __forceinline__ int mc_cloud(long long ci, const McParams& mp,
     S.cid = (uint32_t)ci;
     S.block = 0xffffffffu;
     S.next = 0;
-    // ring of the cloud (gmodels.hpp:401-416)
+    // ring of the cloud (gmodels.hpp:401-416): the first ring whose
+    // cumulative count exceeds the cloud id.  The reference scans the
+    // counts linearly; they are non-decreasing, so the count of entries
+    // <= ci, found by a branch-free binary search (the same trip count on
+    // every lane), is the same index.
     int rnidx = 0, tidx;
-    while (rnidx < mp.csum_len && ci >= mp.csum[rnidx]) rnidx++;
+    {
+        const int len = mp.csum_len;
+        int step = len > 0 ? 1 << (31 - __clz(len)) : 0;
+        for (; step > 0; step >>= 1)
+            if (rnidx + step <= len && mp.csum[rnidx + step - 1] <= ci)
+                rnidx += step;
+    }
     if (rnidx >= mp.csum_len) return -1;     // cloud beyond the budget
     const float* rpc = d.rp.cvalues;
     const float* rpp = d.rp.pvalues;

@@ -151,7 +151,9 @@ int gbkhip_smdisk_evaluate(gbkhip_ctx* ctx, const gbkhip_disk* disk,
  * Philox4x32-10 stream keyed by (seed, cloud id), so results do not depend
  * on launch shape; the spectral cube is accumulated in int64 fixed point and
  * converted once, so reruns are bit-identical.  ncloudscsum and hasordint
- * are device arrays (ncloudscsum_len entries / rp.n entries). */
+ * are device arrays (ncloudscsum_len entries / rp.n entries); ncloudscsum
+ * holds cumulative counts, so it is non-decreasing (a cloud's ring is found
+ * by binary search). */
 int gbkhip_mcdisk_evaluate(gbkhip_ctx* ctx, float cflux, int nclouds,
                            const int* ncloudscsum, int ncloudscsum_len,
                            const unsigned char* hasordint,
@@ -268,10 +270,11 @@ int gbkhip_batch_evaluate(gbkhip_batch* batch, const float* params,
                           int nsets, double* chi2, float* cubes,
                           int use_graph);
 /* evaluate() of an mcdisk batch.  csum (device, nsets x csum_len int32):
- * each set's cumulative cloud counts per sub-ring, the integral of its
- * density traits over cflux (_mcdisk.py:16-189; the host computes it, as the
- * reference does); qscale (device, nsets FP64): each set's fixed-point scale
- * (gbkhip_mcdisk_fixed_scale of its total cloud count).  A set's result
+ * each set's cumulative (non-decreasing) cloud counts per sub-ring, the
+ * integral of its density traits over cflux (_mcdisk.py:16-189; the host
+ * computes it, as the reference does); qscale (device, nsets FP64): each
+ * set's fixed-point scale (gbkhip_mcdisk_fixed_scale of its total cloud
+ * count).  A set's result
  * depends only on its own parameters and counts. */
 int gbkhip_batch_evaluate_clouds(gbkhip_batch* batch, const float* params,
                                  const int* csum, const double* qscale,
