@@ -259,17 +259,28 @@ class BatchEvaluator:
             self._pcube = driver.mem_copy_h2d(
                 cube.make_pcube(size_hi, step_hi, be_fft))
             d.pcube = self._pcube.data_ptr()
-        self._obs = driver.mem_copy_h2d(np.ascontiguousarray(obs, np.float32))
+        obs = np.array(obs, np.float32)
+        err = None if error is None or np.isscalar(error) else \
+            np.array(error, np.float32)
+        if mask is not None:
+            # masked pixels contribute a zero residual even where their data
+            # or error hold the NaN that marks them (the objective's
+            # residual kernel does the same): zero data, unit error there
+            mask = np.asarray(mask, np.float32)
+            obs[mask == 0] = 0.0
+            if err is not None:
+                err[mask == 0] = 1.0
+        self._obs = driver.mem_copy_h2d(obs)
         d.obs = self._obs.data_ptr()
-        if error is None or np.isscalar(error):
+        if err is None:
             d.err = None
             d.err_scalar = 1.0 if error is None else float(error)
         else:
-            self._err = driver.mem_copy_h2d(np.asarray(error, np.float32))
+            self._err = driver.mem_copy_h2d(err)
             d.err = self._err.data_ptr()
             d.err_scalar = 1.0
         if mask is not None:
-            self._mask = driver.mem_copy_h2d(np.asarray(mask, np.float32))
+            self._mask = driver.mem_copy_h2d(mask)
             d.mask = self._mask.data_ptr()
         d.weight = float(weight)
         d.max_batch = int(max_batch)

@@ -25,8 +25,11 @@ __global__ void residual_kernel(const float* __restrict__ od,
         float r = md[i] - od[i];
         if (oe) r /= oe[i];
         if (mw) r *= mw[i];
-        if (om) r *= om[i];
-        if (mm) r *= mm[i];
+        // a zero mask entry gives a zero residual even where the data (or
+        // the model, after a mask with apply) hold the NaN that marks the
+        // pixel as masked; the reference multiplies, NaN * 0 = NaN
+        if (om) r = om[i] != 0.0f ? r * om[i] : 0.0f;
+        if (mm) r = mm[i] != 0.0f ? r * mm[i] : 0.0f;
         res[i] = r * weight;
     }
 }

@@ -301,6 +301,9 @@ class DModelMMaps(DModel):
     def step(self):
         return self._dcube.step()[:2]
 
+    def zero(self):
+        return self._dcube.zero()[:2]
+
     def dtype(self):
         return self._dcube.dtype()
 

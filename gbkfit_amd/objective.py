@@ -58,11 +58,22 @@ class Data:
 
 class Objective:
     """Residuals and likelihood of `models` against `datasets` (one dict of
-    Data per model, keyed like the data model's keys())."""
+    Data per model, keyed like the data model's keys(), or one
+    gbkfit_amd.dataset.Dataset per model, checked against its data model as
+    the reference does: keys, dtype, size, step and zero,
+    objective.py:79-110)."""
 
     def __init__(self, datasets, models, weights=None):
+        from .dataset import Dataset, check_compatible
         self._datasets = list(datasets)
         self._models = list(models)
+        if len(self._datasets) != len(self._models):
+            raise RuntimeError(
+                f"the number of datasets and models are not equal "
+                f"({len(self._datasets)} != {len(self._models)})")
+        for i, (ds, model) in enumerate(zip(self._datasets, self._models)):
+            if isinstance(ds, Dataset):
+                check_compatible(ds, model.dmodel(), i)
         self._weights = weights or [dict() for _ in self._models]
         self._prepared = False
 

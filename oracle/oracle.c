@@ -1039,8 +1039,10 @@ void or_residual(const double* obs_d, const double* obs_e, const double* obs_m,
         double r = mdl_d[i] - obs_d[i];
         if (obs_e) r /= obs_e[i];
         if (mdl_w) r *= mdl_w[i];
-        if (obs_m) r *= obs_m[i];
-        if (mdl_m) r *= mdl_m[i];
+        /* a zero mask entry gives a zero residual, also where the masked
+           value is NaN (the reference multiplies: NaN * 0 = NaN) */
+        if (obs_m) r = obs_m[i] != 0.0 ? r * obs_m[i] : 0.0;
+        if (mdl_m) r = mdl_m[i] != 0.0 ? r * mdl_m[i] : 0.0;
         res[i] = r * weight;
     }
 }

@@ -317,10 +317,15 @@ def test_residual_and_sum(drv):
     om = (rng.uniform(0, 1, n) > 0.2).astype(np.float32)
     mw = rng.uniform(0, 1, n).astype(np.float32)
     mm = (rng.uniform(0, 1, n) > 0.1).astype(np.float32)
+    # masked pixels carry NaN data / NaN model values: zero residual there
+    arrs[0][om == 0] = np.nan
+    arrs[1][:50][mm[:50] == 0] = np.nan
     res = drv.mem_alloc_d(n, np.float32)
     be.residual(H.dev(drv, arrs[0]), H.dev(drv, oe), H.dev(drv, om),
                 H.dev(drv, arrs[1]), H.dev(drv, mw), H.dev(drv, mm), 0.7, res)
     ref = oracle.residual(arrs[0], oe, om, arrs[1], mw, mm, 0.7)
+    assert np.all(np.isfinite(ref)) and np.all(ref[om == 0] == 0)
+    assert np.array_equal(res.cpu().numpy() == 0, ref == 0)
     assert H.normwise(res.cpu().numpy(), ref) < 1e-6
     s = drv.mem_alloc_d(1, np.float32)
     be.residual_sum(res, True, s)
