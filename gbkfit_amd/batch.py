@@ -78,6 +78,10 @@ N.register_signatures({
                                    _c.c_int, _c.POINTER(_c.c_int)], _c.c_int),
     'gbkhip_batch_set_timing': ([_c.c_void_p, _c.c_int], _c.c_int),
     'gbkhip_batch_set_stream': ([_c.c_void_p, _c.c_void_p], _c.c_int),
+    'gbkhip_batch_set_mmaps': ([_c.c_void_p, _c.c_int, _c.c_void_p,
+                                _c.c_float, _c.c_float, _c.c_float,
+                                _c.c_void_p, _c.c_void_p, _c.c_void_p,
+                                _c.c_void_p, _c.c_void_p], _c.c_int),
     'gbkhip_batch_release': ([_c.c_void_p], _c.c_int),
     'gbkhip_batch_memory': ([_c.c_void_p, _c.POINTER(_c.c_ulonglong),
                              _c.POINTER(_c.c_ulonglong),
@@ -187,13 +191,21 @@ class BatchEvaluator:
             padded-cube pipeline from a deposited hi-res cube; mcdisk: each
             set's cloud budget computed on the host as the reference does,
             src/gbkfit/model/gmodels/_mcdisk.py:16-189);
-    obs:    observed low-res cube (numpy, z-y-x); error: scalar or cube.
+    obs:    observed low-res cube (numpy, z-y-x); error: scalar or cube;
+            mask: cube or None.  With a DModelMMaps data model the objective
+            is its moment maps (mmaps.py:136-201, objective.py:279-308):
+            obs maps each key 'mmap<order>' to its (y, x) data map (a dict,
+            or a gbkfit_amd.dataset.DatasetMMaps whose items also give the
+            errors and masks), error is a scalar or a dict of scalars or
+            maps, mask a dict of maps or None, weight a scalar or a dict of
+            per-key weights.
     """
 
     def __init__(self, driver, dmodel, gmodel, obs, error=None, mask=None,
                  weight=1.0, max_batch=64, template_params=None,
                  path='auto'):
         self.driver = driver
+        self.mmaps = type(dmodel).__name__ == 'DModelMMaps'
         comps = gmodel.components
         if len(comps) != 1 or not hasattr(comps[0].disk, 'traits') or \
                 type(comps[0].disk).__name__ not in ('SMDisk', 'MCDisk'):
@@ -259,7 +271,11 @@ class BatchEvaluator:
             self._pcube = driver.mem_copy_h2d(
                 cube.make_pcube(size_hi, step_hi, be_fft))
             d.pcube = self._pcube.data_ptr()
-        obs = np.array(obs, np.float32)
+        if self.mmaps:
+            mm_args = self._mmap_data(dmodel, obs, error, mask, weight)
+            obs, error, mask, weight = None, None, None, 1.0
+        else:
+            obs = np.array(obs, np.float32)
         err = None if error is None or np.isscalar(error) else \
             np.array(error, np.float32)
         if mask is not None:
@@ -270,8 +286,9 @@ class BatchEvaluator:
             obs[mask == 0] = 0.0
             if err is not None:
                 err[mask == 0] = 1.0
-        self._obs = driver.mem_copy_h2d(obs)
-        d.obs = self._obs.data_ptr()
+        if obs is not None:
+            self._obs = driver.mem_copy_h2d(obs)
+            d.obs = self._obs.data_ptr()
         if err is None:
             d.err = None
             d.err_scalar = 1.0 if error is None else float(error)
@@ -308,6 +325,8 @@ class BatchEvaluator:
         N.call('gbkhip_batch_create', self._ctx.handle,
                _c.byref(d), _c.byref(h))
         self.handle = h
+        if self.mmaps:
+            self._set_mmaps(*mm_args)
         self._mc_cap = 0
         if self.mc:   # capacity for the template's budget, with headroom
             self.reserve_clouds(int(1.5 * max(int(csum0[-1]), 1)))
@@ -315,6 +334,81 @@ class BatchEvaluator:
         self._chi2_dev = None
         self._cubes_dev = None
         del t, dev
+
+    def _mmap_data(self, dmodel, obs, error, mask, weight):
+        """Stacked (norders, y, x) data, error and mask maps of the
+        moment-map objective, in the data model's key order."""
+        keys = list(dmodel.keys())
+        ny, nx = dmodel.size()[::-1]
+
+        def item(src, key):
+            if src is None:
+                return None
+            if isinstance(src, dict):
+                return src.get(key)
+            return src[key] if key in src else None
+
+        d_maps, e_maps, m_maps, e_sc, w = [], [], [], [], []
+        for key in keys:
+            it = item(obs, key)
+            if it is None:
+                raise RuntimeError(f"no data map for '{key}'")
+            if hasattr(it, 'error') and hasattr(it, 'mask'):   # Data item
+                dd, ee, mm = it.data(), it.error(), it.mask()
+            else:
+                dd, ee, mm = it, None, None
+            e = error if np.isscalar(error) or error is None \
+                else item(error, key)
+            if e is None:
+                e = ee
+            m = mm if mask is None else item(mask, key)
+            dd = np.array(dd, np.float32)
+            if dd.shape != (ny, nx):
+                raise RuntimeError(f"data map '{key}' has shape {dd.shape}, "
+                                   f"not {(ny, nx)}")
+            if e is None or np.isscalar(e):
+                e_sc.append(1.0 if e is None else float(e))
+                e_maps.append(None)
+            else:
+                e_sc.append(1.0)
+                e_maps.append(np.array(e, np.float32))
+            m_maps.append(None if m is None else np.asarray(m, np.float32))
+            d_maps.append(dd)
+            w.append(float(weight if np.isscalar(weight)
+                           else weight.get(key, 1.0)))
+        any_e = any(x is not None for x in e_maps)
+        any_m = any(x is not None for x in m_maps)
+        D = np.stack(d_maps)
+        E = np.stack([x if x is not None else np.full((ny, nx), e_sc[i],
+                                                      np.float32)
+                      for i, x in enumerate(e_maps)]) if any_e else None
+        M = np.stack([x if x is not None else np.ones((ny, nx), np.float32)
+                      for x in m_maps]) if any_m else None
+        if M is not None:
+            # masked pixels contribute a zero residual even where their data
+            # or error hold the NaN that marks them
+            D[M == 0] = 0.0
+            if E is not None:
+                E[M == 0] = 1.0
+        cube = dmodel.dcube()
+        return (np.array(dmodel.orders(), np.int32), D, E,
+                np.array(e_sc, np.float32), M, np.array(w, np.float32),
+                float(dmodel._mask_cutoff), float(cube.zero()[2]),
+                float(cube.step()[2]))
+
+    def _set_mmaps(self, orders, D, E, esc, M, W, cutoff, zero_z, step_z):
+        dv = self.driver
+        self._mm_obs = dv.mem_copy_h2d(D)
+        self._mm_err = None if E is None else dv.mem_copy_h2d(E)
+        self._mm_mask = None if M is None else dv.mem_copy_h2d(M)
+        self._mm_host = (orders, esc, W)
+        N.call('gbkhip_batch_set_mmaps', self.handle, len(orders),
+               orders.ctypes.data, _c.c_float(cutoff), _c.c_float(zero_z),
+               _c.c_float(step_z), self._mm_obs.data_ptr(),
+               None if E is None else self._mm_err.data_ptr(),
+               esc.ctypes.data,
+               None if M is None else self._mm_mask.data_ptr(),
+               W.ctypes.data)
 
     def upload(self, params_list):
         """Packs parameter dicts into the persistent device buffer (mcdisk:

@@ -20,6 +20,7 @@
 #include "mcdisk.hpp"
 #include "thick.hpp"
 #include "sepfilter.hpp"
+#include "moments.hpp"
 
 #include <climits>
 #include <cmath>
@@ -114,7 +115,9 @@ __global__ __launch_bounds__(256) void batch_smdisk_thick(
 }
 
 struct DataArgs {
-    const float* obs;     // (nz, ny, nx) low-res observed cube
+    // (nz, ny, nx) low-res observed cube; nullptr: no cube residual (the
+    // moment-map objective takes the model cubes instead)
+    const float* obs;
     const float* err;     // same shape or nullptr
     const float* mask;    // same shape or nullptr
     float err_scalar;     // used when err == nullptr (1 = none)
@@ -148,6 +151,7 @@ __global__ __launch_bounds__(256) void batch_downscale_chi2(
                                (size_t)(bz + dz) * hnx * hny];
         const float m = sum * nf;
         if (lo_out) lo_out[set * lstride + i] = m;
+        if (!da.obs) continue;
         // objective.hpp:7-33 with mdl_w = mdl_m = none
         float r = m - da.obs[i];
         r /= da.err ? da.err[i] : da.err_scalar;
@@ -176,6 +180,56 @@ __global__ void batch_chi2_final(const double* __restrict__ partials,
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) out[set0 + set] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// Moment-map objective (mmaps.py:136-201 through objective.py:279-308):
+// per spaxel the requested moments of the set's low-res model cube
+// (moments.hpp, the plain kernel's arithmetic) and, per order, the residual
+// against the data map (objective.hpp:7-33 with the model's mask = the
+// spaxel's validity and unit model weights), FP64 partial sums of r^2, one
+// per block.  A zero mask entry (data or model) gives a zero residual, so
+// spaxels without model flux, whose moments are NaN, add nothing.
+struct MMapArgs {
+    int norders;
+    int ord[kMaxOrders];
+    float cutoff, zero_z, step_z;
+    const float* obs;      // (norders, ny, nx) data maps
+    const float* err;      // same shape or nullptr
+    const float* mask;     // same shape or nullptr
+    float err_scalar[kMaxOrders];
+    float weight[kMaxOrders];
+};
+
+__global__ __launch_bounds__(256) void batch_mmaps_chi2(
+    const float* __restrict__ lo, long long lstride, int nx, int ny, int nz,
+    MMapArgs ma, double* __restrict__ partials, int nparts)
+{
+    __shared__ double sh[4];
+    const int set = blockIdx.y;
+    const long long plane = (long long)nx * ny;
+    const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    double acc = 0.0;
+    if (c < plane) {
+        float md[kMaxOrders], m0;
+        const bool valid = spaxel_moments(lo + set * lstride + c, plane, nz,
+                                          ma.step_z, ma.zero_z, ma.cutoff,
+                                          ma.norders, ma.ord, md, m0);
+        for (int i = 0; i < ma.norders; ++i) {
+            const long long gi = i * plane + c;
+            float r = md[i] - ma.obs[gi];
+            r /= ma.err ? ma.err[gi] : ma.err_scalar[i];
+            if (ma.mask) r = ma.mask[gi] != 0.0f ? r * ma.mask[gi] : 0.0f;
+            r = valid ? r : 0.0f;
+            r *= ma.weight[i];
+            acc += (double)r * (double)r;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        partials[(size_t)set * nparts + blockIdx.x] =
+            sh[0] + sh[1] + sh[2] + sh[3];
 }
 
 // ---------------------------------------------------------------------------
@@ -505,7 +559,7 @@ __global__ __launch_bounds__(T) void fz_c2r_down_chi2(
                     zb[j] = ez + zl * sz;
                 }
             }
-            ob[j] = gi[j] >= 0 ? da.obs[gi[j]] : 0.0f;
+            ob[j] = gi[j] >= 0 && da.obs ? da.obs[gi[j]] : 0.0f;
             er[j] = gi[j] >= 0 && da.err ? da.err[gi[j]] : da.err_scalar;
             mk[j] = gi[j] >= 0 && da.mask ? da.mask[gi[j]] : 1.0f;
         }
@@ -522,6 +576,7 @@ __global__ __launch_bounds__(T) void fz_c2r_down_chi2(
             }
             const float m = sum * nf;
             if (lo_out) lo_out[set * lstride + gi[j]] = m;
+            if (!da.obs) continue;
             // objective.hpp:7-33 with mdl_w = mdl_m = none
             float r = m - ob[j];
             r /= er[j];
@@ -1066,6 +1121,7 @@ __global__ __launch_bounds__(256) void sep_zpass(
             if (Z >= nzl) break;
             const size_t gi = (size_t)Z * plane + col;
             if (lo_out) lo_out[set * lstride + gi] = o[j];
+            if (!da.obs) continue;
             // objective.hpp:7-33 with mdl_w = mdl_m = none
             float r = o[j] - da.obs[gi];
             r /= da.err ? da.err[gi] : da.err_scalar;
@@ -1147,6 +1203,12 @@ struct Batch {
     DevBuf hi;              // chunk x padded cube
     DevBuf spec;            // chunk x complex work spectrum
     DevBuf partials;
+    // moment-map objective (gbkhip_batch_set_mmaps): the model cubes of a
+    // chunk (when the caller asks for none) and the per-block partials
+    bool mm_on = false;
+    MMapArgs mm;
+    int mm_nparts = 0;
+    DevBuf mm_lo, mm_partials;
     DevBuf icube;           // mcdisk: chunk x int64 fixed-point cube
     // binned mcdisk deposits (when a tile spectrum fits LDS): per set of a
     // chunk, mc_cap records, ranks and sorted records, tile counts, offsets
@@ -1261,6 +1323,9 @@ static size_t work_bytes(const Batch& B)
 {
     const size_t mb = (size_t)B.desc.max_batch * B.lanes;
     size_t n = mb * (size_t)B.nparts * sizeof(double);
+    if (B.mm_on)
+        n += mb * ((size_t)B.n_lo * sizeof(float) +
+                   (size_t)B.mm_nparts * sizeof(double));
     if (B.sep.on) {
         if (B.sep.cube)
             n += mb * (size_t)B.n_hi * sizeof(float);
@@ -1295,6 +1360,10 @@ static void alloc_work(Batch& B)
     if (B.work) return;
     const size_t mb = (size_t)B.desc.max_batch * B.lanes;
     B.partials.alloc(mb * B.nparts * sizeof(double));
+    if (B.mm_on) {
+        B.mm_lo.alloc(mb * (size_t)B.n_lo * sizeof(float));
+        B.mm_partials.alloc(mb * (size_t)B.mm_nparts * sizeof(double));
+    }
     if (B.sep.on) {
         if (B.sep.cube)
             B.hi.alloc(mb * B.n_hi * sizeof(float));
@@ -1333,6 +1402,8 @@ static void release_work(Batch& B)
 {
     drop_graphs(B);          // graphs hold the buffer addresses
     B.partials.release();
+    B.mm_lo.release();
+    B.mm_partials.release();
     B.fused.S.release();
     B.fused.lines.release();
     B.sep.P1.release();
@@ -1558,6 +1629,7 @@ static void plan_fused(Context& c, Batch& B)
     F.on = true;
 }
 
+// cubes: the chunk's model cubes (set set0 first), or nullptr
 static void run_chunk_fused(Batch& b, const float* params, int set0,
                             int nsets, double* chi2, float* cubes, int ci)
 {
@@ -1624,8 +1696,7 @@ static void run_chunk_fused(Batch& b, const float* params, int set0,
                           D.scale[2], D.edge_hi[2], D.size_lo[0],
                           D.size_lo[1], D.size_lo[2], F.tiles_x,
                           div_magic((unsigned)(F.kx * F.ky)),
-                          div_magic((unsigned)F.kx), da,
-                          cubes ? cubes + (size_t)set0 * b.n_lo : nullptr,
+                          div_magic((unsigned)F.kx), da, cubes,
                           b.n_lo, b.partials.as<double>(), b.nparts);
         GBK_HIP(hipGetLastError());
     }
@@ -1875,6 +1946,7 @@ static void deposit_hi(Batch& b, const float* P, int set0, int nsets,
     GBK_HIP(hipGetLastError());
 }
 
+// cubes: the chunk's model cubes (set set0 first), or nullptr
 static void run_chunk_sep(Batch& b, const float* params, int set0,
                           int nsets, double* chi2, float* cubes, int ci,
                           int lane)
@@ -1971,8 +2043,7 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
         GBK_SEP_DISPATCH(sep_zpass, D.scale[2], P.tz, grid, dim3(kSepThreads),
                          0, c.stream, P2 + s0 * P.p2stride, P.p2stride,
                          P.wz.as<float>(), nxl, nyl, nzl, nz, P.zblocks, da,
-                         cubes ? cubes + (size_t)(set0 + s0) * b.n_lo
-                               : nullptr,
+                         cubes ? cubes + (size_t)s0 * b.n_lo : nullptr,
                          b.n_lo, partials + (size_t)s0 * b.nparts, b.nparts,
                          gflags + (size_t)s0 * nz * P.ngy, P.ngy,
                          sep_jy(P.ty), P.zero_part.as<double>());
@@ -2015,20 +2086,11 @@ static void compute_zero_partials(Batch& b)
     GBK_HIP(hipStreamSynchronize(c.stream));
 }
 
-static void run_chunk(Batch& b, const float* params, int set0, int nsets,
-                      double* chi2, float* cubes, int lane)
+// The padded-cube pipeline; cubes: the chunk's model cubes or nullptr.
+static void run_chunk_padded(Batch& b, const float* params, int set0,
+                             int nsets, double* chi2, float* cubes, int ci)
 {
     Context& c = *b.ctx;
-    const int ci = set0 / b.desc.max_batch;
-    mark(b, ci, 0);
-    if (b.sep.on) {
-        run_chunk_sep(b, params, set0, nsets, chi2, cubes, ci, lane);
-        return;
-    }
-    if (b.fused.on) {
-        run_chunk_fused(b, params, set0, nsets, chi2, cubes, ci);
-        return;
-    }
     const gbkhip_batch_desc& D = b.desc;
     const gbkhip_grid& g = D.grid;
     const float* P = params + (size_t)set0 * D.pstride;
@@ -2058,14 +2120,46 @@ static void run_chunk(Batch& b, const float* params, int set0, int nsets,
                        b.n_hi, D.size_hi[0], D.size_hi[1], D.scale[0],
                        D.scale[1], D.scale[2], D.edge_hi[0], D.edge_hi[1],
                        D.edge_hi[2], D.size_lo[0], D.size_lo[1],
-                       D.size_lo[2], da,
-                       cubes ? cubes + (size_t)set0 * b.n_lo : nullptr,
+                       D.size_lo[2], da, cubes,
                        b.n_lo, b.partials.as<double>(), b.nparts);
     GBK_HIP(hipGetLastError());
     hipLaunchKernelGGL(batch_chi2_final, dim3(nsets), dim3(256), 0, c.stream,
                        b.partials.as<double>(), b.nparts, chi2, set0);
     GBK_HIP(hipGetLastError());
     mark(b, ci, 3);
+}
+
+static void run_chunk(Batch& b, const float* params, int set0, int nsets,
+                      double* chi2, float* cubes, int lane)
+{
+    Context& c = *b.ctx;
+    const int ci = set0 / b.desc.max_batch;
+    const size_t mb = (size_t)b.desc.max_batch * lane;   // sets before lane
+    // the chunk's model cubes: the caller's, or (moment-map objective) the
+    // lane's own buffer
+    float* lo = cubes ? cubes + (size_t)set0 * b.n_lo
+                      : b.mm_on ? b.mm_lo.as<float>() + mb * b.n_lo
+                                : nullptr;
+    mark(b, ci, 0);
+    if (b.sep.on)
+        run_chunk_sep(b, params, set0, nsets, chi2, lo, ci, lane);
+    else if (b.fused.on)
+        run_chunk_fused(b, params, set0, nsets, chi2, lo, ci);
+    else
+        run_chunk_padded(b, params, set0, nsets, chi2, lo, ci);
+    if (b.mm_on) {
+        // the moment-map chi^2 replaces the (empty) cube residual's
+        double* part = b.mm_partials.as<double>() + mb * b.mm_nparts;
+        const gbkhip_batch_desc& D = b.desc;
+        hipLaunchKernelGGL(batch_mmaps_chi2, dim3(b.mm_nparts, nsets),
+                           dim3(256), 0, c.stream, lo, b.n_lo, D.size_lo[0],
+                           D.size_lo[1], D.size_lo[2], b.mm, part,
+                           b.mm_nparts);
+        GBK_HIP(hipGetLastError());
+        hipLaunchKernelGGL(batch_chi2_final, dim3(nsets), dim3(256), 0,
+                           c.stream, part, b.mm_nparts, chi2, set0);
+        GBK_HIP(hipGetLastError());
+    }
 }
 
 static void ensure_lane_streams(Batch& B)
@@ -2150,7 +2244,6 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
                   "need at least two rnodes");
         GBK_CHECK(D.max_batch >= 1, GBKHIP_ERR_ARG, "max_batch must be >= 1");
         GBK_CHECK(D.pstride >= 1, GBKHIP_ERR_ARG, "pstride must be >= 1");
-        GBK_CHECK(D.obs != nullptr, GBKHIP_ERR_ARG, "observed cube required");
         for (int i = 0; i < 3; ++i) {
             GBK_CHECK(D.size_lo[i] > 0 && D.size_hi[i] > 0 && D.scale[i] > 0,
                       GBKHIP_ERR_ARG, "sizes must be positive");
@@ -2293,6 +2386,9 @@ static void evaluate_impl(gbkhip_batch* b, const float* params,
     GBK_CHECK(nsets >= 0, GBKHIP_ERR_ARG, "negative set count");
     if (nsets == 0) return;
     GBK_CHECK(params && chi2, GBKHIP_ERR_ARG, "null params or chi2");
+    GBK_CHECK(B.desc.obs || B.mm_on, GBKHIP_ERR_ARG,
+              "no data: an observed cube at creation or moment maps "
+              "(gbkhip_batch_set_mmaps)");
     // the launches go to the context's stream member: point it at this
     // batch's stream for the duration of the call (under the context lock,
     // so batches of one context can be driven from several threads)
@@ -2377,6 +2473,54 @@ int gbkhip_mcdisk_fixed_scale(float cflux, int nclouds,
     return guarded([&] {
         GBK_CHECK(grid && out, GBKHIP_ERR_ARG, "null grid or out");
         *out = mcdisk_fixed_scale(cflux, nclouds, *grid);
+    });
+}
+
+int gbkhip_batch_set_mmaps(gbkhip_batch* b, int norders, const int* orders,
+                           float cutoff, float zero_z, float step_z,
+                           const float* obs, const float* err,
+                           const float* err_scalars, const float* mask,
+                           const float* weights)
+{
+    return guarded([&] {
+        GBK_CHECK(b != nullptr, GBKHIP_ERR_ARG, "null batch");
+        Batch& B = b->impl;
+        GBK_HIP(hipSetDevice(B.device));
+        if (norders == 0) {
+            GBK_CHECK(B.desc.obs != nullptr, GBKHIP_ERR_ARG,
+                      "no observed cube to return to");
+            release_work(B);
+            B.mm_on = false;
+            return;
+        }
+        GBK_CHECK(norders > 0 && norders <= kMaxOrders && orders && obs,
+                  GBKHIP_ERR_ARG, "1-8 orders and the data maps required");
+        GBK_CHECK(err || err_scalars, GBKHIP_ERR_ARG,
+                  "error maps or one error per order required");
+        MMapArgs m{};
+        m.norders = norders;
+        for (int i = 0; i < kMaxOrders; ++i) {
+            m.ord[i] = i < norders ? orders[i] : 0;
+            m.err_scalar[i] = i < norders && err_scalars ? err_scalars[i]
+                                                          : 1.0f;
+            m.weight[i] = i < norders && weights ? weights[i] : 1.0f;
+        }
+        for (int i = 0; i < norders; ++i) {
+            GBK_CHECK(m.ord[i] >= 0 && m.ord[i] <= 7 &&
+                      (i == 0 || m.ord[i] > m.ord[i - 1]), GBKHIP_ERR_ARG,
+                      "orders must be sorted, distinct and within 0-7");
+        }
+        m.cutoff = cutoff;
+        m.zero_z = zero_z;
+        m.step_z = step_z;
+        m.obs = obs;
+        m.err = err;
+        m.mask = mask;
+        release_work(B);      // the moment buffers join the work buffers
+        B.mm = m;
+        B.mm_on = true;
+        B.mm_nparts = (int)ceil_div((long long)B.desc.size_lo[0] *
+                                    B.desc.size_lo[1], 256);
     });
 }
 

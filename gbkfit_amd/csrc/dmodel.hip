@@ -2,6 +2,7 @@
 // plus the direct convolution and spectral weight cube of the gmodel backend
 // (core/gmodel/gmodels.hpp:9-82).
 #include "common.hpp"
+#include "moments.hpp"
 
 namespace gbkhip {
 
@@ -48,8 +49,7 @@ __global__ void mask_kernel(float cutoff, int apply, long long n, float* d,
 }
 
 // dmodels.hpp:76-228.  One lane per spaxel; lanes consecutive in x so each
-// channel plane is read as coalesced rows.  Sums run over z in the same
-// order as the reference (FP32).
+// channel plane is read as coalesced rows (moment sums: moments.hpp).
 __global__ void moments_kernel(int nx, int ny, int nz, float step_z,
                                float zero_z, const float* __restrict__ d,
                                const float* __restrict__ w, float cutoff,
@@ -66,46 +66,18 @@ __global__ void moments_kernel(int nx, int ny, int nz, float step_z,
     md += blockIdx.y * bmap * norders;
     mm += blockIdx.y * bmap;
     if (mw) mw += blockIdx.y * bmap * norders;
-    int ord[8];
-    for (int i = 0; i < 8; ++i) ord[i] = i < norders ? orders[i] : 0;
-    const int max_order = ord[norders - 1];
-    int m = 0;
-    float m0s = 0.0f;
-    for (int z = 0; z < nz; ++z) m0s += d[c + z * plane] * step_z;
-    const bool valid = fabsf(m0s) > cutoff;
+    int ord[kMaxOrders];
+    for (int i = 0; i < kMaxOrders; ++i) ord[i] = i < norders ? orders[i] : 0;
+    float mo[kMaxOrders], m0;
+    const bool valid = spaxel_moments(d + c, plane, nz, step_z, zero_z,
+                                      cutoff, norders, ord, mo, m0);
     mm[c] = valid ? 1.0f : 0.0f;
-    const float nan = __builtin_nanf("");
-    const float m0 = valid ? m0s : nan;
-    if (ord[m] == 0) { md[c + m * plane] = m0; m++; }
+    for (int i = 0; i < norders; ++i) md[c + i * plane] = mo[i];
     if (w) {
         float ws = 0.0f;
         for (int z = 0; valid && z < nz; ++z)
             ws += w[c + z * plane] * d[c + z * plane] * step_z / m0;
-        mw[c] = valid ? ws : nan;
-    }
-    if (max_order == 0) return;
-    float m1s = 0.0f;
-    for (int z = 0; valid && z < nz; ++z)
-        m1s += d[c + z * plane] * (zero_z + z * step_z) * step_z;
-    const float m1 = valid ? m1s / m0 : nan;
-    if (m < norders && ord[m] == 1) { md[c + m * plane] = m1; m++; }
-    if (max_order == 1) return;
-    float m2s = 0.0f;
-    for (int z = 0; valid && z < nz; ++z) {
-        const float dv = zero_z + z * step_z - m1;
-        m2s += d[c + z * plane] * dv * dv * step_z;
-    }
-    const float m2 = valid ? sqrtf(m2s / m0) : nan;
-    if (m < norders && ord[m] == 2) { md[c + m * plane] = m2; m++; }
-    for (; m < norders; ++m) {
-        float s = 0.0f;
-        for (int z = 0; valid && z < nz; ++z) {
-            const float dv = zero_z + z * step_z - m1;
-            float p = 1.0f;
-            for (int e = 0; e < ord[m]; ++e) p *= dv;
-            s += d[c + z * plane] * p;
-        }
-        md[c + m * plane] = valid ? s / m0 : nan;
+        mw[c] = valid ? ws : __builtin_nanf("");
     }
 }
 

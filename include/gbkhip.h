@@ -231,7 +231,8 @@ typedef struct {
     int edge_hi[3];          /* downscale offset (DCube edge) */
     int has_conv;            /* PSF and/or LSF present */
     const float* pcube;      /* device, padded PSF x LSF cube, centre at 0 */
-    const float* obs;        /* device, observed low-res cube */
+    const float* obs;        /* device, observed low-res cube; NULL when the
+                                objective is moment maps (set_mmaps) */
     const float* err;        /* device error cube or NULL */
     const float* mask;       /* device mask cube or NULL */
     float err_scalar;        /* error used when err is NULL (1: none) */
@@ -342,6 +343,22 @@ int gbkhip_batch_line_stats(gbkhip_batch* batch, long long* counts);
  * count. */
 /* Stream of later evaluate() calls (NULL: the context's stream). */
 int gbkhip_batch_set_stream(gbkhip_batch* batch, void* stream);
+/* Moment-map objective instead of the cube residual: the reference's
+ * DModelMMaps + Objective (src/gbkfit/model/dmodels/mmaps.py:136-201,
+ * objective.py:279-308; moments core/dmodel/dmodels.hpp:76-228).  chi2[s]
+ * becomes sum over the orders and spaxels of r^2, r = (moment - data) /
+ * error, zero where a data mask entry is 0 or the model's moments are
+ * invalid (|sum I dv| <= cutoff), times the order's weight.  orders (host):
+ * norders sorted distinct values 0-7; obs (device, norders x ny x nx): the
+ * data maps; err (device, same shape) or err_scalars (host, one per
+ * order); mask (device, same shape) or NULL; weights (host, one per order)
+ * or NULL (1).  zero_z / step_z: the low-res spectral axis.  norders = 0
+ * returns to the cube objective.  Frees the work buffers and graphs. */
+int gbkhip_batch_set_mmaps(gbkhip_batch* batch, int norders,
+                           const int* orders, float cutoff, float zero_z,
+                           float step_z, const float* obs, const float* err,
+                           const float* err_scalars, const float* mask,
+                           const float* weights);
 /* Frees the work buffers (everything sized by max_batch) and the cached
  * graphs; the next evaluate() allocates them again.  Resident state (layout,
  * kernel spectra, plans) stays. */
