@@ -121,6 +121,7 @@ struct DataArgs {
     const float* err;     // same shape or nullptr
     const float* mask;    // same shape or nullptr
     float err_scalar;     // used when err == nullptr (1 = none)
+    float inv_err_scalar; // 1 / err_scalar
     float weight;
 };
 
@@ -1054,7 +1055,13 @@ __global__ __launch_bounds__(256, 4) void sep_ypass(
 
 // lo[Z][Y][X] = sum_w Wz[w] P2[sz Z + w][Y][X], then the residual against
 // the data and one FP64 partial sum of r^2 per block (fixed order).  A block
-// covers 64 X x 4 Y columns and kSepJZ channels.
+// covers 64 X x 4 Y columns and kSepJZ channels: wave w takes row Y0 + w, so
+// whether an input row was written by the y pass is uniform over the wave
+// (one ballot per wave).  Every input load is issued: rows the y pass left
+// unwritten read a 64-float zero row instead (cached, no HBM traffic), so
+// the loads go out back to back with a scalar base and a lane offset.  The
+// residual divides by the error through its reciprocal (v_rcp_f32, within
+// an ulp of the quotient).
 template <int SZ, int TZ>
 __global__ __launch_bounds__(256) void sep_zpass(
     const float* __restrict__ P2, long long p2stride, const float* wz,
@@ -1062,51 +1069,60 @@ __global__ __launch_bounds__(256) void sep_zpass(
     float* __restrict__ lo_out, long long lstride,
     double* __restrict__ partials, int nparts,
     const unsigned char* __restrict__ gflag, int ngy, int jy,
-    const double* __restrict__ zero_part)
+    const double* __restrict__ zero_part, const float* __restrict__ zrow)
 {
-    constexpr int T = kSepThreads, J = kSepJZ, TXB = 64, TYB = T / TXB;
+    constexpr int T = kSepThreads, J = kSepJZ, TXB = 64;
     constexpr int NIN = SZ * (J - 1) + TZ;
+    constexpr int NW = (NIN + 63) / 64;         // ballot words
     __shared__ float s_w[TZ];
     __shared__ double red[T / 64];
-    __shared__ unsigned char s_live[TYB][NIN];
     const int set = blockIdx.z / zblocks, zb = blockIdx.z % zblocks;
-    const int X = blockIdx.x * TXB + threadIdx.x % TXB;
-    const int Y = blockIdx.y * TYB + threadIdx.x / TXB;
+    const int lane = threadIdx.x & 63;
+    // the wave index in a scalar register: row bases are wave-uniform
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int X0 = blockIdx.x * TXB;
+    const int Y = blockIdx.y * (T / TXB) + wv;          // uniform per wave
     const int Z0 = zb * J;
     for (int i = threadIdx.x; i < TZ; i += T) s_w[i] = wz[i];
-    // which (channel, row) inputs the y pass wrote: the others are zero
-    int any = 0;
-    for (int i = threadIdx.x; i < TYB * NIN; i += T) {
-        const int yy = i / NIN, k = i - yy * NIN;
-        const int z = SZ * Z0 + k, Yr = blockIdx.y * TYB + yy;
-        s_live[yy][k] = (z < need_z && Yr < nyl)
-            ? gflag[((size_t)set * need_z + z) * ngy + Yr / jy] : 0;
-        any |= s_live[yy][k];
+    // which of the wave's input rows (channel SZ Z0 + i of row Y) the y pass
+    // wrote: the others are zero
+    unsigned long long live[NW];
+    unsigned long long any = 0ull;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        const int i = 64 * k + lane, z = SZ * Z0 + i;
+        const bool f = i < NIN && z < need_z && Y < nyl &&
+            gflag[((size_t)set * need_z + z) * ngy + Y / jy] != 0;
+        live[k] = __ballot(f);
+        any |= live[k];
     }
     const int part = (blockIdx.y * gridDim.x + blockIdx.x) * zblocks + zb;
+    const int plane = nyl * nxl;
     // a block whose model is zero everywhere has the data's own partial
     // sum, computed once at set-up by this kernel with no live input (same
     // arithmetic, so the same bits): the data are not read again
-    if (!__syncthreads_or(any) && zero_part) {
+    if (!__syncthreads_or(any != 0ull) && zero_part) {
         if (threadIdx.x == 0)
             partials[(size_t)set * nparts + part] = zero_part[part];
+        const int X = X0 + lane;
         if (lo_out && X < nxl && Y < nyl) {
-            const size_t plane = (size_t)nyl * nxl, col = (size_t)Y * nxl + X;
+            float* lo = lo_out + set * lstride + Y * nxl + X;
             for (int j = 0; j < J && Z0 + j < nzl; ++j)
-                lo_out[set * lstride + (size_t)(Z0 + j) * plane + col] = 0.0f;
+                lo[(size_t)(Z0 + j) * plane] = 0.0f;
         }
         return;
     }
     double acc = 0.0;
-    if (X < nxl && Y < nyl) {
-        const size_t plane = (size_t)nyl * nxl, col = (size_t)Y * nxl + X;
-        const float* src = P2 + set * p2stride + col;
-        const unsigned char* lv = s_live[threadIdx.x / TXB];
+    if (Y < nyl) {
+        // lanes past the last column read the row's last value (in bounds)
+        const unsigned xo = min(lane, nxl - 1 - X0);
+        const float* row = P2 + set * p2stride + (size_t)Y * nxl + X0;
         float v[NIN];
 #pragma unroll
         for (int i = 0; i < NIN; ++i) {
-            const int z = SZ * Z0 + i;
-            v[i] = lv[i] ? src[(size_t)z * plane] : 0.0f;
+            const float* src = ((live[i >> 6] >> (i & 63)) & 1ull)
+                ? row + (size_t)(SZ * Z0 + i) * plane : zrow;
+            v[i] = src[xo];
         }
         float o[J] = {};
 #pragma unroll
@@ -1115,23 +1131,50 @@ __global__ __launch_bounds__(256) void sep_zpass(
 #pragma unroll
             for (int j = 0; j < J; ++j) o[j] = fmaf(ww, v[SZ * j + w], o[j]);
         }
+        const int X = X0 + lane;
+        if (X < nxl) {
+            // column offsets: a wave-uniform part and the lane's column
+            const int col0 = Y * nxl + X0;
+            const int jn = min(J, nzl - Z0);      // outputs of this block
+            if (lo_out) {
+                float* lo = lo_out + set * lstride + col0;
 #pragma unroll
-        for (int j = 0; j < J; ++j) {
-            const int Z = Z0 + j;
-            if (Z >= nzl) break;
-            const size_t gi = (size_t)Z * plane + col;
-            if (lo_out) lo_out[set * lstride + gi] = o[j];
-            if (!da.obs) continue;
-            // objective.hpp:7-33 with mdl_w = mdl_m = none
-            float r = o[j] - da.obs[gi];
-            r /= da.err ? da.err[gi] : da.err_scalar;
-            if (da.mask) r *= da.mask[gi];
-            r *= da.weight;
-            acc += (double)r * (double)r;
+                for (int j = 0; j < J; ++j)
+                    if (j < jn) lo[(size_t)(Z0 + j) * plane + xo] = o[j];
+            }
+            if (da.obs) {
+                // every data load in flight before the first use (channels
+                // past the cube read the last one and are not summed)
+                size_t zo[J];
+                float ob[J], ie[J], mk[J];
+#pragma unroll
+                for (int j = 0; j < J; ++j) {
+                    zo[j] = (size_t)min(Z0 + j, nzl - 1) * plane + col0;
+                    ob[j] = (da.obs + zo[j])[xo];
+                }
+                if (da.err) {
+#pragma unroll
+                    for (int j = 0; j < J; ++j) ie[j] = (da.err + zo[j])[xo];
+                }
+                if (da.mask) {
+#pragma unroll
+                    for (int j = 0; j < J; ++j) mk[j] = (da.mask + zo[j])[xo];
+                }
+#pragma unroll
+                for (int j = 0; j < J; ++j) {
+                    // objective.hpp:7-33 with mdl_w = mdl_m = none
+                    float r = o[j] - ob[j];
+                    r *= da.err ? __builtin_amdgcn_rcpf(ie[j])
+                                : da.inv_err_scalar;
+                    if (da.mask) r *= mk[j];
+                    r *= da.weight;
+                    if (j < jn) acc += (double)r * (double)r;
+                }
+            }
         }
     }
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    if (lane == 0) red[wv] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
         double t = 0.0;
@@ -1155,6 +1198,7 @@ struct Sep {
     DevBuf segmask;               // work: per set and row, live segments
     DevBuf gflags;                // work: per set, channel, y-pass group
     DevBuf zero_part;             // z-pass partials of a zero model (data)
+    DevBuf zrow;                  // 64 zeros: the z pass's unwritten rows
     int ngy = 0;                  // y-pass output groups per channel
     int nchunks = 0;              // S1 channel chunks
     int last_sets = 0;            // sets of the last chunk run on lane 0
@@ -1685,6 +1729,7 @@ static void run_chunk_fused(Batch& b, const float* params, int set0,
     da.err = D.err;
     da.mask = D.mask;
     da.err_scalar = D.err_scalar;
+    da.inv_err_scalar = 1.0f / D.err_scalar;
     da.weight = D.weight;
     {
         dim3 grid(F.tiles_x * F.tiles_y, nsets);
@@ -2036,6 +2081,7 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
     da.err = D.err;
     da.mask = D.mask;
     da.err_scalar = D.err_scalar;
+    da.inv_err_scalar = 1.0f / D.err_scalar;
     da.weight = D.weight;
     slices(P.zblocks, [&](int s0, int ns) {
         dim3 grid(ceil_div(nxl, 64), ceil_div(nyl, kSepThreads / 64),
@@ -2046,7 +2092,8 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
                          cubes ? cubes + (size_t)s0 * b.n_lo : nullptr,
                          b.n_lo, partials + (size_t)s0 * b.nparts, b.nparts,
                          gflags + (size_t)s0 * nz * P.ngy, P.ngy,
-                         sep_jy(P.ty), P.zero_part.as<double>());
+                         sep_jy(P.ty), P.zero_part.as<double>(),
+                         P.zrow.as<float>());
         GBK_HIP(hipGetLastError());
     });
     mark(b, ci, 4);
@@ -2068,6 +2115,8 @@ static void compute_zero_partials(Batch& b)
     const int nxl = D.size_lo[0], nyl = D.size_lo[1], nzl = D.size_lo[2];
     const int nz = P.need[2];
     P.zero_part.alloc((size_t)b.nparts * sizeof(double));
+    P.zrow.alloc(64 * sizeof(float));
+    GBK_HIP(hipMemsetAsync(P.zrow.ptr, 0, 64 * sizeof(float), c.stream));
     DevBuf dead((size_t)nz * P.ngy);
     GBK_HIP(hipMemsetAsync(dead.ptr, 0, (size_t)nz * P.ngy, c.stream));
     DataArgs da;
@@ -2075,13 +2124,15 @@ static void compute_zero_partials(Batch& b)
     da.err = D.err;
     da.mask = D.mask;
     da.err_scalar = D.err_scalar;
+    da.inv_err_scalar = 1.0f / D.err_scalar;
     da.weight = D.weight;
     dim3 grid(ceil_div(nxl, 64), ceil_div(nyl, kSepThreads / 64), P.zblocks);
     GBK_SEP_DISPATCH(sep_zpass, D.scale[2], P.tz, grid, dim3(kSepThreads), 0,
                      c.stream, P.P2.as<float>(), P.p2stride,
                      P.wz.as<float>(), nxl, nyl, nzl, nz, P.zblocks, da,
                      nullptr, b.n_lo, P.zero_part.as<double>(), b.nparts,
-                     dead.as<unsigned char>(), P.ngy, sep_jy(P.ty), nullptr);
+                     dead.as<unsigned char>(), P.ngy, sep_jy(P.ty), nullptr,
+                     P.zrow.as<float>());
     GBK_HIP(hipGetLastError());
     GBK_HIP(hipStreamSynchronize(c.stream));
 }
@@ -2114,6 +2165,7 @@ static void run_chunk_padded(Batch& b, const float* params, int set0,
     da.err = D.err;
     da.mask = D.mask;
     da.err_scalar = D.err_scalar;
+    da.inv_err_scalar = 1.0f / D.err_scalar;
     da.weight = D.weight;
     dim3 grid(b.nparts, nsets);
     hipLaunchKernelGGL(batch_downscale_chi2, grid, dim3(256), 0, c.stream, hi,
