@@ -284,13 +284,41 @@ __global__ __launch_bounds__(256) void batch_lines_thin(
     const int y = blockIdx.y * 4 + threadIdx.y;
     const int set = blockIdx.z;
     const gbkhip_disk d = set_disk(tmpl, lay, params + (size_t)set * pstride);
-    // the set's node-0 geometry (two sincos) once per block
+    // the set's node-0 geometry (two sincos) once per block, and whether
+    // the block's 64 x 4 spaxels all lie beyond the disk edge: a spaxel's
+    // deprojected radius is at least its sky distance from the centre
+    // (the thin deprojection only stretches y), so a block whose nearest
+    // point is farther than the last node (with a margin far above FP32
+    // rounding) holds no line (flat disks; loose / tilted ones evaluate
+    // every spaxel)
     __shared__ NodeGeom s_ng;
-    if (threadIdx.x == 0 && threadIdx.y == 0) s_ng = node_geom(d, 0);
+    __shared__ int s_cull;
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const NodeGeom ng = node_geom(d, 0);
+        s_ng = ng;
+        int cull = 0;
+        if (!d.loose && !d.tilted) {
+            const float rmax = d.rnodes[d.nrnodes - 1] * 1.0001f + 1e-6f;
+            auto gap = [](float a, float b, float c) {
+                const float lo = fminf(a, b), hi = fmaxf(a, b);
+                return fmaxf(fmaxf(lo - c, c - hi), 0.0f);
+            };
+            const int xa = blockIdx.x * 64, ya = blockIdx.y * 4;
+            const float dx = gap(g.spat_zero[0] + xa * g.spat_step[0],
+                                 g.spat_zero[0] + (xa + 63) * g.spat_step[0],
+                                 ng.xpos);
+            const float dy = gap(g.spat_zero[1] + ya * g.spat_step[1],
+                                 g.spat_zero[1] + (ya + 3) * g.spat_step[1],
+                                 ng.ypos);
+            cull = dx * dx + dy * dy > rmax * rmax;
+        }
+        s_cull = cull;
+    }
     __syncthreads();
     if (y >= ny) return;               // whole waves: a wave is one row
+    if (s_cull && form == 1 && segmask && blockIdx.x < 63) return;
     float4 out = make_float4(0, 0, 1, 0);
-    if (x < nx && x < need_x && y < need_y) {
+    if (!s_cull && x < nx && x < need_x && y < need_y) {
         const bool need_theta = disk_needs_theta(d);
         const NodeGeom ng = s_ng;
         const VoxelLine V = smdisk_voxel(d, g, ng, x, y, 0, need_theta, false);
