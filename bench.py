@@ -37,11 +37,13 @@ A serving leg (skip with --no-serve) replays a Poisson stream of
 single-set requests (default 1000/s for 60 s) over 64 resident galaxies with
 cube sizes log-uniform from 64x64x32 to 256x256x128, half smdisk, half mcdisk
 with 2^20 clouds; each request returns chi^2 and moment maps 0/1/2 through
-gbkfit_amd.serve.Server (continuous batching over 4 streams, HBM-budget
-admission), in steady state: before the stream every galaxy has captured
-the graphs of batches of 1-8 sets and holds its work buffers while they fit
-the budget.  Its throughput and p50/p99 latencies go into the `serving`
-object.
+gbkfit_amd.serve.Server (continuous batching over 4 streams, up to 8
+galaxies per slot launch, HBM-budget admission), in steady state: before
+the stream every galaxy has captured the graphs of batches of 1-16 sets and
+holds its work buffers.  Its throughput and p50/p99 latencies go into the
+`serving` object, with a 10 s run under a budget of half the galaxies' work
+bytes (`admission`: releases and re-admissions included) and an offered-rate
+sweep up to the first rate not sustained (`sweep`, `saturation_rate`).
 
 A ragged leg (skip with --no-ragged) autotunes 32 galaxies of log-uniform
 sizes from 64x64x32 to 512x512x256 (smdisk / mcdisk with 2^20..2^24 clouds,
@@ -160,13 +162,30 @@ def build_galaxy_model(driver, size, scale=(1, 1, 1), rmax=None,
 
 
 def serve_benchmark(drv, ngal, seconds, rate, slots=4, max_batch=16,
-                    seed=77):
-    """Poisson stream of single-set requests (chi^2 + moment maps 0/1/2)
+                    seed=77, group=8, admission_seconds=10.0,
+                    sweep_rates=(2000.0, 4000.0, 8000.0, 16000.0),
+                    sweep_seconds=5.0):
+    """Poisson streams of single-set requests (chi^2 + moment maps 0/1/2)
     over `ngal` resident galaxies with cube sizes log-uniform from
-    64x64x32 to 256x256x128."""
-    import threading
+    64x64x32 to 256x256x128, half mcdisk with 2^20 clouds.
+
+    Three measurements on the same galaxies:
+      main:      `rate` requests/s for `seconds`, every galaxy running with
+                 the work arena of the slot it is launched in (work memory
+                 = slots x the largest galaxy's arena; steady state: each
+                 galaxy's graphs for batches of 1-8 in every slot captured);
+      admission: the same rate for `admission_seconds` with work buffers
+                 of each galaxy's own under a budget of half their total,
+                 so the server releases idle galaxies (least recently used)
+                 to admit others (allocation and graph capture included);
+      sweep:     `sweep_seconds` at each of `sweep_rates` until the first
+                 rate the server does not sustain (throughput below 0.9 of
+                 the offered rate, or p99 above 10 times the main p99);
+                 saturation = the highest rate sustained.
+    A slot launches up to `group` galaxies together."""
     import torch
-    from gbkfit_amd.serve import GpuGalaxy, Server, make_streams
+    from gbkfit_amd.serve import (GpuGalaxy, Server, make_arenas,
+                                  make_streams)
     rng = np.random.default_rng(seed)
     streams = make_streams(slots)
     gals, sizes = [], []
@@ -186,62 +205,95 @@ def serve_benchmark(drv, ngal, seconds, rate, slots=4, max_batch=16,
                               template_params=truth, orders=(0, 1, 2),
                               streams=streams))
         sizes.append(size)
-    t_setup = time.perf_counter() - t0
     works = [g.work_bytes() for g in gals]
-    free, total = torch.cuda.mem_get_info()
-    budget = int(min(sum(works), 0.5 * free))
-    srv = Server(hbm_budget=budget, slots=slots, max_batch=max_batch)
-    for i, g in enumerate(gals):
-        srv.add(i, g)
-    # arrival schedule and request parameters, fixed by the seed
-    n = int(rate * seconds)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
-    targets = rng.integers(0, ngal, n)
-    pars = [dict(xpos=0.0, ypos=0.0, vsys=0.0,
-                 posa=float(rng.uniform(0, 360)),
-                 incl=float(rng.uniform(20, 80)), bpt_a=1.0,
-                 bpt_s=0.15 * sizes[t][0], vpt_rt=0.05 * sizes[t][0],
-                 vpt_vt=float(rng.uniform(100, 300)),
-                 dpt_a=float(rng.uniform(5, 50)),
-                 **({'bht_s': 1.0} if kinds[t] == 'mcdisk' else {}))
-            for t in targets]
-    # steady state: every galaxy's graphs for batches of 1-8 captured, and
-    # its work buffers kept while they fit the budget (the server counts
-    # them and releases idle ones, least recently used first, for others)
-    used = 0
-    for i, g in enumerate(gals):
-        p0 = next(pars[j] for j in range(n) if targets[j] == i) \
-            if i in set(targets.tolist()) else None
-        if p0 is None:
-            continue
-        for k in range(1, 9):
-            g.results(g.launch([p0] * k, 0), k)
-        if used + g.work_bytes() <= budget:
-            used += g.work_bytes()
-        else:
-            g.release()
-    torch.cuda.synchronize()
-    futs = []
-    srv.start()
-    start = time.perf_counter()
-    for i in range(n):
-        delay = start + arrivals[i] - time.perf_counter()
-        if delay > 0:
-            time.sleep(delay)
-        futs.append(srv.submit(int(targets[i]), pars[i]))
-    for f in futs:
-        f.result(300)
-    end = time.perf_counter()
-    srv.stop()
-    lat = srv.latency_percentiles()
-    out = dict(requests=n, offered_rate=rate, seconds=seconds,
-               galaxies=ngal, slots=slots, max_batch=max_batch,
-               throughput=n / (end - start), **lat,
-               launches=srv.stats['launches'],
-               mean_batch=float(np.mean(srv.stats['batch_sizes'])),
-               max_batch_seen=srv.stats['max_batch_seen'],
-               hbm_budget_bytes=budget, work_bytes_total=int(sum(works)),
-               releases=srv.stats['releases'],
+    arena = max(g.arena_bytes() for g in gals)
+    arenas = make_arenas(arena, slots)
+
+    def request(t, r):
+        return dict(xpos=0.0, ypos=0.0, vsys=0.0,
+                    posa=float(r.uniform(0, 360)),
+                    incl=float(r.uniform(20, 80)), bpt_a=1.0,
+                    bpt_s=0.15 * sizes[t][0], vpt_rt=0.05 * sizes[t][0],
+                    vpt_vt=float(r.uniform(100, 300)),
+                    dpt_a=float(r.uniform(5, 50)),
+                    **({'bht_s': 1.0} if kinds[t] == 'mcdisk' else {}))
+
+    # steady state: every galaxy's graphs for batches of 1..8 captured in
+    # every slot's arena
+    def warm():
+        r0 = np.random.default_rng(seed + 1)
+        for i, g in enumerate(gals):
+            g.use_arenas(arenas)
+            p0 = request(i, r0)
+            for slot in range(slots):
+                for k in range(1, 9):
+                    g.results(g.launch([p0] * k, slot), k)
+        torch.cuda.synchronize()
+
+    warm()
+    t_setup = time.perf_counter() - t0
+
+    def stream(rate_, seconds_, budget, sseed):
+        r = np.random.default_rng(sseed)
+        n = int(rate_ * seconds_)
+        arrivals = np.cumsum(r.exponential(1.0 / rate_, n))
+        targets = r.integers(0, ngal, n)
+        pars = [request(t, r) for t in targets]
+        if budget is not None:          # work buffers of the galaxies' own
+            for g in gals:
+                g.use_arenas(None)
+        srv = Server(hbm_budget=budget, slots=slots, max_batch=max_batch,
+                     group=group)
+        for i, g in enumerate(gals):
+            srv.add(i, g)
+        futs = []
+        srv.start()
+        start = time.perf_counter()
+        for i in range(n):
+            delay = start + arrivals[i] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            futs.append(srv.submit(int(targets[i]), pars[i]))
+        for f in futs:
+            f.result(300)
+        end = time.perf_counter()
+        srv.stop()
+        st = srv.stats
+        return dict(offered_rate=rate_, seconds=seconds_, requests=n,
+                    throughput=n / (end - start),
+                    **srv.latency_percentiles(), launches=st['launches'],
+                    galaxy_launches=st['galaxy_launches'],
+                    mean_batch=float(np.mean(st['batch_sizes'])),
+                    max_batch_seen=st['max_batch_seen'],
+                    releases=st['releases'], hbm_budget_bytes=budget)
+
+    main = stream(rate, seconds, None, seed + 2)
+    print(f"serving main done: p99 {main['p99_ms']:.2f} ms, mean batch "
+          f"{main['mean_batch']:.2f}", file=sys.stderr, flush=True)
+    admission = stream(rate, admission_seconds, int(0.5 * sum(works)),
+                       seed + 3)
+    print(f"serving admission done: {admission['releases']} releases",
+          file=sys.stderr, flush=True)
+    for g in gals:    # back to the slot arenas (graphs still cached)
+        g.use_arenas(arenas)
+    sweep, saturation = [], rate
+    for k, rr in enumerate(sweep_rates):
+        res = stream(rr, sweep_seconds, None, seed + 10 + k)
+        ok = (res['throughput'] >= 0.9 * rr and
+              res['p99_ms'] <= 10 * main['p99_ms'])
+        sweep.append(dict(res, sustained=ok))
+        print(f"serving sweep {rr:.0f} req/s: {res['throughput']:.0f} "
+              f"req/s, p99 {res['p99_ms']:.2f} ms", file=sys.stderr,
+              flush=True)
+        if not ok:
+            break
+        saturation = rr
+    out = dict(main, galaxies=ngal, slots=slots, max_batch=max_batch,
+               group=group, work_bytes_total=int(sum(works)),
+               arena_bytes_per_slot=int(arena),
+               work_bytes_in_use=int(arena) * slots,
+               admission=admission, sweep=sweep,
+               saturation_rate=saturation,
                cube_sizes=dict(min=list(min(sizes)), max=list(max(sizes))),
                models=dict(smdisk=kinds.count('smdisk'),
                            mcdisk=kinds.count('mcdisk'),

@@ -80,6 +80,9 @@ _SIGS = {
                           _c.c_int),
     'gbkhip_mmaps_moments': ([_vp, _ip, _fp, _fp, _vp, _vp, _c.c_float,
                               _c.c_int, _vp, _vp, _vp, _vp], _c.c_int),
+    'gbkhip_mmaps_moments_batched': ([_vp, _ip, _fp, _fp, _vp, _c.c_float,
+                                      _c.c_int, _vp, _vp, _vp, _c.c_int],
+                                     _c.c_int),
     'gbkhip_smdisk_evaluate': ([_vp, _c.POINTER(Disk), _c.POINTER(Grid),
                                 _c.POINTER(Outputs)], _c.c_int),
     'gbkhip_mcdisk_evaluate': ([_vp, _c.c_float, _c.c_int, _vp, _c.c_int, _vp,

@@ -78,6 +78,10 @@ N.register_signatures({
                                    _c.c_int, _c.POINTER(_c.c_int)], _c.c_int),
     'gbkhip_batch_set_timing': ([_c.c_void_p, _c.c_int], _c.c_int),
     'gbkhip_batch_set_stream': ([_c.c_void_p, _c.c_void_p], _c.c_int),
+    'gbkhip_batch_arena_bytes': ([_c.c_void_p,
+                                  _c.POINTER(_c.c_ulonglong)], _c.c_int),
+    'gbkhip_batch_bind_work': ([_c.c_void_p, _c.c_void_p, _c.c_ulonglong],
+                               _c.c_int),
     'gbkhip_batch_set_mmaps': ([_c.c_void_p, _c.c_int, _c.c_void_p,
                                 _c.c_float, _c.c_float, _c.c_float,
                                 _c.c_void_p, _c.c_void_p, _c.c_void_p,
@@ -569,6 +573,24 @@ class BatchEvaluator:
     def release(self):
         """Frees the work buffers; the next evaluation allocates them."""
         N.call('gbkhip_batch_release', self.handle)
+
+    def arena_bytes(self):
+        """Bytes of a work arena this batch can run with (bind_work)."""
+        v = _c.c_ulonglong()
+        N.call('gbkhip_batch_arena_bytes', self.handle, _c.byref(v))
+        return v.value
+
+    def bind_work(self, arena):
+        """Runs the next evaluations with their work buffers in `arena` (a
+        device tensor of at least arena_bytes() bytes that no other batch
+        uses at the same time), or in buffers of their own (None).  Graphs
+        captured with an arena stay cached for it."""
+        if arena is None:
+            N.call('gbkhip_batch_bind_work', self.handle, None, 0)
+            return
+        nbytes = arena.numel() * arena.element_size()
+        N.call('gbkhip_batch_bind_work', self.handle, arena.data_ptr(),
+               _c.c_ulonglong(nbytes))
 
     def memory(self):
         """dict(resident=bytes, work=bytes, allocated=bool)."""

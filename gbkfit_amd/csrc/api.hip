@@ -320,6 +320,33 @@ int gbkhip_mmaps_moments(gbkhip_ctx* ctx, const int size[3],
     });
 }
 
+int gbkhip_mmaps_moments_batched(gbkhip_ctx* ctx, const int size[3],
+                                 const float step[3], const float zero[3],
+                                 const float* dcube_d, float cutoff,
+                                 int norders, const int* orders,
+                                 float* mmaps_d, float* mmaps_m, int nbatch)
+{
+    return guarded([&] {
+        Context* c = ctx_of(ctx);
+        check_size3(size, "size");
+        check_ptr(step, "step");
+        check_ptr(zero, "zero");
+        check_ptr(dcube_d, "dcube_d");
+        check_ptr(orders, "orders");
+        check_ptr(mmaps_d, "mmaps_d");
+        check_ptr(mmaps_m, "mmaps_m");
+        GBK_CHECK(norders >= 1 && norders <= 8, GBKHIP_ERR_ARG,
+                  "norders must be in [1, 8]");
+        GBK_CHECK(nbatch >= 0 && nbatch <= 65535, GBKHIP_ERR_ARG,
+                  "nbatch must be within [0, 65535] (one grid row per cube)");
+        if (nbatch == 0) return;
+        const long long plane = (long long)size[0] * size[1];
+        launch_moments(*c, size, step, zero, dcube_d, nullptr, cutoff,
+                       norders, orders, mmaps_d, mmaps_m, nullptr, nbatch,
+                       plane * size[2], plane);
+    });
+}
+
 // ---- GModel -----------------------------------------------------------------
 int gbkhip_smdisk_evaluate(gbkhip_ctx* ctx, const gbkhip_disk* disk,
                            const gbkhip_grid* grid, const gbkhip_outputs* out)

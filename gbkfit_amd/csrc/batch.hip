@@ -1290,6 +1290,10 @@ struct Batch {
     DevBuf overflow;        // int: set when a set's clouds exceed mc_cap
     int max_grid_z = 65535; // device limit on gridDim.z (and y)
     unsigned long long work_gen = 0;    // bumped whenever work is allocated
+    // caller-owned work arena (gbkhip_batch_bind_work): the work buffers
+    // are views into it, or allocations of their own when it is null
+    char* arena = nullptr;
+    size_t arena_bytes = 0;
     const int* csum = nullptr;       // mcdisk inputs of the current call
     const double* qscale = nullptr;
     int nparts = 0;
@@ -1299,7 +1303,7 @@ struct Batch {
     hipStream_t stream = nullptr;   // evaluate() stream; null: the context's
     // graph cache: (params, nsets, chi2, cubes) -> executable graph
     std::map<std::tuple<uintptr_t, int, uintptr_t, uintptr_t, uintptr_t,
-                        uintptr_t>,
+                        uintptr_t, uintptr_t>,
              hipGraphExec_t> graphs;
     hipStream_t cap_stream = nullptr;
     // chunks in flight: with 2 lanes, odd chunks run on the aux stream with
@@ -1427,38 +1431,51 @@ static void drop_graphs(Batch& B)
     B.graphs.clear();
 }
 
-static void alloc_work(Batch& B)
+// Places the work buffers, in a fixed order: views into the bound arena
+// (256-byte aligned, so one arena always gets the same layout and graphs
+// captured with it stay valid) or allocations of their own.  dry: only the
+// arena bytes the layout needs.  The int64 cube of unbinned mcdisk relies on
+// staying zero between evaluations, so it is always the batch's own.
+static size_t place_work(Batch& B, bool dry)
 {
-    if (B.work) return;
     const size_t mb = (size_t)B.desc.max_batch * B.lanes;
-    B.partials.alloc(mb * B.nparts * sizeof(double));
+    size_t off = 0;
+    auto take = [&](DevBuf& d, size_t n) {
+        off = (off + 255) & ~(size_t)255;
+        if (!dry) {
+            if (B.arena) d.view(B.arena + off, n);
+            else d.alloc(n);
+        }
+        off += n;
+    };
+    take(B.partials, mb * B.nparts * sizeof(double));
     if (B.mm_on) {
-        B.mm_lo.alloc(mb * (size_t)B.n_lo * sizeof(float));
-        B.mm_partials.alloc(mb * (size_t)B.mm_nparts * sizeof(double));
+        take(B.mm_lo, mb * (size_t)B.n_lo * sizeof(float));
+        take(B.mm_partials, mb * (size_t)B.mm_nparts * sizeof(double));
     }
     if (B.sep.on) {
         if (B.sep.cube)
-            B.hi.alloc(mb * B.n_hi * sizeof(float));
+            take(B.hi, mb * B.n_hi * sizeof(float));
         else
-            B.fused.lines.alloc(mb * (size_t)B.sep.need[1] * B.sep.need[0] *
-                                sizeof(float4));
-        B.sep.P1.alloc(mb * (size_t)B.sep.p1stride * sizeof(float));
-        B.sep.P2.alloc(mb * (size_t)B.sep.p2stride * sizeof(float));
-        B.sep.masks.alloc(mb * (size_t)B.sep.need[1] * sizeof(uint64_t));
+            take(B.fused.lines, mb * (size_t)B.sep.need[1] *
+                                    B.sep.need[0] * sizeof(float4));
+        take(B.sep.P1, mb * (size_t)B.sep.p1stride * sizeof(float));
+        take(B.sep.P2, mb * (size_t)B.sep.p2stride * sizeof(float));
+        take(B.sep.masks, mb * (size_t)B.sep.need[1] * sizeof(uint64_t));
         if (!B.sep.cube)
-            B.sep.segmask.alloc(mb * (size_t)B.sep.need[1] *
-                                sizeof(uint64_t));
-        B.sep.gflags.alloc(mb * (size_t)B.sep.need[2] * B.sep.ngy);
+            take(B.sep.segmask, mb * (size_t)B.sep.need[1] *
+                                    sizeof(uint64_t));
+        take(B.sep.gflags, mb * (size_t)B.sep.need[2] * B.sep.ngy);
     } else if (B.fused.on) {
-        B.fused.S.alloc(mb * B.fused.sstride * sizeof(float2));
-        B.fused.lines.alloc(mb * B.fused.lstride * sizeof(float4));
+        take(B.fused.S, mb * B.fused.sstride * sizeof(float2));
+        take(B.fused.lines, mb * B.fused.lstride * sizeof(float4));
     } else {
-        B.hi.alloc(mb * B.n_hi * sizeof(float));
-        if (B.desc.has_conv) B.spec.alloc(mb * B.n_c * sizeof(float2));
+        take(B.hi, mb * B.n_hi * sizeof(float));
+        if (B.desc.has_conv) take(B.spec, mb * B.n_c * sizeof(float2));
     }
     if (B.desc.mcdisk && B.mc_tw) {
-        B.mcbin.alloc((size_t)B.lanes * mc_lane_bytes(B));
-    } else if (B.desc.mcdisk) {
+        take(B.mcbin, (size_t)B.lanes * mc_lane_bytes(B));
+    } else if (B.desc.mcdisk && !dry) {
         // zeroed once; the conversion re-zeroes what it reads
         B.icube.alloc(mb * B.n_hi * sizeof(uint64_t));
         hipLaunchKernelGGL(batch_zero_u64, dim3(4096), dim3(256), 0,
@@ -1466,10 +1483,59 @@ static void alloc_work(Batch& B)
                            (long long)(mb * B.n_hi));
         GBK_HIP(hipGetLastError());
     }
+    return off;
+}
+
+static void alloc_work(Batch& B)
+{
+    if (B.work) return;
+    if (B.arena)
+        GBK_CHECK(place_work(B, true) <= B.arena_bytes, GBKHIP_ERR_ARG,
+                  "the bound work arena is smaller than the batch needs "
+                  "(gbkhip_batch_memory)");
+    place_work(B, false);
     B.work = true;
     ++B.work_gen;
 }
 
+// Lets go of the work buffers (frees the batch's own; views of an arena
+// just end) without dropping graphs.
+static void drop_work_buffers(Batch& B)
+{
+    B.partials.release();
+    B.mm_lo.release();
+    B.mm_partials.release();
+    B.fused.S.release();
+    B.fused.lines.release();
+    B.sep.P1.release();
+    B.sep.P2.release();
+    B.sep.masks.release();
+    B.sep.segmask.release();
+    B.sep.gflags.release();
+    B.hi.release();
+    B.spec.release();
+    B.icube.release();
+    B.mcbin.release();
+    B.work = false;
+}
+
+// Frees the work buffers for the current binding: graphs captured with the
+// batch's own buffers go with them, graphs of arenas stay (their memory is
+// the caller's and keeps its layout).
+static void release_binding(Batch& B)
+{
+    for (auto it = B.graphs.begin(); it != B.graphs.end();) {
+        if (std::get<6>(it->first) == 0) {
+            (void)hipGraphExecDestroy(it->second);
+            it = B.graphs.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    drop_work_buffers(B);
+}
+
+// The layout changes (lanes, capacity, objective): every graph goes.
 static void release_work(Batch& B)
 {
     drop_graphs(B);          // graphs hold the buffer addresses
@@ -2492,7 +2558,8 @@ static void evaluate_impl(gbkhip_batch* b, const float* params,
     }
     const auto key = std::make_tuple((uintptr_t)params, nsets,
                                      (uintptr_t)chi2, (uintptr_t)cubes,
-                                     (uintptr_t)csum, (uintptr_t)qscale);
+                                     (uintptr_t)csum, (uintptr_t)qscale,
+                                     (uintptr_t)B.arena);
     auto it = B.graphs.find(key);
     if (it == B.graphs.end()) {
         if (!B.cap_stream)
@@ -2619,7 +2686,7 @@ int gbkhip_batch_release(gbkhip_batch* b)
         Batch& B = b->impl;
         GBK_HIP(hipSetDevice(B.device));
         // hipFree waits for work still using the buffers
-        release_work(B);
+        release_binding(B);
     });
 }
 
@@ -2639,6 +2706,34 @@ int gbkhip_batch_memory(gbkhip_batch* b, unsigned long long* resident,
         *resident = r;
         *work = work_bytes(B);
         *allocated = B.work ? 1 : 0;
+    });
+}
+
+int gbkhip_batch_arena_bytes(gbkhip_batch* b, unsigned long long* bytes)
+{
+    return guarded([&] {
+        GBK_CHECK(b && bytes, GBKHIP_ERR_ARG, "null argument");
+        *bytes = place_work(b->impl, true);
+    });
+}
+
+int gbkhip_batch_bind_work(gbkhip_batch* b, void* arena,
+                           unsigned long long bytes)
+{
+    return guarded([&] {
+        GBK_CHECK(b != nullptr, GBKHIP_ERR_ARG, "null batch");
+        Batch& B = b->impl;
+        char* a = static_cast<char*>(arena);
+        if (a == B.arena && (size_t)bytes == B.arena_bytes) return;
+        GBK_CHECK(!a || place_work(B, true) <= bytes, GBKHIP_ERR_ARG,
+                  "the work arena is smaller than the batch needs "
+                  "(gbkhip_batch_arena_bytes)");
+        GBK_HIP(hipSetDevice(B.device));
+        // own buffers are freed with their graphs; an arena's graphs stay
+        // valid for it (keyed by its address)
+        release_binding(B);
+        B.arena = a;
+        B.arena_bytes = a ? (size_t)bytes : 0;
     });
 }
 

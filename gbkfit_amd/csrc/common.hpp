@@ -65,6 +65,7 @@ int guarded(F&& f)
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    bool owned = true;      // false: a view into memory someone else owns
     DevBuf() = default;
     explicit DevBuf(size_t n) { alloc(n); }
     DevBuf(const DevBuf&) = delete;
@@ -76,15 +77,23 @@ struct DevBuf {
         if (n) GBK_HIP(hipMalloc(&ptr, n));
         bytes = n;
     }
+    void view(void* p, size_t n)
+    {
+        release();
+        ptr = n ? p : nullptr;
+        bytes = n;
+        owned = false;
+    }
     void ensure(size_t n)
     {
         if (n > bytes) alloc(n);
     }
     void release()
     {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr && owned) (void)hipFree(ptr);
         ptr = nullptr;
         bytes = 0;
+        owned = true;
     }
     template<typename T> T* as() const { return static_cast<T*>(ptr); }
 };

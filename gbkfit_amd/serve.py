@@ -9,13 +9,18 @@ galaxy) and runs a scheduler that
 
   * admits work by HBM budget: a galaxy's work buffers (per-set spectra)
     are allocated when it is scheduled and released, least recently used
-    first, when another galaxy needs the room;
+    first, when another galaxy needs the room -- or, with slot arenas
+    (make_arenas), every galaxy runs with the work buffers of the slot it
+    is launched in, so the work memory is that of the slots and nothing is
+    ever released;
   * batches on arrival: whenever an execution slot (a HIP stream) is free,
     it takes the galaxy with the oldest waiting request and launches up to
-    max_batch of its waiting requests as one batched evaluation; requests
-    that arrive while others are on the GPU join later batches as slots
-    free up, so batches form continuously and work of several galaxies
-    overlaps on the device;
+    max_batch of its waiting requests as one batched evaluation, and with
+    group > 1 does the same for up to group - 1 further galaxies (oldest
+    request first) in the same slot, one stream and one completion for all
+    of them; requests that arrive while others are on the GPU join later
+    batches as slots free up, so batches form continuously and work of
+    several galaxies overlaps on the device;
   * completes requests by polling each slot's event, then resolves their
     futures with chi^2 (and moment maps when the galaxy was added with
     orders);
@@ -70,29 +75,33 @@ class Server:
 
     hbm_budget: bytes the work buffers of all galaxies may occupy together
     (None: no limit); slots: concurrent batched evaluations; max_batch:
-    requests per launch (at most the galaxies' own max_batch); freeze_gc:
+    requests per galaxy and launch (at most the galaxies' own max_batch);
+    group: galaxies a slot launches together (their evaluations queued on
+    the slot's stream back to back, completing together); freeze_gc:
     freeze the objects alive at start() out of the cyclic collector until
     stop() (a process-wide setting: pass False to leave the collector
     alone).
     """
 
     def __init__(self, hbm_budget=None, slots=4, max_batch=16,
-                 clock=time.perf_counter, freeze_gc=True):
-        if slots < 1 or max_batch < 1:
-            raise ValueError("slots and max_batch must be >= 1")
+                 clock=time.perf_counter, freeze_gc=True, group=1):
+        if slots < 1 or max_batch < 1 or group < 1:
+            raise ValueError("slots, max_batch and group must be >= 1")
         self.hbm_budget = hbm_budget
         self.nslots = slots
         self.max_batch = max_batch
+        self.group = group
         self.freeze_gc = freeze_gc
         self._frozen = False
         self.clock = clock
         self._entries = collections.OrderedDict()
-        self._slots = [None] * slots          # (entry, requests, token)
+        self._slots = [None] * slots   # [(entry, requests, token), ...]
         self._cv = threading.Condition()
         self._thread = None
         self._stop = False
-        self.stats = dict(launches=0, requests=0, completed=0,
-                          releases=0, max_batch_seen=0, batch_sizes=[])
+        self.stats = dict(launches=0, galaxy_launches=0, requests=0,
+                          completed=0, releases=0, max_batch_seen=0,
+                          batch_sizes=[])
         self.latencies = []
 
     # -- registry ---------------------------------------------------------
@@ -152,19 +161,20 @@ class Server:
         for i, s in enumerate(self._slots):
             if s is None:
                 continue
-            entry, reqs, token = s
-            if not entry.galaxy.done(token):
+            # the slot's launches share its stream: done when all are
+            if not all(e.galaxy.done(token) for e, _, token in s):
                 continue
-            results = entry.galaxy.results(token, len(reqs))
             now = self.clock()
-            for r, res in zip(reqs, results):
-                r.t_done = now
-                self.latencies.append(now - r.t_submit)
-                r.future.set_result(res)
-            entry.busy = False
-            entry.last_used = now
+            for entry, reqs, token in s:
+                results = entry.galaxy.results(token, len(reqs))
+                for r, res in zip(reqs, results):
+                    r.t_done = now
+                    self.latencies.append(now - r.t_submit)
+                    r.future.set_result(res)
+                entry.busy = False
+                entry.last_used = now
+                self.stats['completed'] += len(reqs)
             self._slots[i] = None
-            self.stats['completed'] += len(reqs)
             done_any = True
         return done_any
 
@@ -176,28 +186,32 @@ class Server:
             if not cands:
                 break
             cands.sort(key=lambda e: e.pending[0].t_submit)
-            entry = None
-            for e in cands:
-                if self._admit(e):
-                    entry = e
-                    break
-            if entry is None:
-                break
-            n = min(self.max_batch, entry.galaxy.max_batch, len(entry.pending))
-            reqs = [entry.pending.popleft() for _ in range(n)]
             slot = self._slots.index(None)
-            now = self.clock()
-            for r in reqs:
-                r.t_launch = now
-            try:
-                token = entry.galaxy.launch([r.params for r in reqs], slot)
-            except Exception as exc:        # fail these requests, keep serving
+            group = []
+            for e in cands:
+                if len(group) == self.group:
+                    break
+                if not self._admit(e):
+                    continue
+                n = min(self.max_batch, e.galaxy.max_batch, len(e.pending))
+                reqs = [e.pending.popleft() for _ in range(n)]
+                now = self.clock()
                 for r in reqs:
-                    r.future.set_exception(exc)
-                continue
-            entry.busy = True
-            entry.last_used = now
-            self._slots[slot] = (entry, reqs, token)
+                    r.t_launch = now
+                try:
+                    token = e.galaxy.launch([r.params for r in reqs], slot)
+                except Exception as exc:    # fail these requests, keep serving
+                    for r in reqs:
+                        r.future.set_exception(exc)
+                    continue
+                e.busy = True
+                e.last_used = now
+                group.append((e, reqs, token))
+                self.stats['galaxy_launches'] += 1
+            if not group:
+                break
+            self._slots[slot] = group
+            n = sum(len(r) for _, r, _ in group)
             self.stats['launches'] += 1
             self.stats['batch_sizes'].append(n)
             self.stats['max_batch_seen'] = max(self.stats['max_batch_seen'], n)
@@ -282,7 +296,7 @@ class GpuGalaxy:
 
     def __init__(self, driver, dmodel, gmodel, obs, error=None, mask=None,
                  weight=1.0, max_batch=16, template_params=None,
-                 orders=None, streams=None):
+                 orders=None, streams=None, arenas=None):
         from .batch import BatchEvaluator
         import torch
         self._t = torch
@@ -332,17 +346,33 @@ class GpuGalaxy:
             self._h_qscale = torch.empty(max_batch, dtype=torch.float64,
                                          pin_memory=True)
         self._streams = streams
+        # slot arenas: one work arena per slot (each at least arena_bytes()
+        # of this galaxy), shared with the other galaxies of that slot
+        self._arenas = arenas
         self.ev.release()            # admission allocates the work buffers
         self._allocated = False
 
     # -- server interface -------------------------------------------------
     def work_bytes(self):
-        return self.ev.memory()['work']
+        # with slot arenas the work memory belongs to the slots
+        return 0 if self._arenas is not None else self.ev.memory()['work']
+
+    def arena_bytes(self):
+        return self.ev.arena_bytes()
+
+    def use_arenas(self, arenas):
+        """Runs with the slots' work arenas (a list, one per slot, each at
+        least arena_bytes()), or with work buffers of its own (None)."""
+        self.ev.release()
+        self._allocated = False
+        self._arenas = arenas
 
     def allocated(self):
-        return self._allocated
+        return self._allocated or self._arenas is not None
 
     def release(self):
+        if self._arenas is not None:
+            return
         self.ev.release()
         self._allocated = False
 
@@ -357,6 +387,8 @@ class GpuGalaxy:
             self._h_csum[:n].copy_(t.from_numpy(csum))
             self._h_qscale[:n].copy_(t.from_numpy(qs))
         cubes = self._cubes.data_ptr() if self._cubes is not None else None
+        self.ev.bind_work(None if self._arenas is None
+                          else self._arenas[slot])
         with t.cuda.stream(stream):
             self._params[:n].copy_(self._h_params[:n], non_blocking=True)
             self.ev.set_stream(stream)
@@ -374,12 +406,14 @@ class GpuGalaxy:
                        cubes, 1)
             self._allocated = True
             if self.orders is not None:
-                be = self.driver.backends().dmodel(np.float32)
-                for i in range(n):
-                    be.mmaps_moments(self._lo_size, self._lo_step,
-                                     self._lo_zero, self._cubes[i], None,
-                                     1e-6, self._orders_dev, self._mmaps[i],
-                                     None, self._mmask[i])
+                # every set's maps in one launch (the plain kernel's
+                # arithmetic, cube by cube)
+                N.call('gbkhip_mmaps_moments_batched',
+                       self.driver.context().handle, N._i3(self._lo_size),
+                       N._f3(self._lo_step), N._f3(self._lo_zero),
+                       self._cubes.data_ptr(), 1e-6, len(self.orders),
+                       self._orders_dev.data_ptr(), self._mmaps.data_ptr(),
+                       self._mmask.data_ptr(), n)
                 self._h_mmaps[:n].copy_(self._mmaps[:n], non_blocking=True)
             self._h_chi2[:n].copy_(self._chi2[:n], non_blocking=True)
             ev = t.cuda.Event()
@@ -399,6 +433,14 @@ class GpuGalaxy:
 
     def close(self):
         self.ev.close()
+
+
+def make_arenas(nbytes, n):
+    """n device work arenas of nbytes each (one per server slot); nbytes is
+    the largest arena_bytes() of the galaxies served."""
+    import torch
+    return [torch.empty(int(nbytes), dtype=torch.uint8, device='cuda')
+            for _ in range(n)]
 
 
 def make_streams(n):

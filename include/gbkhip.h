@@ -96,6 +96,17 @@ int gbkhip_mmaps_moments(gbkhip_ctx* ctx, const int size[3],
                          float cutoff, int norders, const int* orders,
                          float* mmaps_d, float* mmaps_m, float* mmaps_w);
 
+/* Moment maps of nbatch cubes in one launch: no counterpart in the
+ * reference (which computes one model's maps per call); for batched callers
+ * such as a serving loop.  dcube_d (nbatch, nz, ny, nx) contiguous; mmaps_d
+ * (nbatch, norders, ny, nx); mmaps_m (nbatch, ny, nx); no weight cube.  Each
+ * cube's maps equal gbkhip_mmaps_moments' of that cube. */
+int gbkhip_mmaps_moments_batched(gbkhip_ctx* ctx, const int size[3],
+                                 const float step[3], const float zero[3],
+                                 const float* dcube_d, float cutoff,
+                                 int norders, const int* orders,
+                                 float* mmaps_d, float* mmaps_m, int nbatch);
+
 /* ---- GModel backend: GModelf32 (host/src/gmodels.cpp, 83/88 positional
  *      arguments, host/include/gbkfit/host/gmodels.hpp:25-132) ------------ */
 typedef struct {
@@ -343,6 +354,19 @@ int gbkhip_batch_line_stats(gbkhip_batch* batch, long long* counts);
  * count. */
 /* Stream of later evaluate() calls (NULL: the context's stream). */
 int gbkhip_batch_set_stream(gbkhip_batch* batch, void* stream);
+/* Work arena (no counterpart in the reference): the batch's work buffers
+ * (everything sized by max_batch) become views into `arena`, a device
+ * buffer the caller owns, of at least gbkhip_batch_arena_bytes bytes, laid
+ * out the same way every time.  Batches that never run at the same time
+ * (one serving slot's stream) can share one arena, so the work memory of
+ * many resident galaxies is that of the slots, not of every galaxy.  A
+ * graph captured with an arena stays cached for it: rebinding a batch to an
+ * arena it used before costs nothing.  NULL returns to buffers of the
+ * batch's own.  The caller keeps an arena allocated while any batch may
+ * run with it, and does not run two batches with one arena at once. */
+int gbkhip_batch_arena_bytes(gbkhip_batch* batch, unsigned long long* bytes);
+int gbkhip_batch_bind_work(gbkhip_batch* batch, void* arena,
+                           unsigned long long bytes);
 /* Moment-map objective instead of the cube residual: the reference's
  * DModelMMaps + Objective (src/gbkfit/model/dmodels/mmaps.py:136-201,
  * objective.py:279-308; moments core/dmodel/dmodels.hpp:76-228).  chi2[s]
@@ -360,8 +384,9 @@ int gbkhip_batch_set_mmaps(gbkhip_batch* batch, int norders,
                            const float* err_scalars, const float* mask,
                            const float* weights);
 /* Frees the work buffers (everything sized by max_batch) and the cached
- * graphs; the next evaluate() allocates them again.  Resident state (layout,
- * kernel spectra, plans) stays. */
+ * graphs that use them; the next evaluate() allocates them again.  Graphs
+ * captured with a work arena (gbkhip_batch_bind_work) stay cached for it.
+ * Resident state (layout, kernel spectra, plans) stays. */
 int gbkhip_batch_release(gbkhip_batch* batch);
 /* Bytes of resident state and of work buffers (needed whether allocated or
  * not), and whether the work buffers are allocated now. */

@@ -308,6 +308,39 @@ def test_moments(drv):
     assert H.normwise(mw.cpu().numpy()[0], rw[0]) < 1e-5
 
 
+def test_moments_batched_equal_one_by_one(drv):
+    """gbkhip_mmaps_moments_batched: the maps of three cubes in one launch
+    equal gbkhip_mmaps_moments' of each cube, bit for bit."""
+    import torch
+    from gbkfit_amd import _native as N
+    fx = np.load(cases.__file__.replace('cases.py', 'golden_ref.npz'))
+    base = fx['moments/cube']
+    nz, ny, nx = base.shape
+    rng = np.random.default_rng(4)
+    cubes = np.stack([base, base[:, ::-1, :] * 0.5,
+                      base + rng.normal(0, 1e-3, base.shape)]).astype(
+                          np.float32)
+    orders = np.array([0, 1, 2, 4], np.int32)
+    be = drv.backends().dmodel(np.float32)
+    dev = H.dev(drv, cubes)
+    od = H.dev(drv, orders, np.int32)
+    md = drv.mem_alloc_d((3, 4, ny, nx), np.float32)
+    mm = drv.mem_alloc_d((3, ny, nx), np.float32)
+    N.call('gbkhip_mmaps_moments_batched', drv.context().handle,
+           N._i3((nx, ny, nz)), N._f3((1, 1, 10)), N._f3((0, 0, -120)),
+           dev.data_ptr(), 1e-6, 4, od.data_ptr(), md.data_ptr(),
+           mm.data_ptr(), 3)
+    for i in range(3):
+        d1 = drv.mem_alloc_d((4, ny, nx), np.float32)
+        m1 = drv.mem_alloc_d((ny, nx), np.float32)
+        be.mmaps_moments((nx, ny, nz), (1, 1, 10), (0, 0, -120), dev[i],
+                         None, 1e-6, od, d1, None, m1)
+        torch.cuda.synchronize()
+        assert np.array_equal(md[i].cpu().numpy(), d1.cpu().numpy(),
+                              equal_nan=True), i
+        assert np.array_equal(mm[i].cpu().numpy(), m1.cpu().numpy()), i
+
+
 def test_residual_and_sum(drv):
     be = drv.backends().objective(np.float32)
     rng = np.random.default_rng(2)

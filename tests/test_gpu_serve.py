@@ -97,3 +97,56 @@ def test_served_chi2_matches_direct_batch(galaxies):
     res = g.results(tok, 4)
     chi2 = g.ev.evaluate(sets[1][:4], use_graph=False)
     assert [r['chi2'] for r in res] == list(chi2)
+
+
+def test_slot_arenas_grouped_equal_alone(galaxies):
+    """Every galaxy runs with the work arena of its slot (one arena per
+    slot, shared by the galaxies launched there), several galaxies per slot
+    launch: each request's chi^2 and maps equal those of the request alone
+    with the galaxy's own buffers, bit for bit; no work buffers are
+    released."""
+    from gbkfit_amd.serve import Server, make_arenas
+    drv, gals, sets = galaxies
+    arena = max(g.arena_bytes() for g in gals.values())
+    arenas = make_arenas(arena, 3)
+    for g in gals.values():
+        g.use_arenas(arenas)
+    srv = Server(slots=3, max_batch=8, group=2)
+    for k, g in gals.items():
+        srv.add(k, g)
+    reqs = [(k, i) for k in gals for i in range(len(sets[k]))]
+    random.Random(9).shuffle(reqs)
+    srv.start()
+    futs = {key: srv.submit(key[0], sets[key[0]][key[1]]) for key in reqs}
+    srv.stop()
+    served = {key: f.result(60) for key, f in futs.items()}
+    print('stats', {k: v for k, v in srv.stats.items() if k != 'batch_sizes'})
+    assert srv.stats['completed'] == len(reqs)
+    assert srv.stats['releases'] == 0
+    assert srv.stats['galaxy_launches'] >= srv.stats['launches']
+    for g in gals.values():
+        g.use_arenas(None)
+    for (k, i), res in served.items():
+        alone = gals[k].results(gals[k].launch([sets[k][i]], 0), 1)[0]
+        assert alone['chi2'] == res['chi2'], (k, i)
+        assert np.array_equal(alone['mmaps'], res['mmaps'], equal_nan=True)
+
+
+def test_arena_binding_and_rejection(galaxies):
+    """A batch alternating between two arenas and its own buffers gives the
+    same chi^2 each time; an arena below arena_bytes() is refused."""
+    import torch
+    from gbkfit_amd import _native as N
+    drv, gals, sets = galaxies
+    ev = gals[2].ev
+    need = ev.arena_bytes()
+    a = torch.empty(need, dtype=torch.uint8, device='cuda')
+    b = torch.empty(need + 4096, dtype=torch.uint8, device='cuda')
+    ref = ev.evaluate(sets[2][:3])
+    for arena in (a, b, a, None, b):
+        ev.bind_work(arena)
+        assert np.array_equal(ev.evaluate(sets[2][:3]), ref)
+    small = torch.empty(need - 256, dtype=torch.uint8, device='cuda')
+    with pytest.raises(N.GbkHipError):
+        ev.bind_work(small)
+    ev.bind_work(None)

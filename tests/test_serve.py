@@ -211,3 +211,50 @@ def test_start_freezes_resident_objects_until_stop():
         assert s2.submit('b', 2.0).result(5) is not None
     finally:
         s2.stop()
+
+
+def test_group_launches_several_galaxies_in_one_slot():
+    """group > 1: a free slot takes up to `group` galaxies, oldest request
+    first, each with its own batch; the slot completes when all of them
+    have."""
+    s = Server(slots=1, max_batch=4, group=3)
+    g = {k: FakeGalaxy(1) for k in 'abcd'}
+    for k, v in g.items():
+        s.add(k, v)
+    futs = {k: [s.submit(k, float(i)) for i in range(2)] for k in 'dcba'}
+    s.step()
+    assert [len(g[k].launched) for k in 'abcd'] == [0, 1, 1, 1]
+    assert s.stats['launches'] == 1 and s.stats['galaxy_launches'] == 3
+    assert s.stats['batch_sizes'] == [6]
+    # a waits for the slot; two of the three finished is not done
+    for t in g['d'].launched + g['c'].launched:
+        t['done'] = True
+    s.step()
+    assert not futs['d'][0].done() and not g['a'].launched
+    finish_all(g['b'])
+    s.step()                   # slot completes, then a launches in it
+    assert [f.result(0)['chi2'] for f in futs['d']] == [0.0, 2.0]
+    assert len(g['a'].launched) == 1
+    finish_all(g['a'])
+    s.step()
+    assert all(f.result(0) for fs in futs.values() for f in fs)
+    assert s.stats['completed'] == 8
+
+
+def test_group_admission_per_galaxy():
+    """Each galaxy of a group is admitted on its own: the ones that do not
+    fit the budget wait for a later launch."""
+    s = Server(hbm_budget=100, slots=2, max_batch=4, group=3)
+    g = {k: FakeGalaxy(40) for k in 'abc'}
+    for k, v in g.items():
+        s.add(k, v)
+    fs = [s.submit(k, 1.0) for k in 'abc']
+    s.step()                   # a and b fit (80); c would need 120
+    assert len(g['a'].launched) == 1 and len(g['b'].launched) == 1
+    assert not g['c'].launched
+    finish_all(*g.values())
+    s.step()                   # a, b done; c admitted (LRU release)
+    assert len(g['c'].launched) == 1 and s.stats['releases'] == 1
+    finish_all(*g.values())
+    s.step()
+    assert all(f.result(0)['chi2'] == 2.0 for f in fs)
