@@ -87,21 +87,25 @@ void upload_twiddles(DevBuf& buf, int n, int count)
 }
 
 // Radices of the register-tiled engine: the power of two is split evenly
-// into stages of radix <= 16, then one stage per factor 3 and 5.
+// into stages of radix <= 16, then one stage per factor 3, 5, 7, 11 and 13
+// (the reference's padding to multiples of 32 leaves such factors, e.g.
+// 224 = 32 x 7); larger primes take the generic kernels.
 static bool fast_radices(int n, std::vector<int>& R)
 {
-    int m = n, a2 = 0, a3 = 0, a5 = 0;
+    int m = n, a2 = 0;
     while (m % 2 == 0) { m /= 2; ++a2; }
-    while (m % 3 == 0) { m /= 3; ++a3; }
-    while (m % 5 == 0) { m /= 5; ++a5; }
+    int odd[5] = {0, 0, 0, 0, 0};
+    const int primes[5] = {3, 5, 7, 11, 13};
+    for (int i = 0; i < 5; ++i)
+        while (m % primes[i] == 0) { m /= primes[i]; ++odd[i]; }
     if (m != 1) return false;
     const int s2 = (a2 + 3) / 4;
     for (int s = 0; s < s2; ++s) {
         const int bits = a2 / s2 + (s < a2 % s2 ? 1 : 0);
         R.push_back(1 << bits);
     }
-    for (int i = 0; i < a3; ++i) R.push_back(3);
-    for (int i = 0; i < a5; ++i) R.push_back(5);
+    for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < odd[i]; ++j) R.push_back(primes[i]);
     return (int)R.size() <= kFMax;
 }
 

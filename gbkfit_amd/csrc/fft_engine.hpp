@@ -111,6 +111,78 @@ template <> struct Bfly<5> {
         v[3] = csub(t2, u2);
     }
 };
+// Odd prime radices 7, 11, 13: X[k] = sum_n v[n] W^(nk), W = exp(i sign
+// 2 pi / P), with the inputs paired as (v[n] + v[P-n], v[n] - v[P-n]):
+//   X[k], X[P-k] = v0 + sum_n cos(2 pi n k / P) b_n
+//                    +- i sign sum_n sin(2 pi n k / P) d_n.
+// cos / sin of 2 pi m / P for m = 1 .. (P-1)/2, in double, rounded once.
+template <int P> struct OddPrime;
+template <> struct OddPrime<7> {
+    static constexpr float c[3] = {0.6234898018587336f, -0.22252093395631434f,
+                                   -0.900968867902419f};
+    static constexpr float s[3] = {0.7818314824680298f, 0.9749279121818236f,
+                                   0.43388373911755823f};
+};
+template <> struct OddPrime<11> {
+    static constexpr float c[5] = {0.8412535328311812f, 0.41541501300188644f,
+                                   -0.142314838273285f, -0.654860733945285f,
+                                   -0.9594929736144974f};
+    static constexpr float s[5] = {0.5406408174555976f, 0.9096319953545183f,
+                                   0.9898214418809328f, 0.7557495743542583f,
+                                   0.28173255684142967f};
+};
+template <> struct OddPrime<13> {
+    static constexpr float c[6] = {0.8854560256532099f, 0.5680647467311559f,
+                                   0.120536680255323f, -0.35460488704253545f,
+                                   -0.7485107481711012f, -0.970941817426052f};
+    static constexpr float s[6] = {0.4647231720437685f, 0.8229838658936564f,
+                                   0.992708874098054f, 0.9350162426854148f,
+                                   0.6631226582407952f, 0.23931566428755768f};
+};
+
+template <int P> struct BflyOdd {
+    __device__ static constexpr int perm(int q) { return q; }
+    __device__ static void run(float2* v, float sign)
+    {
+        constexpr int H = (P - 1) / 2;
+        float2 b[H], d[H];
+        float2 s0 = v[0];
+#pragma unroll
+        for (int n = 1; n <= H; ++n) {
+            b[n - 1] = cadd(v[n], v[P - n]);
+            d[n - 1] = csub(v[n], v[P - n]);
+            s0 = cadd(s0, b[n - 1]);
+        }
+        float2 out[P];
+        out[0] = s0;
+#pragma unroll
+        for (int k = 1; k <= H; ++k) {
+            float2 t = v[0];
+            float ux = 0.0f, uy = 0.0f;
+#pragma unroll
+            for (int n = 1; n <= H; ++n) {
+                const int m = (n * k) % P;
+                const int mm = m <= H ? m : P - m;
+                const float c = OddPrime<P>::c[mm - 1];
+                const float sn = m <= H ? OddPrime<P>::s[mm - 1]
+                                        : -OddPrime<P>::s[mm - 1];
+                t.x = fmaf(c, b[n - 1].x, t.x);
+                t.y = fmaf(c, b[n - 1].y, t.y);
+                ux = fmaf(sn, d[n - 1].x, ux);
+                uy = fmaf(sn, d[n - 1].y, uy);
+            }
+            const float2 u = make_float2(-sign * uy, sign * ux);
+            out[k] = cadd(t, u);
+            out[P - k] = csub(t, u);
+        }
+#pragma unroll
+        for (int q = 0; q < P; ++q) v[q] = out[q];
+    }
+};
+template <> struct Bfly<7> : BflyOdd<7> {};
+template <> struct Bfly<11> : BflyOdd<11> {};
+template <> struct Bfly<13> : BflyOdd<13> {};
+
 template <> struct Bfly<8> {
     __device__ static constexpr int perm(int q) { return q; }
     __device__ static void run(float2* v, float sign)
@@ -386,6 +458,9 @@ __device__ void fast_fft(float2* buf, const FAxis& a, const float2* tw,
         case 4: fstage<4, LSH, T>(buf, a0, l, j0, nb, ns, tws, nsm, tw, sign); break;
         case 5: fstage<5, LSH, T>(buf, a0, l, j0, nb, ns, tws, nsm, tw, sign); break;
         case 8: fstage<8, LSH, T>(buf, a0, l, j0, nb, ns, tws, nsm, tw, sign); break;
+        case 7: fstage<7, LSH, T>(buf, a0, l, j0, nb, ns, tws, nsm, tw, sign); break;
+        case 11: fstage<11, LSH, T>(buf, a0, l, j0, nb, ns, tws, nsm, tw, sign); break;
+        case 13: fstage<13, LSH, T>(buf, a0, l, j0, nb, ns, tws, nsm, tw, sign); break;
         default: fstage<16, LSH, T>(buf, a0, l, j0, nb, ns, tws, nsm, tw, sign); break;
         }
     }

@@ -161,11 +161,13 @@ def test_fft_r2c_c2r(drv):
 
 @pytest.mark.parametrize('shape', [(128, 48, 320), (30, 300, 50),
                                    (9, 40, 1024), (2, 1536, 6),
-                                   (3, 2048, 4), (5, 17, 34), (7, 96, 192)])
+                                   (3, 2048, 4), (5, 17, 34), (7, 96, 192),
+                                   (7, 224, 448), (11, 26, 352),
+                                   (13, 77, 416), (4, 91, 286)])
 def test_fft_sizes_all_radices(drv, shape):
-    """Lengths that exercise every radix (16, 8, 4, 2, 3, 5), every
-    lines-per-block and register bucket of the tiled engine, and the generic
-    fallback (2048 lines, prime 17)."""
+    """Lengths that exercise every radix (16, 8, 4, 2, 3, 5, 7, 11, 13),
+    every lines-per-block and register bucket of the tiled engine, and the
+    generic fallback (2048 lines, prime 17)."""
     be = drv.backends().fft(np.float32)
     rng = np.random.default_rng(sum(shape))
     a = rng.standard_normal(shape).astype(np.float32)
