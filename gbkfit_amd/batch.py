@@ -43,7 +43,8 @@ class BDesc(_c.Structure):
                 ('max_batch', _c.c_int), ('path', _c.c_int),
                 ('mcdisk', _c.c_int), ('cflux', _c.c_float),
                 ('csum_len', _c.c_int), ('hoi', _c.c_void_p),
-                ('seed', _c.c_ulonglong)]
+                ('seed', _c.c_ulonglong), ('nextra', _c.c_int),
+                ('extra', _c.c_void_p)]
 
 
 N.register_signatures({
@@ -186,14 +187,59 @@ class SetPacker:
         return out
 
 
+class ModelPacker:
+    """Parameter vectors of a gmodel with several disk components: each
+    component's SetPacker vector side by side, component i's part starting
+    at bases[i].  Parameter dicts carry the gmodel's merged names ('xpos',
+    'cmp1_xpos', ...; gmodels/_detail.py:320-342); the attributes of the
+    first component's packer (disk, offsets, trait_offsets, trait_ints)
+    are passed through."""
+
+    def __init__(self, gmodel, template_params):
+        self.components = gmodel.components
+        self.maps = gmodel._maps
+        self.packers, self.bases = [], []
+        base = 0
+        for c, m in zip(self.components, self.maps):
+            pk = SetPacker(c.disk, self._sub(template_params, c, m))
+            self.packers.append(pk)
+            self.bases.append(base)
+            base += pk.pstride
+        self.pstride = base
+        first = self.packers[0]
+        self.disk = first.disk
+        self.offsets = first.offsets
+        self.trait_offsets = first.trait_offsets
+        self.trait_ints = first.trait_ints
+
+    @staticmethod
+    def _sub(params, c, m):
+        return {n: params[m[n]] for n in c.pdescs()}
+
+    def pack(self, params_list):
+        return np.concatenate(
+            [pk.pack([self._sub(p, c, m) for p in params_list])
+             for pk, c, m in zip(self.packers, self.components, self.maps)],
+            axis=1)
+
+    def pack_arrays(self, arrays, nsets):
+        return np.concatenate(
+            [pk.pack_arrays(self._sub(arrays, c, m), nsets)
+             for pk, c, m in zip(self.packers, self.components, self.maps)],
+            axis=1)
+
+
 class BatchEvaluator:
     """chi^2 of many parameter sets of one disk model against one cube.
 
     dmodel: DModelSCube (its DCube geometry, PSF and LSF are used);
-    gmodel: GModelKinematics2D/3D with exactly one SpectralSMDisk or
-            SpectralMCDisk component (thick smdisk and mcdisk: direct or
-            padded-cube pipeline from a deposited hi-res cube; mcdisk: each
-            set's cloud budget computed on the host as the reference does,
+    gmodel: GModelKinematics2D/3D whose first component is a SpectralSMDisk
+            or SpectralMCDisk and whose further components (if any) are
+            SpectralSMDisks, summed into one cube as the reference sums
+            components (gmodels/_detail.py:405-422; thick smdisk, mcdisk and
+            several components: direct or padded-cube pipeline from a
+            deposited hi-res cube; mcdisk: each set's cloud budget computed
+            on the host as the reference does,
             src/gbkfit/model/gmodels/_mcdisk.py:16-189);
     obs:    observed low-res cube (numpy, z-y-x); error: scalar or cube;
             mask: cube or None.  With a DModelMMaps data model the objective
@@ -211,10 +257,13 @@ class BatchEvaluator:
         self.driver = driver
         self.mmaps = type(dmodel).__name__ == 'DModelMMaps'
         comps = gmodel.components
-        if len(comps) != 1 or not hasattr(comps[0].disk, 'traits') or \
-                type(comps[0].disk).__name__ not in ('SMDisk', 'MCDisk'):
+        kinds = [type(getattr(c, 'disk', None)).__name__ for c in comps]
+        if kinds[0] not in ('SMDisk', 'MCDisk') or \
+                any(k != 'SMDisk' for k in kinds[1:]) or \
+                not all(hasattr(c.disk, 'traits') for c in comps):
             raise RuntimeError(
-                "BatchEvaluator needs one smdisk or mcdisk component")
+                "BatchEvaluator needs a first smdisk or mcdisk component "
+                "and smdisk components after it")
         if getattr(gmodel, 'ocomponents', ()) or \
                 not hasattr(gmodel, 'evaluate_scube'):
             raise RuntimeError("BatchEvaluator takes spectral gmodels "
@@ -237,8 +286,9 @@ class BatchEvaluator:
                               step_hi[2], zero_hi[2])
         if template_params is None:
             template_params = {n: np.zeros(s) if s > 1 else 0.0
-                               for n, s in disk.pdescs().items()}
-        self.packer = SetPacker(disk, template_params)
+                               for n, s in gmodel.pdescs().items()}
+        self.packer = SetPacker(disk, template_params) if len(comps) == 1 \
+            else ModelPacker(gmodel, template_params)
         t = driver_torch()
         dev = driver.device
         self._keep = []
@@ -265,6 +315,14 @@ class BatchEvaluator:
                 bt.off_cvalues, bt.off_pvalues = self.packer.trait_offsets[k]
             d.traits[i] = bt
         d.pstride = self.packer.pstride
+        if len(comps) > 1:
+            self._extra = (BDesc * (len(comps) - 1))()
+            for j in range(1, len(comps)):
+                self._fill_disk(self._extra[j - 1], comps[j].disk,
+                                self.packer.packers[j],
+                                self.packer.bases[j])
+            d.nextra = len(comps) - 1
+            d.extra = _c.cast(self._extra, _c.c_void_p)
         d.grid = self.grid
         d.size_lo[:] = list(cube.size())
         d.size_hi[:] = list(size_hi)
@@ -338,6 +396,34 @@ class BatchEvaluator:
         self._chi2_dev = None
         self._cubes_dev = None
         del t, dev
+
+    def _fill_disk(self, e, disk, pk, base):
+        """The disk fields of a further component's desc: its sub-ring
+        nodes, traits and offsets (shifted by `base`) in the set vector."""
+        e.loose, e.tilted = int(disk.loose), int(disk.tilted)
+        rn = np.ascontiguousarray(disk.subrnodes, np.float32)
+        self._keep.append(rn)
+        e.nrnodes = rn.size
+        e.rnodes = rn.ctypes.data_as(_c.POINTER(_c.c_float))
+        o = pk.offsets
+        e.vsys_count = o['vsys'][1]
+        e.off_vsys = base + o['vsys'][0]
+        e.off_xpos = base + o['xpos'][0]
+        e.off_ypos = base + o['ypos'][0]
+        e.off_posa = base + o['posa'][0]
+        e.off_incl = base + o['incl'][0]
+        for i, k in enumerate(_SLOTS):
+            bt = BTraits()
+            if k in pk.trait_ints:
+                uids, cc, pc = pk.trait_ints[k]
+                self._keep.extend([uids, cc, pc])
+                bt.n = uids.size
+                bt.uids = uids.ctypes.data_as(_c.POINTER(_c.c_int))
+                bt.ccounts = cc.ctypes.data_as(_c.POINTER(_c.c_int))
+                bt.pcounts = pc.ctypes.data_as(_c.POINTER(_c.c_int))
+                oc, op = pk.trait_offsets[k]
+                bt.off_cvalues, bt.off_pvalues = base + oc, base + op
+            e.traits[i] = bt
 
     def _mmap_data(self, dmodel, obs, error, mask, weight):
         """Stacked (norders, y, x) data, error and mask maps of the

@@ -101,7 +101,7 @@ template <int J>
 __global__ __launch_bounds__(256) void batch_smdisk_thick(
     gbkhip_disk tmpl, SetLayout lay, const float* __restrict__ params,
     int pstride, gbkhip_grid g, float* __restrict__ cubes, long long cstride,
-    int nxe, int nze)
+    int nxe, int nze, int assign)
 {
     __shared__ ThickLds S;
     const int set = blockIdx.z;
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void batch_smdisk_thick(
     gbkhip_outputs o;
     memset(&o, 0, sizeof(o));
     thick_row_segment<J, false>(d, g, ng, need_theta, blockIdx.y, nxe, nze,
-                                cubes + set * cstride, true, o, S);
+                                cubes + set * cstride, assign != 0, o, S);
 }
 
 struct DataArgs {
@@ -1255,6 +1255,15 @@ struct Fused {
     int kx = 1, ky = 1, tiles_x = 1, tiles_y = 1;
 };
 
+// A disk of the batch: the template (device trait arrays and sub-ring
+// nodes) and where each set's parameters of it are in the set vector.
+struct DiskLayout {
+    gbkhip_disk tmpl;
+    SetLayout lay;
+    DevBuf ints, rnodes;
+    std::vector<int> h_uids;
+};
+
 struct Batch {
     Context* ctx;
     int device = 0;
@@ -1290,6 +1299,8 @@ struct Batch {
     DevBuf overflow;        // int: set when a set's clouds exceed mc_cap
     int max_grid_z = 65535; // device limit on gridDim.z (and y)
     unsigned long long work_gen = 0;    // bumped whenever work is allocated
+    // components after the first (smdisk), added into the hi-res cube
+    std::vector<std::unique_ptr<DiskLayout>> extras;
     // caller-owned work arena (gbkhip_batch_bind_work): the work buffers
     // are views into it, or allocations of their own when it is null
     char* arena = nullptr;
@@ -1328,45 +1339,46 @@ struct Batch {
     }
 };
 
-static void build_layout(Batch& b)
+static void build_disk(const gbkhip_batch_desc& D, DiskLayout& L)
 {
-    const gbkhip_batch_desc& D = b.desc;
     const int nn = D.nrnodes;
-    b.lay.vsys = D.vsys_count > 0 ? D.off_vsys : -1;
-    b.lay.xpos = D.off_xpos;
-    b.lay.ypos = D.off_ypos;
-    b.lay.posa = D.off_posa;
-    b.lay.incl = D.off_incl;
-    std::memset(&b.tmpl, 0, sizeof(b.tmpl));
-    b.tmpl.loose = D.loose;
-    b.tmpl.tilted = D.tilted;
-    b.tmpl.nrnodes = nn;
+    GBK_CHECK(nn >= 2 && D.rnodes, GBKHIP_ERR_ARG,
+              "a disk needs at least two rnodes");
+    L.lay.vsys = D.vsys_count > 0 ? D.off_vsys : -1;
+    L.lay.xpos = D.off_xpos;
+    L.lay.ypos = D.off_ypos;
+    L.lay.posa = D.off_posa;
+    L.lay.incl = D.off_incl;
+    std::memset(&L.tmpl, 0, sizeof(L.tmpl));
+    L.tmpl.loose = D.loose;
+    L.tmpl.tilted = D.tilted;
+    L.tmpl.nrnodes = nn;
     int total = 0;
     for (int k = 0; k < 9; ++k) total += 3 * D.traits[k].n;
-    b.h_uids.assign(std::max(total, 1), 0);
+    L.h_uids.assign(std::max(total, 1), 0);
     int pos = 0;
     std::vector<std::pair<int, int>> where(9, {-1, 0});
     for (int k = 0; k < 9; ++k) {
         const gbkhip_batch_traits& t = D.traits[k];
-        b.lay.cvalues[k] = t.n ? t.off_cvalues : -1;
-        b.lay.pvalues[k] = t.n ? t.off_pvalues : -1;
+        L.lay.cvalues[k] = t.n ? t.off_cvalues : -1;
+        L.lay.pvalues[k] = t.n ? t.off_pvalues : -1;
         if (!t.n) continue;
         GBK_CHECK(t.n <= 4, GBKHIP_ERR_ARG, "at most 4 traits per kind");
         where[k] = {pos, t.n};
         for (int i = 0; i < t.n; ++i) {
-            b.h_uids[pos + i] = t.uids[i];
-            b.h_uids[pos + t.n + i] = t.ccounts[i];
-            b.h_uids[pos + 2 * t.n + i] = t.pcounts[i];
+            L.h_uids[pos + i] = t.uids[i];
+            L.h_uids[pos + t.n + i] = t.ccounts[i];
+            L.h_uids[pos + 2 * t.n + i] = t.pcounts[i];
         }
         pos += 3 * t.n;
     }
-    b.ints.alloc(b.h_uids.size() * sizeof(int));
-    GBK_HIP(hipMemcpy(b.ints.ptr, b.h_uids.data(),
-                      b.h_uids.size() * sizeof(int), hipMemcpyHostToDevice));
-    gbkhip_traits* tr[9] = {&b.tmpl.rp, &b.tmpl.rh, &b.tmpl.vp, &b.tmpl.vh,
-                            &b.tmpl.dp, &b.tmpl.dh, &b.tmpl.zp, &b.tmpl.sp,
-                            &b.tmpl.wp};
-    const int* base = b.ints.as<int>();
+    L.ints.alloc(L.h_uids.size() * sizeof(int));
+    GBK_HIP(hipMemcpy(L.ints.ptr, L.h_uids.data(),
+                      L.h_uids.size() * sizeof(int), hipMemcpyHostToDevice));
+    gbkhip_traits* tr[9] = {&L.tmpl.rp, &L.tmpl.rh, &L.tmpl.vp, &L.tmpl.vh,
+                            &L.tmpl.dp, &L.tmpl.dh, &L.tmpl.zp, &L.tmpl.sp,
+                            &L.tmpl.wp};
+    const int* base = L.ints.as<int>();
     for (int k = 0; k < 9; ++k) {
         if (where[k].first < 0) continue;
         const int p0 = where[k].first, n = where[k].second;
@@ -1375,12 +1387,37 @@ static void build_layout(Batch& b)
         tr[k]->ccounts = base + p0 + n;
         tr[k]->pcounts = base + p0 + 2 * n;
     }
-    b.rnodes.alloc(nn * sizeof(float));
-    GBK_HIP(hipMemcpy(b.rnodes.ptr, D.rnodes, nn * sizeof(float),
+    L.rnodes.alloc(nn * sizeof(float));
+    GBK_HIP(hipMemcpy(L.rnodes.ptr, D.rnodes, nn * sizeof(float),
                       hipMemcpyHostToDevice));
-    b.tmpl.rnodes = b.rnodes.as<float>();
-    b.tmpl.opacity = nullptr;
-    b.thin = D.traits[1].n == 0;
+    L.tmpl.rnodes = L.rnodes.as<float>();
+    L.tmpl.opacity = nullptr;
+}
+
+static void build_layout(Batch& b)
+{
+    DiskLayout L;
+    build_disk(b.desc, L);
+    b.tmpl = L.tmpl;
+    b.lay = L.lay;
+    b.h_uids.swap(L.h_uids);
+    std::swap(b.ints.ptr, L.ints.ptr);
+    std::swap(b.ints.bytes, L.ints.bytes);
+    std::swap(b.rnodes.ptr, L.rnodes.ptr);
+    std::swap(b.rnodes.bytes, L.rnodes.bytes);
+    b.thin = b.desc.traits[1].n == 0;
+    // further smdisk components of the same gmodel, summed into the cube
+    // after the first (gmodels summing components, _detail.py:405-422)
+    for (int i = 0; i < b.desc.nextra; ++i) {
+        const gbkhip_batch_desc& E = b.desc.extra[i];
+        GBK_CHECK(!E.mcdisk, GBKHIP_ERR_UNSUPPORTED,
+                  "further components must be smdisk disks");
+        GBK_CHECK((E.traits[1].n == 0) == b.thin || b.desc.mcdisk,
+                  GBKHIP_ERR_ARG,
+                  "components of one gmodel are all thin or all thick");
+        b.extras.emplace_back(new DiskLayout);
+        build_disk(E, *b.extras.back());
+    }
 }
 
 // Work buffers: everything sized by max_batch (per-set spectra or padded
@@ -2008,6 +2045,44 @@ static bool sep_cheaper(const Batch& B)
 // Hi-res model cubes of sets [set0, set0 + nsets) of a chunk on `lane` into
 // hi (n_hi per set): mcdisk (binned deposits when a tile spectrum fits LDS,
 // else int64 atomics into a fixed-point cube) or a thick smdisk.
+// smdisk voxels of one disk into the sets' hi-res cubes (assign: write,
+// zero where no line lands; else add).  The direct pipeline reads only its
+// inputs [0, need) of each axis; the padded-cube pipeline the whole grid.
+static void launch_thick(Batch& b, const gbkhip_disk& tmpl,
+                         const SetLayout& lay, const float* P, int nsets,
+                         float* hi, int assign)
+{
+    Context& c = *b.ctx;
+    const gbkhip_batch_desc& D = b.desc;
+    const gbkhip_grid& g = D.grid;
+    const bool part = b.sep.on && b.sep.cube;
+    const int nxe = part ? b.sep.need[0] : g.spat_size[0];
+    const int nye = part ? b.sep.need[1] : g.spat_size[1];
+    const int nze = part ? b.sep.need[2] : g.spec_size;
+    dim3 grid(ceil_div(nxe, 64), nye, nsets);
+    const int nz = g.spat_size[2];
+    if (nz <= 16)
+        hipLaunchKernelGGL(batch_smdisk_thick<4>, grid, dim3(256), 0,
+                           c.stream, tmpl, lay, P, D.pstride, g, hi, b.n_hi,
+                           nxe, nze, assign);
+    else if (nz <= 32)
+        hipLaunchKernelGGL(batch_smdisk_thick<8>, grid, dim3(256), 0,
+                           c.stream, tmpl, lay, P, D.pstride, g, hi, b.n_hi,
+                           nxe, nze, assign);
+    else
+        hipLaunchKernelGGL(batch_smdisk_thick<16>, grid, dim3(256), 0,
+                           c.stream, tmpl, lay, P, D.pstride, g, hi, b.n_hi,
+                           nxe, nze, assign);
+    GBK_HIP(hipGetLastError());
+}
+
+// The components after the first, added into the cubes in order.
+static void deposit_extras(Batch& b, const float* P, int nsets, float* hi)
+{
+    for (const auto& e : b.extras)
+        launch_thick(b, e->tmpl, e->lay, P, nsets, hi, 0);
+}
+
 static void deposit_hi(Batch& b, const float* P, int set0, int nsets,
                        int lane, float* hi)
 {
@@ -2062,25 +2137,7 @@ static void deposit_hi(Batch& b, const float* P, int set0, int nsets,
         hipLaunchKernelGGL(batch_fixed_to_hi, dim3(1024, nsets), dim3(256), 0,
                            c.stream, icube, hi, b.n_hi, b.qscale + set0);
     } else {
-        // the direct pipeline reads only its inputs [0, need) of each axis
-        const bool part = b.sep.on && b.sep.cube;
-        const int nxe = part ? b.sep.need[0] : g.spat_size[0];
-        const int nye = part ? b.sep.need[1] : g.spat_size[1];
-        const int nze = part ? b.sep.need[2] : g.spec_size;
-        dim3 grid(ceil_div(nxe, 64), nye, nsets);
-        const int nz = g.spat_size[2];
-        if (nz <= 16)
-            hipLaunchKernelGGL(batch_smdisk_thick<4>, grid, dim3(256), 0,
-                               c.stream, b.tmpl, b.lay, P, D.pstride, g, hi,
-                               b.n_hi, nxe, nze);
-        else if (nz <= 32)
-            hipLaunchKernelGGL(batch_smdisk_thick<8>, grid, dim3(256), 0,
-                               c.stream, b.tmpl, b.lay, P, D.pstride, g, hi,
-                               b.n_hi, nxe, nze);
-        else
-            hipLaunchKernelGGL(batch_smdisk_thick<16>, grid, dim3(256), 0,
-                               c.stream, b.tmpl, b.lay, P, D.pstride, g, hi,
-                               b.n_hi, nxe, nze);
+        launch_thick(b, b.tmpl, b.lay, P, nsets, hi, 1);
     }
     GBK_HIP(hipGetLastError());
 }
@@ -2116,6 +2173,7 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
                            (long long)nsets * ny, nullptr);
         GBK_HIP(hipGetLastError());
         deposit_hi(b, par, set0, nsets, lane, hi);
+        deposit_extras(b, par, nsets, hi);
         mark(b, ci, 1);
         dim3 grid(ny, ceil_div(nz, P.zc), nsets);
         const size_t lds = (size_t)P.zc * P.mp * sizeof(float);
@@ -2249,6 +2307,7 @@ static void run_chunk_padded(Batch& b, const float* params, int set0,
                            b.tmpl, b.lay, P, D.pstride, g, hi, b.n_hi);
         GBK_HIP(hipGetLastError());
     }
+    deposit_extras(b, P, nsets, hi);
     mark(b, ci, 1);
     if (D.has_conv)
         fft_convolve_with(c, *b.plan, hi, b.spec.as<float2>(),
@@ -2452,6 +2511,12 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
                 }
             }
             plan_fused(*c, B);
+            if (D.nextra > 0) {
+                // further components: the model is a deposited cube
+                GBK_CHECK(D.path != 2, GBKHIP_ERR_UNSUPPORTED,
+                          "the fused FFT pipeline takes one thin disk");
+                B.fused.on = false;
+            }
             if ((D.path == 0 || D.path == 3) && plan_sep(B) &&
                 (D.path == 3 || sep_cheaper(B))) {
                 B.fused.on = false;
@@ -2459,8 +2524,9 @@ int gbkhip_batch_create(gbkhip_ctx* ctx, const gbkhip_batch_desc* desc,
                                  ceil_div(D.size_lo[1], kSepThreads / 64) *
                                  B.sep.zblocks);
                 B.sep.on = true;
-                // thick disks and mcdisk deposit a hi-res cube first
-                B.sep.cube = D.mcdisk || !B.thin;
+                // thick disks, mcdisk and several components deposit a
+                // hi-res cube first
+                B.sep.cube = D.mcdisk || !B.thin || D.nextra > 0;
                 B.ka_psf.clear();
                 B.ka_lsf.clear();
                 // two lanes overlap one chunk's compute-bound x pass with

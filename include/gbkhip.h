@@ -227,7 +227,7 @@ typedef struct {
     int off_pvalues;         /* offset of this kind's parameters in a set */
 } gbkhip_batch_traits;
 
-typedef struct {
+typedef struct gbkhip_batch_desc_s {
     int loose, tilted;
     int nrnodes;
     const float* rnodes;     /* host, sub-ring nodes */
@@ -263,6 +263,15 @@ typedef struct {
     const unsigned char* hoi;   /* device, per density trait: analytic
                                    integral (one sub-ring) or not */
     unsigned long long seed; /* Philox key of the cloud stream */
+    /* further components of the gmodel (smdisk; all thin or all thick like
+     * the first), summed into the model cube after the first one
+     * (gmodels/_detail.py:405-422): each a desc of which only the disk
+     * fields (loose .. traits) are read, their offsets into the same
+     * parameter vector.  Read at creation only.  With further components
+     * the direct pipeline deposits a hi-res cube and the fused FFT
+     * pipeline is not used. */
+    int nextra;
+    const struct gbkhip_batch_desc_s* extra;
 } gbkhip_batch_desc;
 
 typedef struct gbkhip_batch gbkhip_batch;
