@@ -130,20 +130,37 @@ __device__ __forceinline__ void thick_row_segment(
                 if (!live && !first) continue;
                 if (live) {
                     // B: this lane's column of its wave's partial spectrum
+                    // (read-modify-write of LDS the lane owns, channels two
+                    // at a time so two reads are in flight).
                     float* my = &S.tile[wave][0][lane];
                     for (int k = 0; k < cn; ++k) my[k * 64] = 0.0f;
-                    LineEval e;
-                    e.zero = g.spec_zero;
-                    e.step = g.spec_step;
 #pragma unroll
                     for (int j = 0; j < J; ++j) {
                         const int k0 = max((int)(win[j] & 0xffffu), c0);
                         const int k1 = min((int)(win[j] >> 16), c0 + cn - 1);
-                        e.amp = amp[j];
-                        e.v = vc[j];
-                        e.inv2s2 = i2s[j];
-                        for (int k = k0; k <= k1; ++k)
-                            my[(k - c0) * 64] += e.at(k);
+                        // flux N(zero + k step; v, s): exp(-t) as one
+                        // v_exp_f32 of -t log2(e)
+                        const float zmv = g.spec_zero - vc[j];
+                        const float cc = -i2s[j] * 1.44269504088896341f;
+                        const float a = amp[j];
+                        float* dst = my + (k0 - c0) * 64;
+                        float kf = (float)k0;
+                        int k = k0;
+                        for (; k < k1; k += 2, kf += 2.0f, dst += 128) {
+                            const float m0 = dst[0], m1 = dst[64];
+                            const float d0 = fmaf(kf, g.spec_step, zmv);
+                            const float d1 = fmaf(kf + 1.0f, g.spec_step,
+                                                  zmv);
+                            dst[0] = fmaf(a, __builtin_amdgcn_exp2f(
+                                                 d0 * d0 * cc), m0);
+                            dst[64] = fmaf(a, __builtin_amdgcn_exp2f(
+                                                  d1 * d1 * cc), m1);
+                        }
+                        if (k == k1) {
+                            const float d0 = fmaf(kf, g.spec_step, zmv);
+                            dst[0] = fmaf(a, __builtin_amdgcn_exp2f(
+                                                 d0 * d0 * cc), dst[0]);
+                        }
                     }
                     __syncthreads();
                 }
