@@ -694,15 +694,15 @@ def algorithmic_bytes(ev, info, live=1.0, glive=1.0, seg=1.0):
 
 def algorithmic_flops(ev, info, live=1.0, glive=1.0):
     """FP32 flops per parameter set of the direct pipeline's filter passes:
-    one multiply-add per tap of the block-mean-folded kernel (s + 2 edge
-    taps along each axis), zero-padded taps not counted; the x and y passes
-    only over their live tiles / groups (see algorithmic_bytes)."""
+    one multiply-add per tap an output uses (the block-mean-folded kernel,
+    s + 2 edge taps along each axis, without the negligible ends the
+    pipeline drops), zero-padded taps not counted; the x and y passes only
+    over their live tiles / groups (see algorithmic_bytes)."""
     if info['pipeline'] != 'direct':
         return {}
     nxl, nyl, nzl = ev.size_lo
     nx, ny, nz = info['fft_size']
-    s, e = ev.desc.scale, ev.desc.edge_hi
-    tx, ty, tz = (s[i] + 2 * e[i] for i in range(3))
+    (tx, ty, tz), _ = ev.taps()
     return {'sep_xpass': 2 * nz * ny * nxl * tx * live,
             'sep_ypass': 2 * nz * nyl * nxl * ty * glive,
             'sep_zpass': 2 * nzl * nyl * nxl * tz}

@@ -59,6 +59,7 @@ N.register_signatures({
     'gbkhip_mcdisk_fixed_scale': ([_c.c_float, _c.c_int, _c.c_void_p,
                                    _c.POINTER(_c.c_double)], _c.c_int),
     'gbkhip_batch_info': ([_c.c_void_p, _c.POINTER(_c.c_int)], _c.c_int),
+    'gbkhip_batch_taps': ([_c.c_void_p, _c.POINTER(_c.c_int)], _c.c_int),
     'gbkhip_batch_set_tiles': ([_c.c_void_p, _c.c_int, _c.c_int], _c.c_int),
     'gbkhip_batch_reserve_clouds': ([_c.c_void_p, _c.c_longlong], _c.c_int),
     'gbkhip_batch_check': ([_c.c_void_p, _c.POINTER(_c.c_int)], _c.c_int),
@@ -615,6 +616,13 @@ class BatchEvaluator:
         pipe = ('padded', 'fused', 'direct')[v[0]]
         return dict(pipeline=pipe, fused=pipe == 'fused',
                     fft_size=(v[1], v[2], v[3]), nparts=v[4])
+
+    def taps(self):
+        """Direct pipeline: (taps per output along x, y, z; inputs skipped
+        before the first tap along x, y, z)."""
+        v = (_c.c_int * 6)()
+        N.call('gbkhip_batch_taps', self.handle, v)
+        return tuple(v[:3]), tuple(v[3:])
 
     def set_tiles(self, lanes=2, ysplit=1):
         """Direct pipeline tiles: chunks in flight (1 or 2) and the split of

@@ -801,9 +801,10 @@ constexpr int kSepXLines = 2;
 template <int SX, int TX>
 __global__ __launch_bounds__(256) void sep_xpass(
     const float4* __restrict__ lines, long long lstride, int lpitch,
-    gbkhip_grid g, const float* __restrict__ wx, int need_x, int need_z,
-    int nxl, int mp, int zc, float* __restrict__ P1, long long p1stride,
-    int ny1, const unsigned long long* __restrict__ rowmask,
+    gbkhip_grid g, const float* __restrict__ wx, int ox, int need_x,
+    int need_z, int nxl, int mp, int zc, float* __restrict__ P1,
+    long long p1stride, int ny1,
+    const unsigned long long* __restrict__ rowmask,
     const unsigned long long* __restrict__ segmask)
 {
     extern __shared__ float4 M4[];        // [zc][mp] floats
@@ -859,7 +860,7 @@ __global__ __launch_bounds__(256) void sep_xpass(
             if (L.x != 0.0f) deposit(L, x);
         }
         __syncthreads();
-        sep_xfilter<SX, TX>(M, wx, mp, zn, z0, nxl, ny1, out);
+        sep_xfilter<SX, TX>(M + ox, wx, mp, zn, z0, nxl, ny1, out);
         __syncthreads();            // the tile is free for the next chunk
     }
 }
@@ -874,8 +875,8 @@ __global__ __launch_bounds__(256) void sep_xpass(
 template <int SX, int TX>
 __global__ __launch_bounds__(256) void sep_xpass_cube(
     const float* __restrict__ hi, long long hstride, int hnx, int hny,
-    const float* __restrict__ wx, int need_x, int need_z, int nxl, int mp,
-    int zc, float* __restrict__ P1, long long p1stride, int ny1,
+    const float* __restrict__ wx, int ox, int need_x, int need_z, int nxl,
+    int mp, int zc, float* __restrict__ P1, long long p1stride, int ny1,
     unsigned long long* __restrict__ rowmask)
 {
     extern __shared__ float4 M4[];        // [zc][mp] floats
@@ -910,7 +911,7 @@ __global__ __launch_bounds__(256) void sep_xpass_cube(
     if (!__syncthreads_or(live)) return;
     if (threadIdx.x == 0)
         atomicOr(&rowmask[(size_t)set * ny1 + y], 1ull << blockIdx.y);
-    sep_xfilter<SX, TX>(M, s_w, mp, zn, z0, nxl, ny1,
+    sep_xfilter<SX, TX>(M + ox, s_w, mp, zn, z0, nxl, ny1,
                         P1 + set * p1stride + (size_t)y * nxl);
 }
 
@@ -928,10 +929,10 @@ __global__ __launch_bounds__(256) void sep_xpass_cube(
 template <int SY, int TY>
 __global__ __launch_bounds__(256, 4) void sep_ypass(
     const float* __restrict__ P1, long long p1stride,
-    const float* __restrict__ wy, int nxl, int nyl, int ny1, int need_z,
-    int yblk, float* __restrict__ P2, long long p2stride,
+    const float* __restrict__ wy, int oy, int nxl, int nyl, int ny1,
+    int need_z, int yblk, float* __restrict__ P2, long long p2stride,
     const unsigned long long* __restrict__ rowmask, int zc, int zpb,
-    unsigned char* __restrict__ gflag, int ngy)
+    unsigned char* __restrict__ gflag, int ngy, const float4* __restrict__ zq)
 {
     constexpr int T = kSepThreads, J = sep_jy(TY), XW = kSepXW;
     constexpr int PITCH = kSepYPitch;
@@ -944,6 +945,7 @@ __global__ __launch_bounds__(256, 4) void sep_ypass(
     __shared__ unsigned char s_gl[T];     // live groups, in order
     __shared__ int s_wcnt[T / 64];
     const int X0 = blockIdx.x * XW, Y0 = blockIdx.y * yblk;
+    const int R0 = SY * Y0 + oy;          // input row of staged row 0
     // blockIdx.z = (set, chunk, sub-block of zpb channels of the chunk)
     const int nsub = (zc + zpb - 1) / zpb;
     const int nzb = (need_z + zc - 1) / zc * nsub;
@@ -959,7 +961,7 @@ __global__ __launch_bounds__(256, 4) void sep_ypass(
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int r0 = 0; r0 < nrows; r0 += T) {          // uniform trip count
         const int r = r0 + threadIdx.x;
-        const int y = SY * Y0 + r;
+        const int y = R0 + r;
         const bool f = r < nrows && y < ny1 && ((rm[y] >> chunk) & 1ull);
         if (r < nrows) s_flag[r] = f;
         const unsigned long long b = __ballot(f);
@@ -998,30 +1000,45 @@ __global__ __launch_bounds__(256, 4) void sep_ypass(
     const int rhi = SY * J * s_gl[nl - 1] + NIN;
     constexpr int JH = J / 2, NH = SY * (JH - 1) + TY;
     const int c = lane & (XW - 1), h = lane / XW, X = X0 + c;
+    // staging: U independent loads in flight per thread before the stores
+    constexpr int U = 8;
+    constexpr int Q = XW / 4;                 // float4s per staged row
+    const bool quads = (nxl & 3) == 0 && X0 + XW <= nxl;
+    const int n4 = (rhi - rlo) * Q;
+    // the rows do not change from channel to channel: this thread's
+    // elements e (element i = tid + e T) as bits, staged at all (vb) and
+    // read from P1 (lb: a written row; the others read a zero quad, so
+    // every load issues unconditionally and no flag is read per channel)
+    static_assert(kSepMaxRows * Q <= 32 * T, "one bit per staged element");
+    unsigned vb = 0u, lb = 0u;
+    if (quads) {
+        for (int e = 0; e < 32 && threadIdx.x + e * T < n4; ++e) {
+            const int r = rlo + (threadIdx.x + e * T) / Q;
+            vb |= 1u << e;
+            lb |= (s_flag[r] ? 1u : 0u) << e;
+        }
+    }
     for (int z = z0; z < z1; ++z) {
         const float* src = P1 + set * p1stride + (size_t)z * ny1 * nxl;
-        // staging: U independent loads in flight per thread before the
-        // stores
-        constexpr int U = 8;
-        if ((nxl & 3) == 0 && X0 + XW <= nxl) {
+        if (quads) {
             // whole strip, rows 16-byte aligned: float4 loads
-            constexpr int Q = XW / 4;
-            const int n4 = (rhi - rlo) * Q;
-            for (int i0 = threadIdx.x; i0 < n4; i0 += T * U) {
+            for (int e0 = 0; e0 * T < n4; e0 += U) {      // uniform
                 float4 t[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const int i = i0 + u * T, r = rlo + i / Q, q = i % Q;
-                    t[u] = (i < n4 && s_flag[r])
-                               ? *reinterpret_cast<const float4*>(
-                                     src + (size_t)(SY * Y0 + r) * nxl + X0 +
-                                     4 * q)
-                               : make_float4(0, 0, 0, 0);
+                    const int e = e0 + u, i = threadIdx.x + e * T;
+                    const int r = rlo + i / Q, q = i % Q;
+                    const float4* p = ((lb >> e) & 1u)
+                        ? reinterpret_cast<const float4*>(
+                              src + (size_t)(R0 + r) * nxl + X0 + 4 * q)
+                        : zq;
+                    t[u] = *p;
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const int i = i0 + u * T, r = rlo + i / Q, q = i % Q;
-                    if (i < n4) {
+                    const int e = e0 + u, i = threadIdx.x + e * T;
+                    const int r = rlo + i / Q, q = i % Q;
+                    if ((vb >> e) & 1u) {
                         float* d = tile + r * PITCH + 4 * q;
                         d[0] = t[u].x;
                         d[1] = t[u].y;
@@ -1039,7 +1056,7 @@ __global__ __launch_bounds__(256, 4) void sep_ypass(
                     const int i = i0 + u * T, r = rlo + i / XW, c = i % XW;
                     const int X = X0 + c;
                     t[u] = (i < n1 && s_flag[r] && X < nxl)
-                               ? src[(size_t)(SY * Y0 + r) * nxl + X] : 0.0f;
+                               ? src[(size_t)(R0 + r) * nxl + X] : 0.0f;
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
@@ -1216,6 +1233,8 @@ struct Sep {
     bool cube = false;            // S1 reads a deposited hi-res cube
     int need[3] = {0, 0, 0};      // inputs that reach the outputs
     int tx = 0, ty = 0, tz = 0;   // tap buckets (zero-padded)
+    int ox = 0, oy = 0;           // inputs before an output's first tap
+    int kept[3] = {0, 0, 0};      // taps an output uses, per axis
     int mp = 0;                   // S1 LDS row length (odd)
     int zc = 0;                   // channels per S1 block (tile <= 64 KB)
     int yblk = 0;                 // Y outputs per S2 block
@@ -1894,11 +1913,11 @@ static int tap_bucket(int n)
 // s X + u through the kernel offset t = d + e - u (c(q) = sum_t k(t)
 // m(q - t), k wrapped so k(t) = k[t mod N]):
 //   W[u] = (1/s) sum_{d < s, |d + e - u| <= e} k[(d + e - u) mod N] / norm,
-// u in [0, s + 2e), zero-padded to the bucket.
-static std::vector<float> block_taps(const std::vector<double>& k, int N,
-                                     int e, int s, double norm, int bucket)
+// u in [0, s + 2e).
+static std::vector<double> block_taps(const std::vector<double>& k, int N,
+                                      int e, int s, double norm)
 {
-    std::vector<float> w(bucket, 0.0f);
+    std::vector<double> w(s + 2 * e, 0.0);
     for (int u = 0; u < s + 2 * e; ++u) {
         double acc = 0.0;
         for (int d = 0; d < s; ++d) {
@@ -1906,9 +1925,42 @@ static std::vector<float> block_taps(const std::vector<double>& k, int N,
             if (t < -e || t > e) continue;
             acc += k[(t % N + N) % N] / norm;
         }
-        w[u] = (float)(acc / s);
+        w[u] = acc / s;
     }
     return w;
+}
+
+// The taps an output needs: W without its negligible ends.  Leading and
+// trailing taps are dropped while each end's sum of |W| stays within 2^-31
+// of the total, so the dropped terms of any output add up to at most 2^-30
+// of the largest input in its window: 2^6 below FP32's unit roundoff, far
+// below the rounding of the FP32 sum itself and of an FFT convolution
+// (whose error is spread over the whole cube at ~1e-7 of its maximum).  A
+// Gaussian cut at 8 sigma keeps about +-5.7 sigma.  Returns the taps
+// zero-padded to their bucket and the count dropped in front (the offset
+// of an output's first tap); bucket 0 when none fits.
+static std::vector<float> trimmed_taps(const std::vector<double>& w,
+                                       int& bucket, int& lead, int& kept)
+{
+    const int n = (int)w.size();
+    double tot = 0.0;
+    for (double v : w) tot += std::fabs(v);
+    const double cut = std::ldexp(tot, -31);
+    int a = 0, b = n;
+    for (double acc = 0.0; a < n - 1; ++a) {
+        acc += std::fabs(w[a]);
+        if (acc > cut) break;
+    }
+    for (double acc = 0.0; b > a + 1; --b) {
+        acc += std::fabs(w[b - 1]);
+        if (acc > cut) break;
+    }
+    lead = a;
+    kept = b - a;
+    bucket = tap_bucket(b - a);
+    std::vector<float> out(std::max(bucket, 0), 0.0f);
+    for (int u = a; u < b && bucket; ++u) out[u - a] = (float)w[u];
+    return out;
 }
 
 // Direct separable pipeline: needs the kernel analysis of plan_fused and a
@@ -1943,16 +1995,23 @@ static bool plan_sep(Batch& B)
     const int sx = D.scale[0], sy = D.scale[1], sz = D.scale[2];
     if (sx < 1 || sx > 3 || sy < 1 || sy > 3 || sz < 1 || sz > 3) return false;
     if (D.grid.spec_size > 65535) return false;    // 16-bit channel windows
-    P.tx = tap_bucket(sx + 2 * ex);
-    P.ty = tap_bucket(sy + 2 * ey);
+    // spatial taps lose their negligible ends (the x and y passes read
+    // from offsets ox, oy); the spectral ones are kept whole
+    const std::vector<float> wx =
+        trimmed_taps(block_taps(px, Nx, ex, sx, ptot), P.tx, P.ox, P.kept[0]);
+    const std::vector<float> wy =
+        trimmed_taps(block_taps(py, Ny, ey, sy, ptot), P.ty, P.oy, P.kept[1]);
     P.tz = tap_bucket(sz + 2 * ez);
+    P.kept[2] = sz + 2 * ez;
     if (!P.tx || !P.ty || !P.tz) return false;
+    std::vector<float> wz(P.tz, 0.0f);
+    {
+        const std::vector<double> w =
+            block_taps(B.ka_lsf, Nz, ez, sz, B.ka_total);
+        for (size_t u = 0; u < w.size(); ++u) wz[u] = (float)w[u];
+    }
     for (int i = 0; i < 3; ++i)
         P.need[i] = D.size_lo[i] * D.scale[i] + 2 * D.edge_hi[i];
-    const std::vector<float> wx = block_taps(px, Nx, ex, sx, ptot, P.tx);
-    const std::vector<float> wy = block_taps(py, Ny, ey, sy, ptot, P.ty);
-    const std::vector<float> wz = block_taps(B.ka_lsf, Nz, ez, sz,
-                                             B.ka_total, P.tz);
     auto up = [](DevBuf& b, const std::vector<float>& h) {
         b.alloc(h.size() * sizeof(float));
         GBK_HIP(hipMemcpy(b.ptr, h.data(), h.size() * sizeof(float),
@@ -1964,7 +2023,8 @@ static bool plan_sep(Batch& B)
     const int nxl = D.size_lo[0], nyl = D.size_lo[1], nzl = D.size_lo[2];
     const int jx = sep_jx(P.tx), jy = sep_jy(P.ty);
     const int nxb = (nxl + jx - 1) / jx;
-    P.mp = (sx * nxb * jx + P.tx) | 1;
+    // the tile holds every deposited column and every filter input
+    P.mp = std::max(sx * nxb * jx + P.ox + P.tx, P.need[0]) | 1;
     // channels per S1 block: at most 16 (the lanes of a group) in a tile of
     // at most 64 KB, spread evenly over the chunks
     const int zmax = std::min(kSepZC, (int)((64 * 1024) / (P.mp * 4)));
@@ -2179,8 +2239,8 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
         const size_t lds = (size_t)P.zc * P.mp * sizeof(float);
         GBK_SEP_DISPATCH(sep_xpass_cube, D.scale[0], P.tx, grid,
                          dim3(kSepThreads), lds, c.stream, hi, b.n_hi,
-                         D.size_hi[0], D.size_hi[1], P.wx.as<float>(), nx,
-                         nz, nxl, P.mp, P.zc, P1, P.p1stride, ny, masks);
+                         D.size_hi[0], D.size_hi[1], P.wx.as<float>(), P.ox,
+                         nx, nz, nxl, P.mp, P.zc, P1, P.p1stride, ny, masks);
         GBK_HIP(hipGetLastError());
     } else {
         unsigned long long* segs = P.segmask.as<unsigned long long>() +
@@ -2199,7 +2259,7 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
         const size_t lds = (size_t)P.zc * P.mp * sizeof(float);
         GBK_SEP_DISPATCH(sep_xpass, D.scale[0], P.tx, grid, dim3(kSepThreads),
                          lds, c.stream, lines, lstride, nx, D.grid,
-                         P.wx.as<float>(), nx, nz, nxl, P.mp, P.zc,
+                         P.wx.as<float>(), P.ox, nx, nz, nxl, P.mp, P.zc,
                          P1, P.p1stride, ny, masks, segs);
         GBK_HIP(hipGetLastError());
     }
@@ -2221,10 +2281,11 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
         const size_t lds = (size_t)rows * kSepYPitch * sizeof(float);
         GBK_SEP_DISPATCH(sep_ypass, D.scale[1], P.ty, grid, dim3(kSepThreads),
                          lds, c.stream, P1 + s0 * P.p1stride, P.p1stride,
-                         P.wy.as<float>(), nxl, nyl, ny, nz, P.yblk,
+                         P.wy.as<float>(), P.oy, nxl, nyl, ny, nz, P.yblk,
                          P2 + s0 * P.p2stride, P.p2stride,
                          masks + (size_t)s0 * ny, P.zc, zpb,
-                         gflags + (size_t)s0 * nz * P.ngy, P.ngy);
+                         gflags + (size_t)s0 * nz * P.ngy, P.ngy,
+                         P.zrow.as<float4>());
         GBK_HIP(hipGetLastError());
     });
     mark(b, ci, 3);
@@ -3087,6 +3148,20 @@ int gbkhip_batch_info(gbkhip_batch* b, int* info)
         info[2] = P.on ? P.need[1] : F.on ? F.ny : B.desc.size_hi[1];
         info[3] = P.on ? P.need[2] : F.on ? F.nz : B.desc.size_hi[2];
         info[4] = B.nparts;
+    });
+}
+
+int gbkhip_batch_taps(gbkhip_batch* b, int* taps)
+{
+    return guarded([&] {
+        GBK_CHECK(b && taps, GBKHIP_ERR_ARG, "null argument");
+        const Sep& P = b->impl.sep;
+        GBK_CHECK(P.on, GBKHIP_ERR_UNSUPPORTED,
+                  "taps apply to the direct pipeline");
+        for (int i = 0; i < 3; ++i) taps[i] = P.kept[i];
+        taps[3] = P.ox;
+        taps[4] = P.oy;
+        taps[5] = 0;
     });
 }
 
