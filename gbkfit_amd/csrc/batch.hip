@@ -944,6 +944,7 @@ __global__ __launch_bounds__(256, 4) void sep_ypass(
     __shared__ unsigned long long s_bits[(kSepMaxRows + 255) / 256 * 4];
     __shared__ unsigned char s_gl[T];     // live groups, in order
     __shared__ int s_wcnt[T / 64];
+    __shared__ float4 s_junk[T / 64];     // staging stores past the rows
     const int X0 = blockIdx.x * XW, Y0 = blockIdx.y * yblk;
     const int R0 = SY * Y0 + oy;          // input row of staged row 0
     // blockIdx.z = (set, chunk, sub-block of zpb channels of the chunk)
@@ -1003,48 +1004,51 @@ __global__ __launch_bounds__(256, 4) void sep_ypass(
     // staging: U independent loads in flight per thread before the stores
     constexpr int U = 8;
     constexpr int Q = XW / 4;                 // float4s per staged row
+    constexpr int RS = T / Q;                 // rows between a thread's quads
+    static_assert(T % Q == 0, "a thread keeps one column quad");
     const bool quads = (nxl & 3) == 0 && X0 + XW <= nxl;
-    const int n4 = (rhi - rlo) * Q;
-    // the rows do not change from channel to channel: this thread's
-    // elements e (element i = tid + e T) as bits, staged at all (vb) and
-    // read from P1 (lb: a written row; the others read a zero quad, so
-    // every load issues unconditionally and no flag is read per channel)
-    static_assert(kSepMaxRows * Q <= 32 * T, "one bit per staged element");
-    unsigned vb = 0u, lb = 0u;
-    if (quads) {
-        for (int e = 0; e < 32 && threadIdx.x + e * T < n4; ++e) {
-            const int r = rlo + (threadIdx.x + e * T) / Q;
-            vb |= 1u << e;
-            lb |= (s_flag[r] ? 1u : 0u) << e;
-        }
-    }
+    // this thread stages quad q4 of rows rt + e RS, e < ne: its addresses
+    // are a base plus a uniform stride.  The rows do not change from
+    // channel to channel: bit e of lb says whether row rt + e RS was
+    // written (the others read a zero quad, so every load issues
+    // unconditionally and no flag is read per channel)
+    const int q4 = threadIdx.x % Q, rt = rlo + threadIdx.x / Q;
+    const int ne = rt < rhi ? (rhi - rt + RS - 1) / RS : 0;
+    const int nemax = (rhi - rlo + RS - 1) / RS;          // uniform
+    static_assert((kSepMaxRows + RS - 1) / RS + U <= 32, "bits of lb");
+    unsigned lb = 0u;
+    if (quads)
+        for (int e = 0; e < ne; ++e)
+            lb |= (s_flag[rt + e * RS] ? 1u : 0u) << e;
+    float* const lt = tile + rt * PITCH + 4 * q4;
+    const int rstep = RS * nxl;               // floats between a thread's rows
     for (int z = z0; z < z1; ++z) {
         const float* src = P1 + set * p1stride + (size_t)z * ny1 * nxl;
         if (quads) {
             // whole strip, rows 16-byte aligned: float4 loads
-            for (int e0 = 0; e0 * T < n4; e0 += U) {      // uniform
+            const float* st = src + (size_t)(R0 + min(rt, rhi - 1)) * nxl +
+                              X0 + 4 * q4;
+            for (int e0 = 0; e0 < nemax; e0 += U) {
                 float4 t[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const int e = e0 + u, i = threadIdx.x + e * T;
-                    const int r = rlo + i / Q, q = i % Q;
+                    const int e = e0 + u;
                     const float4* p = ((lb >> e) & 1u)
-                        ? reinterpret_cast<const float4*>(
-                              src + (size_t)(R0 + r) * nxl + X0 + 4 * q)
+                        ? reinterpret_cast<const float4*>(st + e * rstep)
                         : zq;
                     t[u] = *p;
                 }
+                // every store issues (no branch the loads could sink
+                // into): quads past this thread's rows go to a junk slot
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const int e = e0 + u, i = threadIdx.x + e * T;
-                    const int r = rlo + i / Q, q = i % Q;
-                    if ((vb >> e) & 1u) {
-                        float* d = tile + r * PITCH + 4 * q;
-                        d[0] = t[u].x;
-                        d[1] = t[u].y;
-                        d[2] = t[u].z;
-                        d[3] = t[u].w;
-                    }
+                    const int e = e0 + u;
+                    float* d = e < ne ? lt + e * (RS * PITCH)
+                                      : reinterpret_cast<float*>(s_junk + wv);
+                    d[0] = t[u].x;
+                    d[1] = t[u].y;
+                    d[2] = t[u].z;
+                    d[3] = t[u].w;
                 }
             }
         } else {

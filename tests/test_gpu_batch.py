@@ -426,3 +426,38 @@ def test_nodewise_traits_match_plain_path():
         assert abs(plain - chi2[i]) <= 1e-5 * chi2[i], (i, plain, chi2[i])
         assert H.normwise(cubes[i], model.model_h(s)['scube']['d']) < 1e-5
     ev.close()
+
+
+def test_direct_pipeline_taps(setup):
+    """The taps the direct pipeline uses on the 128x128x96 workload (PSF
+    FWHM 3 px at 2x supersampling): the block-mean-folded x and y kernels,
+    built here from the oracle's PSF image, lose exactly the leading and
+    trailing taps whose |sum| stays within 2^-31 of the total (5 per end of
+    42: 32 kept); the spectral taps are kept whole."""
+    drv, ev, sets, mock, _ = setup
+    assert ev.info()['pipeline'] == 'direct'
+    (tx, ty, tz), (ox, oy, oz) = ev.taps()
+    p = sets[0]
+    geom = P.oracle_geom(p)
+    s = p['scale'][0]
+    sig = p['psf_fwhm'] * P.SIG_FWHM
+    psz = oracle.psf_gauss_size(sig, geom.step_hi[:2])
+    k = oracle.psf_gauss_array(sig, geom.step_hi[:2], psz).sum(axis=0)
+    e = len(k) // 2
+    assert geom.edge_hi[0] == e
+    w = np.zeros(s + 2 * e)
+    for u in range(s + 2 * e):
+        for d in range(s):
+            t = d + e - u
+            if -e <= t <= e:
+                w[u] += k[t + e] / s
+    tot = np.abs(w).sum()
+    lead = np.cumsum(np.abs(w))
+    trail = np.cumsum(np.abs(w[::-1]))
+    a = int(np.sum(lead <= 2.0 ** -31 * tot))
+    b = int(np.sum(trail <= 2.0 ** -31 * tot))
+    print('taps', (tx, ty, tz), 'offsets', (ox, oy, oz), 'of', len(w), a, b)
+    assert (a, b) == (5, 5)
+    assert (tx, ty) == (len(w) - a - b,) * 2 and (ox, oy) == (a, a)
+    ez = geom.edge_hi[2]
+    assert tz == p['scale'][2] + 2 * ez and oz == 0
