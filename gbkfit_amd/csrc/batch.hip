@@ -1114,7 +1114,7 @@ __global__ __launch_bounds__(256, 4) void sep_ypass(
 template <int SZ, int TZ>
 __global__ __launch_bounds__(256) void sep_zpass(
     const float* __restrict__ P2, long long p2stride, const float* wz,
-    int nxl, int nyl, int nzl, int need_z, int zblocks, DataArgs da,
+    int oz, int nxl, int nyl, int nzl, int need_z, int zblocks, DataArgs da,
     float* __restrict__ lo_out, long long lstride,
     double* __restrict__ partials, int nparts,
     const unsigned char* __restrict__ gflag, int ngy, int jy,
@@ -1133,13 +1133,13 @@ __global__ __launch_bounds__(256) void sep_zpass(
     const int Y = blockIdx.y * (T / TXB) + wv;          // uniform per wave
     const int Z0 = zb * J;
     for (int i = threadIdx.x; i < TZ; i += T) s_w[i] = wz[i];
-    // which of the wave's input rows (channel SZ Z0 + i of row Y) the y pass
-    // wrote: the others are zero
+    // which of the wave's input rows (channel SZ Z0 + oz + i of row Y) the
+    // y pass wrote: the others are zero
     unsigned long long live[NW];
     unsigned long long any = 0ull;
 #pragma unroll
     for (int k = 0; k < NW; ++k) {
-        const int i = 64 * k + lane, z = SZ * Z0 + i;
+        const int i = 64 * k + lane, z = SZ * Z0 + oz + i;
         const bool f = i < NIN && z < need_z && Y < nyl &&
             gflag[((size_t)set * need_z + z) * ngy + Y / jy] != 0;
         live[k] = __ballot(f);
@@ -1170,7 +1170,7 @@ __global__ __launch_bounds__(256) void sep_zpass(
 #pragma unroll
         for (int i = 0; i < NIN; ++i) {
             const float* src = ((live[i >> 6] >> (i & 63)) & 1ull)
-                ? row + (size_t)(SZ * Z0 + i) * plane : zrow;
+                ? row + (size_t)(SZ * Z0 + oz + i) * plane : zrow;
             v[i] = src[xo];
         }
         float o[J] = {};
@@ -1237,7 +1237,7 @@ struct Sep {
     bool cube = false;            // S1 reads a deposited hi-res cube
     int need[3] = {0, 0, 0};      // inputs that reach the outputs
     int tx = 0, ty = 0, tz = 0;   // tap buckets (zero-padded)
-    int ox = 0, oy = 0;           // inputs before an output's first tap
+    int ox = 0, oy = 0, oz = 0;   // inputs before an output's first tap
     int kept[3] = {0, 0, 0};      // taps an output uses, per axis
     int mp = 0;                   // S1 LDS row length (odd)
     int zc = 0;                   // channels per S1 block (tile <= 64 KB)
@@ -1908,7 +1908,7 @@ static void run_chunk_fused(Batch& b, const float* params, int set0,
 
 static int tap_bucket(int n)
 {
-    for (int b : {8, 16, 24, 32, 44, 48, 64, 96, 128})
+    for (int b : {8, 12, 16, 24, 32, 44, 48, 64, 96, 128})
         if (n <= b) return b;
     return 0;
 }
@@ -1999,21 +1999,16 @@ static bool plan_sep(Batch& B)
     const int sx = D.scale[0], sy = D.scale[1], sz = D.scale[2];
     if (sx < 1 || sx > 3 || sy < 1 || sy > 3 || sz < 1 || sz > 3) return false;
     if (D.grid.spec_size > 65535) return false;    // 16-bit channel windows
-    // spatial taps lose their negligible ends (the x and y passes read
-    // from offsets ox, oy); the spectral ones are kept whole
+    // the taps lose their negligible ends (the passes read from offsets
+    // ox, oy, oz)
     const std::vector<float> wx =
         trimmed_taps(block_taps(px, Nx, ex, sx, ptot), P.tx, P.ox, P.kept[0]);
     const std::vector<float> wy =
         trimmed_taps(block_taps(py, Ny, ey, sy, ptot), P.ty, P.oy, P.kept[1]);
-    P.tz = tap_bucket(sz + 2 * ez);
-    P.kept[2] = sz + 2 * ez;
+    const std::vector<float> wz =
+        trimmed_taps(block_taps(B.ka_lsf, Nz, ez, sz, B.ka_total), P.tz,
+                     P.oz, P.kept[2]);
     if (!P.tx || !P.ty || !P.tz) return false;
-    std::vector<float> wz(P.tz, 0.0f);
-    {
-        const std::vector<double> w =
-            block_taps(B.ka_lsf, Nz, ez, sz, B.ka_total);
-        for (size_t u = 0; u < w.size(); ++u) wz[u] = (float)w[u];
-    }
     for (int i = 0; i < 3; ++i)
         P.need[i] = D.size_lo[i] * D.scale[i] + 2 * D.edge_hi[i];
     auto up = [](DevBuf& b, const std::vector<float>& h) {
@@ -2076,6 +2071,7 @@ static bool sep_cheaper(const Batch& B)
 #define GBK_SEP_DISPATCH(KERNEL, S, TAPS, ...)                               \
     switch ((S) * 256 + (TAPS)) {                                            \
     GBK_SEP_CASE(KERNEL, 1, 8, 0, __VA_ARGS__)                               \
+    GBK_SEP_CASE(KERNEL, 1, 12, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 1, 16, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 1, 24, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 1, 32, 0, __VA_ARGS__)                              \
@@ -2085,6 +2081,7 @@ static bool sep_cheaper(const Batch& B)
     GBK_SEP_CASE(KERNEL, 1, 96, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 1, 128, 0, __VA_ARGS__)                             \
     GBK_SEP_CASE(KERNEL, 2, 8, 0, __VA_ARGS__)                               \
+    GBK_SEP_CASE(KERNEL, 2, 12, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 2, 16, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 2, 24, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 2, 32, 0, __VA_ARGS__)                              \
@@ -2094,6 +2091,7 @@ static bool sep_cheaper(const Batch& B)
     GBK_SEP_CASE(KERNEL, 2, 96, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 2, 128, 0, __VA_ARGS__)                             \
     GBK_SEP_CASE(KERNEL, 3, 8, 0, __VA_ARGS__)                               \
+    GBK_SEP_CASE(KERNEL, 3, 12, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 3, 16, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 3, 24, 0, __VA_ARGS__)                              \
     GBK_SEP_CASE(KERNEL, 3, 32, 0, __VA_ARGS__)                              \
@@ -2305,7 +2303,8 @@ static void run_chunk_sep(Batch& b, const float* params, int set0,
                   P.zblocks * ns);
         GBK_SEP_DISPATCH(sep_zpass, D.scale[2], P.tz, grid, dim3(kSepThreads),
                          0, c.stream, P2 + s0 * P.p2stride, P.p2stride,
-                         P.wz.as<float>(), nxl, nyl, nzl, nz, P.zblocks, da,
+                         P.wz.as<float>(), P.oz, nxl, nyl, nzl, nz,
+                         P.zblocks, da,
                          cubes ? cubes + (size_t)s0 * b.n_lo : nullptr,
                          b.n_lo, partials + (size_t)s0 * b.nparts, b.nparts,
                          gflags + (size_t)s0 * nz * P.ngy, P.ngy,
@@ -2346,8 +2345,8 @@ static void compute_zero_partials(Batch& b)
     dim3 grid(ceil_div(nxl, 64), ceil_div(nyl, kSepThreads / 64), P.zblocks);
     GBK_SEP_DISPATCH(sep_zpass, D.scale[2], P.tz, grid, dim3(kSepThreads), 0,
                      c.stream, P.P2.as<float>(), P.p2stride,
-                     P.wz.as<float>(), nxl, nyl, nzl, nz, P.zblocks, da,
-                     nullptr, b.n_lo, P.zero_part.as<double>(), b.nparts,
+                     P.wz.as<float>(), P.oz, nxl, nyl, nzl, nz, P.zblocks,
+                     da, nullptr, b.n_lo, P.zero_part.as<double>(), b.nparts,
                      dead.as<unsigned char>(), P.ngy, sep_jy(P.ty), nullptr,
                      P.zrow.as<float>());
     GBK_HIP(hipGetLastError());
@@ -3087,7 +3086,7 @@ int gbkhip_batch_tile_stats(gbkhip_batch* b, long long* counts)
                         const int Y = by * (kSepThreads / 64) + yy;
                         if (Y >= nyl) break;
                         for (int k = 0; k < nin && !live; ++k) {
-                            const int z = sz * zb * kSepJZ + k;
+                            const int z = sz * zb * kSepJZ + B.sep.oz + k;
                             live = z < nz &&
                                    f[((size_t)st * nz + z) * B.sep.ngy +
                                      Y / jy] != 0;
@@ -3165,7 +3164,7 @@ int gbkhip_batch_taps(gbkhip_batch* b, int* taps)
         for (int i = 0; i < 3; ++i) taps[i] = P.kept[i];
         taps[3] = P.ox;
         taps[4] = P.oy;
-        taps[5] = 0;
+        taps[5] = P.oz;
     });
 }
 

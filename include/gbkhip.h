@@ -408,9 +408,8 @@ int gbkhip_batch_memory(gbkhip_batch* batch, unsigned long long* resident,
 int gbkhip_batch_info(gbkhip_batch* batch, int* info);
 /* Direct pipeline: taps[0..2] = the filter taps an output uses along x, y
  * and z (the block-mean-folded kernel without its negligible ends: each
- * end's dropped |taps| sum to at most 2^-31 of the total; the spectral
- * taps are kept whole), taps[3..5] = the inputs skipped before an output's
- * first tap.  No reference counterpart (the reference convolves with the
+ * end's dropped |taps| sum to at most 2^-31 of the total), taps[3..5] = the
+ * inputs skipped before an output's first tap.  No reference counterpart (the reference convolves with the
  * whole kernel by FFT, host/include/gbkfit/host/fft.hpp:199-207). */
 int gbkhip_batch_taps(gbkhip_batch* batch, int* taps);
 int gbkhip_batch_set_timing(gbkhip_batch* batch, int enable);

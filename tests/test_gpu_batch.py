@@ -428,23 +428,11 @@ def test_nodewise_traits_match_plain_path():
     ev.close()
 
 
-def test_direct_pipeline_taps(setup):
-    """The taps the direct pipeline uses on the 128x128x96 workload (PSF
-    FWHM 3 px at 2x supersampling): the block-mean-folded x and y kernels,
-    built here from the oracle's PSF image, lose exactly the leading and
-    trailing taps whose |sum| stays within 2^-31 of the total (5 per end of
-    42: 32 kept); the spectral taps are kept whole."""
-    drv, ev, sets, mock, _ = setup
-    assert ev.info()['pipeline'] == 'direct'
-    (tx, ty, tz), (ox, oy, oz) = ev.taps()
-    p = sets[0]
-    geom = P.oracle_geom(p)
-    s = p['scale'][0]
-    sig = p['psf_fwhm'] * P.SIG_FWHM
-    psz = oracle.psf_gauss_size(sig, geom.step_hi[:2])
-    k = oracle.psf_gauss_array(sig, geom.step_hi[:2], psz).sum(axis=0)
+def _trim(k, s):
+    """Block-mean-folded taps of the kernel k (centred, 2e + 1 values) at
+    downscale s, and the counts of leading and trailing taps whose |sum|
+    stays within 2^-31 of the total."""
     e = len(k) // 2
-    assert geom.edge_hi[0] == e
     w = np.zeros(s + 2 * e)
     for u in range(s + 2 * e):
         for d in range(s):
@@ -452,12 +440,34 @@ def test_direct_pipeline_taps(setup):
             if -e <= t <= e:
                 w[u] += k[t + e] / s
     tot = np.abs(w).sum()
-    lead = np.cumsum(np.abs(w))
-    trail = np.cumsum(np.abs(w[::-1]))
-    a = int(np.sum(lead <= 2.0 ** -31 * tot))
-    b = int(np.sum(trail <= 2.0 ** -31 * tot))
-    print('taps', (tx, ty, tz), 'offsets', (ox, oy, oz), 'of', len(w), a, b)
-    assert (a, b) == (5, 5)
-    assert (tx, ty) == (len(w) - a - b,) * 2 and (ox, oy) == (a, a)
-    ez = geom.edge_hi[2]
-    assert tz == p['scale'][2] + 2 * ez and oz == 0
+    a = int(np.sum(np.cumsum(np.abs(w)) <= 2.0 ** -31 * tot))
+    b = int(np.sum(np.cumsum(np.abs(w[::-1])) <= 2.0 ** -31 * tot))
+    return len(w), a, b
+
+
+def test_direct_pipeline_taps(setup):
+    """The taps the direct pipeline uses on the 128x128x96 workload (PSF
+    FWHM 3 px at 2x supersampling, LSF FWHM 2 channels): the
+    block-mean-folded kernels, built here from the oracle's PSF image and
+    LSF, lose exactly the leading and trailing taps whose |sum| stays within
+    2^-31 of the total (x, y: 5 per end of 42, 32 kept; z: 2 per end of 15,
+    11 kept)."""
+    drv, ev, sets, mock, _ = setup
+    assert ev.info()['pipeline'] == 'direct'
+    (tx, ty, tz), (ox, oy, oz) = ev.taps()
+    p = sets[0]
+    geom = P.oracle_geom(p)
+    sig = p['psf_fwhm'] * P.SIG_FWHM
+    psz = oracle.psf_gauss_size(sig, geom.step_hi[:2])
+    kx = oracle.psf_gauss_array(sig, geom.step_hi[:2], psz).sum(axis=0)
+    assert geom.edge_hi[0] == len(kx) // 2
+    n, a, b = _trim(kx, p['scale'][0])
+    sgz = p['lsf_fwhm'] * P.SIG_FWHM
+    kz = oracle.lsf_gauss_array(sgz, geom.step_hi[2],
+                                oracle.lsf_gauss_size(sgz, geom.step_hi[2]))
+    assert geom.edge_hi[2] == len(kz) // 2
+    nz, az, bz = _trim(kz, p['scale'][2])
+    print('taps', (tx, ty, tz), 'offsets', (ox, oy, oz), 'of', (n, nz))
+    assert (n, a, b) == (42, 5, 5) and (nz, az, bz) == (15, 2, 2)
+    assert (tx, ty, tz) == (n - a - b, n - a - b, nz - az - bz)
+    assert (ox, oy, oz) == (a, a, az)
