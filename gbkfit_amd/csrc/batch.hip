@@ -1111,6 +1111,19 @@ __global__ __launch_bounds__(256, 4) void sep_ypass(
 // the loads go out back to back with a scalar base and a lane offset.  The
 // residual divides by the error through its reciprocal (v_rcp_f32, within
 // an ulp of the quotient).
+// base[byte_off / 4] for a wave-uniform base through a raw buffer load:
+// the base goes into a buffer resource (scalar registers, full 4 GiB
+// range, so no load is clipped) and the lane's byte offset is the only
+// vector operand.
+__device__ __forceinline__ float load_lane(const float* base,
+                                           unsigned byte_off)
+{
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float*>(base), 0, (int)0xffffffffu, 0x00020000);
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)byte_off, 0, 0));
+}
+
 template <int SZ, int TZ>
 __global__ __launch_bounds__(256) void sep_zpass(
     const float* __restrict__ P2, long long p2stride, const float* wz,
@@ -1166,12 +1179,15 @@ __global__ __launch_bounds__(256) void sep_zpass(
         // lanes past the last column read the row's last value (in bounds)
         const unsigned xo = min(lane, nxl - 1 - X0);
         const float* row = P2 + set * p2stride + (size_t)Y * nxl + X0;
+        // wave-uniform row bases (a buffer resource in SGPRs) plus the
+        // lane's byte offset: no per-lane address arithmetic per load
+        const unsigned xb = xo * (unsigned)sizeof(float);
         float v[NIN];
 #pragma unroll
         for (int i = 0; i < NIN; ++i) {
             const float* src = ((live[i >> 6] >> (i & 63)) & 1ull)
                 ? row + (size_t)(SZ * Z0 + oz + i) * plane : zrow;
-            v[i] = src[xo];
+            v[i] = load_lane(src, xb);
         }
         float o[J] = {};
 #pragma unroll
@@ -1199,25 +1215,49 @@ __global__ __launch_bounds__(256) void sep_zpass(
 #pragma unroll
                 for (int j = 0; j < J; ++j) {
                     zo[j] = (size_t)min(Z0 + j, nzl - 1) * plane + col0;
-                    ob[j] = (da.obs + zo[j])[xo];
+                    ob[j] = load_lane(da.obs + zo[j], xb);
                 }
                 if (da.err) {
 #pragma unroll
-                    for (int j = 0; j < J; ++j) ie[j] = (da.err + zo[j])[xo];
+                    for (int j = 0; j < J; ++j)
+                        ie[j] = load_lane(da.err + zo[j], xb);
                 }
                 if (da.mask) {
 #pragma unroll
-                    for (int j = 0; j < J; ++j) mk[j] = (da.mask + zo[j])[xo];
+                    for (int j = 0; j < J; ++j)
+                        mk[j] = load_lane(da.mask + zo[j], xb);
                 }
+                // objective.hpp:7-33 with mdl_w = mdl_m = none; the
+                // uniform choices are branches around whole loops (the same
+                // products in the same order per output; a weight of 1 is
+                // an exact no-op and skipped)
+                float r[J];
 #pragma unroll
-                for (int j = 0; j < J; ++j) {
-                    // objective.hpp:7-33 with mdl_w = mdl_m = none
-                    float r = o[j] - ob[j];
-                    r *= da.err ? __builtin_amdgcn_rcpf(ie[j])
-                                : da.inv_err_scalar;
-                    if (da.mask) r *= mk[j];
-                    r *= da.weight;
-                    if (j < jn) acc += (double)r * (double)r;
+                for (int j = 0; j < J; ++j) r[j] = o[j] - ob[j];
+                if (da.err) {
+#pragma unroll
+                    for (int j = 0; j < J; ++j)
+                        r[j] *= __builtin_amdgcn_rcpf(ie[j]);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < J; ++j) r[j] *= da.inv_err_scalar;
+                }
+                if (da.mask) {
+#pragma unroll
+                    for (int j = 0; j < J; ++j) r[j] *= mk[j];
+                }
+                if (da.weight != 1.0f) {
+#pragma unroll
+                    for (int j = 0; j < J; ++j) r[j] *= da.weight;
+                }
+                if (jn == J) {
+#pragma unroll
+                    for (int j = 0; j < J; ++j)
+                        acc += (double)r[j] * (double)r[j];
+                } else {
+#pragma unroll
+                    for (int j = 0; j < J; ++j)
+                        if (j < jn) acc += (double)r[j] * (double)r[j];
                 }
             }
         }
