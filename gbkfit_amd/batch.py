@@ -102,12 +102,12 @@ _SLOTS = ('rp', 'rh', 'vp', 'vh', 'dp', 'dh', 'zp', 'sp', 'wp')
 def _interp_matrix(nodes, sub):
     """W with W @ values == np.interp(sub, nodes, values)."""
     W = np.zeros((sub.size, nodes.size))
-    for i, x in enumerate(sub):
-        j = int(np.clip(np.searchsorted(nodes, x, side='right') - 1, 0,
-                        nodes.size - 2))
-        t = (x - nodes[j]) / (nodes[j + 1] - nodes[j])
-        W[i, j] = 1 - t
-        W[i, j + 1] = t
+    j = np.clip(np.searchsorted(nodes, sub, side='right') - 1, 0,
+                nodes.size - 2)
+    t = (sub - nodes[j]) / (nodes[j + 1] - nodes[j])
+    rows = np.arange(sub.size)
+    W[rows, j] = 1 - t
+    W[rows, j + 1] = t
     return W
 
 
@@ -119,6 +119,7 @@ class SetPacker:
     def __init__(self, disk, template_params):
         packed = disk.pack(template_params)
         self.disk = disk
+        self._W = None
         off = 0
         self.offsets = {}
         for k in ('vsys', 'xpos', 'ypos', 'posa', 'incl'):
@@ -143,7 +144,9 @@ class SetPacker:
         ones).  Node-wise values are interpolated with one matrix product."""
         d = self.disk
         out = np.zeros((nsets, self.pstride), np.float32)
-        W = _interp_matrix(d.rnodes, d.subrnodes)
+        if self._W is None:                   # the nodes are fixed per disk
+            self._W = _interp_matrix(d.rnodes, d.subrnodes)
+        W = self._W
 
         def col(name, nw):
             v = np.asarray(arrays[name], np.float64).reshape(nsets, -1)
