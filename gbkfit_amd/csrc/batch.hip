@@ -843,12 +843,7 @@ __global__ __launch_bounds__(256) void sep_xpass(
             const unsigned kw = __float_as_uint(L.w);
             const int k0 = max((int)(kw & 0xffffu), z0);
             const int k1 = min((int)(kw >> 16), z0 + zn - 1);
-            LineEval e;
-            e.amp = L.x;
-            e.v = L.y;
-            e.inv2s2 = L.z;
-            e.zero = g.spec_zero;
-            e.step = g.spec_step;
+            const LineEval e = line_from(g, L.x, L.y, L.z);
             for (int k = k0; k <= k1; ++k) M[(k - z0) * mp + x] = e.at(k);
         };
 #pragma unroll

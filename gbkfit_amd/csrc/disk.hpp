@@ -611,26 +611,37 @@ __device__ __forceinline__ void line_window(const gbkhip_grid& g, float v,
 // flux * N(zero + k*step; v, s): the spectral line value of channel k
 // (gauss_1d_pdf, math.hpp:282-287; no channel-width factor, gmodels.hpp:113).
 struct LineEval {
-    float amp, v, inv2s2, zero, step;
+    float amp, v, inv2s2;    // flux / (s sqrt(2 pi)), centre, 1 / (2 s^2)
+    float zmv, cc, step;     // spectral zero - v, -log2(e) / (2 s^2), step
     __device__ __forceinline__ float at(int k) const
     {
-        const float dz = zero + k * step - v;
-        // exp(-t) = 2^(-t * log2 e), single v_exp_f32
-        return amp * __builtin_amdgcn_exp2f(-(dz * dz) * inv2s2 *
-                                            1.44269504088896341f);
+        // amp exp(-(zero + k step - v)^2 / (2 s^2)) = amp 2^((k step + zmv)^2
+        // cc), one v_exp_f32 per channel
+        const float dz = fmaf((float)k, step, zmv);
+        return amp * __builtin_amdgcn_exp2f(dz * dz * cc);
     }
 };
+
+// A line from its amplitude, centre and 1 / (2 s^2) on the spectral grid.
+__device__ __forceinline__ LineEval line_from(const gbkhip_grid& g,
+                                              float amp, float v,
+                                              float inv2s2)
+{
+    LineEval e;
+    e.amp = amp;
+    e.v = v;
+    e.inv2s2 = inv2s2;
+    e.zmv = g.spec_zero - v;
+    e.cc = -inv2s2 * 1.44269504088896341f;
+    e.step = g.spec_step;
+    return e;
+}
 
 __device__ __forceinline__ LineEval line_eval(const gbkhip_grid& g,
                                               float flux, float v, float s)
 {
-    LineEval e;
-    e.amp = flux / (s * 2.50662827463100050f);
-    e.v = v;
-    e.inv2s2 = 1.0f / (2 * s * s);
-    e.zero = g.spec_zero;
-    e.step = g.spec_step;
-    return e;
+    return line_from(g, flux / (s * 2.50662827463100050f), v,
+                     1.0f / (2 * s * s));
 }
 
 }  // namespace gbkhip
