@@ -33,6 +33,12 @@ thick deposit into a hi-res cube, then the direct pipeline's filters; its
 kernel times, roofline and the deposit's op count (W = 60 V + 12 sum n_ch
 on the 3-D grid) go into the `thick_variant` object.
 
+An FFT leg (skip with --no-fft) times the thin workload with an
+elliptical, rotated PSF (axis ratio 0.7, PA 30 deg), which does not factor,
+on the fused FFT pipeline and on the padded-cube one (256 sets per step):
+the `fft_pipelines` object, with per-kernel times and each pipeline's
+roofline.
+
 A serving leg (skip with --no-serve) replays a Poisson stream of
 single-set requests (default 1000/s for 60 s) over 64 resident galaxies with
 cube sizes log-uniform from 64x64x32 to 256x256x128, half smdisk, half mcdisk
@@ -97,7 +103,7 @@ def param_dicts(arrays, n):
     return [{k: float(v[i]) for k, v in arrays.items()} for i in range(n)]
 
 
-def build_model(driver, thick=False):
+def build_model(driver, thick=False, psf=None):
     from gbkfit_amd import gmodels, psflsf
     from gbkfit_amd.dmodels import DModelSCube
     from gbkfit_amd.objective import Model
@@ -115,7 +121,7 @@ def build_model(driver, thick=False):
             make('vpt', 'tan_arctan'), make('dpt', 'uniform'), rnodes=rn)
         gm = gmodels.GModelKinematics2D([cmp])
     dm = DModelSCube((128, 128, 96), (1, 1, 10), scale=(2, 2, 1),
-                     psf=psflsf.PSFGauss(3.0 * SIG),
+                     psf=psf if psf is not None else psflsf.PSFGauss(3.0 * SIG),
                      lsf=psflsf.LSFGauss(20.0 * SIG))
     return Model(driver, dm, gm)
 
@@ -727,7 +733,8 @@ def measured_traffic(kernel):
     return None
 
 
-def batched_leg(drv, args, thick, nsets, steps, warmup, want_check):
+def batched_leg(drv, args, thick, nsets, steps, warmup, want_check,
+                psf=None, path='auto', chunk=None):
     """The batched workload (a thin or thick smdisk on the 128x128x96 cube):
     `warmup` untimed steps, `steps` timed steps of `nsets` sets (graph
     replays), then the same steps launched directly with HIP events around
@@ -737,7 +744,7 @@ def batched_leg(drv, args, thick, nsets, steps, warmup, want_check):
     errors (computed in the CPU leg)."""
     import torch
     from gbkfit_amd.batch import BatchEvaluator
-    model = build_model(drv, thick)
+    model = build_model(drv, thick, psf)
     arrays = workload(nsets, thick=thick)
     # mock data: the model of the first set plus seeded noise
     truth = param_dicts(arrays, 1)[0]
@@ -746,7 +753,8 @@ def batched_leg(drv, args, thick, nsets, steps, warmup, want_check):
     mock = (cube0 + rng.normal(0, 0.01, cube0.shape)).astype(np.float32)
 
     ev = BatchEvaluator(drv, model.dmodel(), model.gmodel(), mock, error=0.01,
-                        max_batch=args.chunk, template_params=truth)
+                        max_batch=chunk or args.chunk, template_params=truth,
+                        path=path)
     if args.tiles and not thick:
         ev.set_tiles(*(int(v) for v in args.tiles.split(',')))
     use_graph = not args.no_graph
@@ -948,6 +956,8 @@ def main():
     ap.add_argument('--ragged-reps', type=int, default=5)
     ap.add_argument('--no-ragged', action='store_true')
     ap.add_argument('--no-mcdisk', action='store_true')
+    ap.add_argument('--no-fft', action='store_true',
+                    help='skip the fused / padded FFT pipeline legs')
     ap.add_argument('--tiles', default=None,
                     help='direct pipeline tiles "lanes,ysplit" (default: '
                          'the evaluator\'s, 2,1)')
@@ -1045,6 +1055,32 @@ def main():
         print(f"thick leg done: {tv['value']:.0f} evals/s", file=sys.stderr,
               flush=True)
 
+    # the FFT pipelines, which take a PSF that does not factor (an elliptical,
+    # rotated Gaussian): the same thin workload on each, 256 sets per step
+    fft_pipelines = None
+    if not args.thick and not args.no_fft:
+        from gbkfit_amd import psflsf
+        fft_pipelines = {}
+        for path in ('fused', 'padded'):
+            fp = batched_leg(drv, args, False, 256, 2, 1, False,
+                             psf=psflsf.PSFGauss(3.0 * SIG, 0.7, 30.0),
+                             path=path, chunk=64)
+            assert fp['info']['pipeline'] == path
+            fft_pipelines[path] = dict(
+                value=fp['value'], unit='evals/s', steps=2,
+                ms_per_step=1e3 * fp['t_batched'] / 2,
+                fft_size=list(fp['info']['fft_size']),
+                kernels=fp['kernels'],
+                roofline=dict(fp['roofline'],
+                              sets_per_launch=fp['per_chunk']))
+            print(f"{path} FFT leg done: {fp['value']:.0f} evals/s",
+                  file=sys.stderr, flush=True)
+        fft_pipelines['workload'] = (
+            'smdisk thin 128x128x96, 2x spatial supersampling, elliptical '
+            'rotated PSF (FWHM 3 px, axis ratio 0.7, PA 30 deg: does not '
+            'factor) + LSF FWHM 2 ch, chi^2 vs seeded mock; 256 parameter '
+            'sets per step in chunks of 64')
+
     serving = None
     if not args.no_serve:
         serving = serve_benchmark(drv, args.serve_galaxies,
@@ -1106,6 +1142,7 @@ def main():
         smdisk_ops=head['smdisk'],
         roofline=dict(head['roofline'], sets_per_launch=head['per_chunk']),
         thick_variant=thick_variant,
+        fft_pipelines=fft_pipelines,
         serving=serving,
         ragged=ragged,
         mcdisk_plain=mcdisk,
