@@ -1064,7 +1064,7 @@ def main():
         for path in ('fused', 'padded'):
             fp = batched_leg(drv, args, False, 256, 2, 1, False,
                              psf=psflsf.PSFGauss(3.0 * SIG, 0.7, 30.0),
-                             path=path, chunk=64)
+                             path=path)
             assert fp['info']['pipeline'] == path
             fft_pipelines[path] = dict(
                 value=fp['value'], unit='evals/s', steps=2,
@@ -1079,7 +1079,7 @@ def main():
             'smdisk thin 128x128x96, 2x spatial supersampling, elliptical '
             'rotated PSF (FWHM 3 px, axis ratio 0.7, PA 30 deg: does not '
             'factor) + LSF FWHM 2 ch, chi^2 vs seeded mock; 256 parameter '
-            'sets per step in chunks of 64')
+            'sets per step, one launch')
 
     serving = None
     if not args.no_serve:
