@@ -25,8 +25,9 @@ def test_measured_traffic_per_launch():
     # every kernel of the direct pipeline has a traffic line, positive and
     # parsed as bytes per launch; template arguments after the name are
     # ignored, unknown kernels give None
-    for k in ('batch_lines_thin', 'sep_xpass<2, 44>', 'sep_ypass<2, 44>',
-              'sep_zpass<1, 16>'):
+    for k in ('batch_lines_thin', 'sep_xpass<2, 32>', 'sep_ypass<2, 32>',
+              'sep_zpass<1, 12>', 'fx_rows_conv<3, 256, 300>',
+              'fft_convolution', 'model_deposit', 'mcdisk_deposit'):
         t = bench.measured_traffic(k)
         assert t is not None and t > 0, k
     assert bench.measured_traffic('no_such_kernel') is None
@@ -35,6 +36,7 @@ def test_measured_traffic_per_launch():
 def test_traffic_lines_well_formed():
     d = os.path.join(ROOT, 'profiles')
     seen = 0
+    names = set()
     for fn in sorted(os.listdir(d)):
         if not fn.endswith('_traffic.txt'):
             continue
@@ -45,5 +47,9 @@ def test_traffic_lines_well_formed():
                         ':', 1)
                     assert ' ' not in name.strip()
                     float(val)
+                    # one summary per kernel: the lookup takes the first
+                    # file's line, so a second one would be silently stale
+                    assert name.strip() not in names, (fn, name)
+                    names.add(name.strip())
                     seen += 1
     assert seen > 0
