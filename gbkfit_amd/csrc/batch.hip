@@ -1106,17 +1106,22 @@ __global__ __launch_bounds__(256, 4) void sep_ypass(
 // the loads go out back to back with a scalar base and a lane offset.  The
 // residual divides by the error through its reciprocal (v_rcp_f32, within
 // an ulp of the quotient).
-// base[byte_off / 4] for a wave-uniform base through a raw buffer load:
-// the base goes into a buffer resource (scalar registers, full 4 GiB
-// range, so no load is clipped) and the lane's byte offset is the only
-// vector operand.
-__device__ __forceinline__ float load_lane(const float* base,
-                                           unsigned byte_off)
+// Raw buffer loads from a wave-uniform base: the base goes into a buffer
+// resource (scalar registers, full 4 GiB range, so no load is clipped), a
+// wave-uniform byte offset into the instruction's scalar offset and the
+// lane's byte offset is the only vector operand.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t lane_rsrc(const float* base)
 {
-    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<float*>(base), 0, (int)0xffffffffu, 0x00020000);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), 0,
+                                             (int)0xffffffffu, 0x00020000);
+}
+
+__device__ __forceinline__ float load_at(__amdgpu_buffer_rsrc_t r,
+                                         unsigned lane_off, unsigned uni_off)
+{
     return __builtin_bit_cast(
-        float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)byte_off, 0, 0));
+        float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)lane_off,
+                                                    (int)uni_off, 0));
 }
 
 template <int SZ, int TZ>
@@ -1174,15 +1179,19 @@ __global__ __launch_bounds__(256) void sep_zpass(
         // lanes past the last column read the row's last value (in bounds)
         const unsigned xo = min(lane, nxl - 1 - X0);
         const float* row = P2 + set * p2stride + (size_t)Y * nxl + X0;
-        // wave-uniform row bases (a buffer resource in SGPRs) plus the
-        // lane's byte offset: no per-lane address arithmetic per load
+        // two resources (the column's P2 slab and the zero row) and the
+        // channel's byte offset in the scalar operand: per load only a
+        // scalar select and add (the set's P2 fits 32-bit offsets: the
+        // host checks)
         const unsigned xb = xo * (unsigned)sizeof(float);
+        const unsigned pb = (unsigned)plane * (unsigned)sizeof(float);
+        const __amdgpu_buffer_rsrc_t rp = lane_rsrc(row), rz = lane_rsrc(zrow);
+        const unsigned zb0 = (unsigned)(SZ * Z0 + oz) * pb;
         float v[NIN];
 #pragma unroll
         for (int i = 0; i < NIN; ++i) {
-            const float* src = ((live[i >> 6] >> (i & 63)) & 1ull)
-                ? row + (size_t)(SZ * Z0 + oz + i) * plane : zrow;
-            v[i] = load_lane(src, xb);
+            const bool lv = (live[i >> 6] >> (i & 63)) & 1ull;
+            v[i] = load_at(lv ? rp : rz, xb, lv ? zb0 + (unsigned)i * pb : 0u);
         }
         float o[J] = {};
 #pragma unroll
@@ -1205,22 +1214,23 @@ __global__ __launch_bounds__(256) void sep_zpass(
             if (da.obs) {
                 // every data load in flight before the first use (channels
                 // past the cube read the last one and are not summed)
-                size_t zo[J];
+                unsigned zo[J];
                 float ob[J], ie[J], mk[J];
+                const __amdgpu_buffer_rsrc_t ro = lane_rsrc(da.obs + col0);
 #pragma unroll
                 for (int j = 0; j < J; ++j) {
-                    zo[j] = (size_t)min(Z0 + j, nzl - 1) * plane + col0;
-                    ob[j] = load_lane(da.obs + zo[j], xb);
+                    zo[j] = (unsigned)min(Z0 + j, nzl - 1) * pb;
+                    ob[j] = load_at(ro, xb, zo[j]);
                 }
                 if (da.err) {
+                    const __amdgpu_buffer_rsrc_t re = lane_rsrc(da.err + col0);
 #pragma unroll
-                    for (int j = 0; j < J; ++j)
-                        ie[j] = load_lane(da.err + zo[j], xb);
+                    for (int j = 0; j < J; ++j) ie[j] = load_at(re, xb, zo[j]);
                 }
                 if (da.mask) {
+                    const __amdgpu_buffer_rsrc_t rm = lane_rsrc(da.mask + col0);
 #pragma unroll
-                    for (int j = 0; j < J; ++j)
-                        mk[j] = load_lane(da.mask + zo[j], xb);
+                    for (int j = 0; j < J; ++j) mk[j] = load_at(rm, xb, zo[j]);
                 }
                 // objective.hpp:7-33 with mdl_w = mdl_m = none; the
                 // uniform choices are branches around whole loops (the same
@@ -2076,6 +2086,11 @@ static bool plan_sep(Batch& B)
     P.ngy = (nyl + jy - 1) / jy;
     P.p1stride = (long long)P.need[2] * P.need[1] * nxl;
     P.p2stride = (long long)P.need[2] * nyl * nxl;
+    // the z pass addresses a set's P2 and the data cube with 32-bit byte
+    // offsets
+    if ((double)P.p2stride * 4 >= 4294967295.0 ||
+        (double)nzl * nyl * nxl * 4 >= 4294967295.0)
+        return false;
     P.zblocks = (nzl + kSepJZ - 1) / kSepJZ;
     return true;
 }
