@@ -9,9 +9,10 @@ SIG_FWHM = 1 / np.sqrt(8 * np.log(2))
 
 
 def c1_params():
-    """Plain smdisk, 64x64x64: 20 nodes at 1.5 px, incl 45,
-    PA 30, exponential (1, 8 px), arctan (vmax 200, rt 6), sigma 20 km/s,
-    channel 10 km/s, PSF FWHM 2.5 px, LSF FWHM 2 channels, 2x spatial."""
+    """Plain smdisk, 64x64x64 (the CPU-runnable case): 20 nodes at 1.5 px,
+    incl 45, PA 30, exponential (1, 8 px), arctan (vmax 200, rt 6), sigma
+    20 km/s, channel 10 km/s, PSF FWHM 2.5 px, LSF FWHM 2 channels, 2x
+    spatial."""
     return dict(size=(64, 64, 64), step=(1, 1, 10), scale=(2, 2, 1),
                 psf_fwhm=2.5, lsf_fwhm=20.0, rnodes=np.arange(0, 30, 1.5),
                 posa=30.0, incl=45.0, vt=200.0, rt=6.0, disp=20.0,
@@ -19,8 +20,8 @@ def c1_params():
 
 
 def c3_params(rng=None, n=1):
-    """Batched smdisk, 128x128x96: per set incl U(20,80),
-    PA U(0,360), vmax U(100,300), rt U(2,20), sigma U(5,50)."""
+    """Batched smdisk, 128x128x96 (the batched workload): per set incl
+    U(20,80), PA U(0,360), vmax U(100,300), rt U(2,20), sigma U(5,50)."""
     rng = rng or np.random.default_rng(0)
     base = dict(size=(128, 128, 96), step=(1, 1, 10), scale=(2, 2, 1),
                 psf_fwhm=3.0, lsf_fwhm=20.0, rnodes=np.arange(0, 30, 1.5),
